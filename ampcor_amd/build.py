@@ -1,0 +1,76 @@
+"""Build recipe of the native code: libampcor_hip.so (hipcc, gfx950) and the
+CPython module ampcor_hip (g++), both in-tree next to this file.
+
+    python -m ampcor_amd.build
+
+Flags: -ffp-contract=off keeps every rounding where the source puts it (the
+FP32 coarse stage is bit-reproducible on the CPU); packed FP32 instructions
+are disabled (-packed-fp32-ops).
+"""
+
+import os
+import shutil
+import subprocess
+import sys
+import sysconfig
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+INCLUDE = os.path.join(ROOT, "include")
+ARCH = os.environ.get("AMPCOR_HIP_ARCH", "gfx950")
+
+HIP_SOURCES = ["kernels.hip", "plan.hip"]
+HIPCC_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def _hipcc():
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def _run(cmd):
+    proc = subprocess.run(cmd, cwd=ROOT, capture_output=True, text=True)
+    noise = "is not a recognized feature for this target"
+    err = "\n".join(l for l in proc.stderr.splitlines() if noise not in l and "packed-fp32-ops" not in l)
+    if proc.returncode != 0:
+        raise RuntimeError(f"command failed ({proc.returncode}): {' '.join(cmd)}\n{err}\n{proc.stdout}")
+    return err
+
+
+def library_path():
+    return os.path.join(HERE, "libampcor_hip.so")
+
+
+def module_path():
+    return os.path.join(HERE, "ampcor_hip" + sysconfig.get_config_var("EXT_SUFFIX"))
+
+
+def build(verbose=False):
+    lib = library_path()
+    cmd = [_hipcc()] + HIPCC_FLAGS + [f"-I{INCLUDE}"] + [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    cmd += ["-o", lib + ".tmp"]
+    out = _run(cmd)
+    os.replace(lib + ".tmp", lib)
+    if verbose and out:
+        print(out, file=sys.stderr)
+    mod = module_path()
+    py_inc = sysconfig.get_paths()["include"]
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", f"-I{py_inc}", f"-I{INCLUDE}",
+           os.path.join(CSRC, "ext_ampcor_hip.cc"), f"-L{HERE}", "-lampcor_hip",
+           "-Wl,-rpath,$ORIGIN", "-o", mod + ".tmp"]
+    out = _run(cmd)
+    os.replace(mod + ".tmp", mod)
+    if verbose and out:
+        print(out, file=sys.stderr)
+    return lib, mod
+
+
+if __name__ == "__main__":
+    for path in build(verbose=True):
+        print(path)

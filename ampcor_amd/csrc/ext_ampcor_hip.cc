@@ -1,0 +1,453 @@
+// ext_ampcor_hip.cc -- the CPython module "ampcor_hip", a thin layer over the C ABI.
+//
+// Same entry points, argument formats and results as the CUDA module
+// (ext/ampcor_cuda/ampcor_cuda.cc:23-44, ext/ampcor_cuda/sequential.cc):
+//   sequential(pairs, (refLines, refSamples), (tgtLines, tgtSamples),
+//              refineMargin, refineFactor, zoomFactor[, precision, device]) -> worker
+//   addReference(worker, raster, idx, (bLine, bSample), (eLine, eSample))  -> None
+//   addTarget(worker, raster, idx, (bLine, bSample), (eLine, eSample))     -> None
+//   adjust(worker) -> tuple of (dLine, dSample), one per pair
+// plus the raster constructors that replace the host SLC capsule of the CPU
+// module (ext/ampcor/slc.cc:50-140): slc_map(filename, samples, lines),
+// raster(buffer, lines, samples[, dtype, device]), slc_pixelSize(); and the
+// batched and product accessors: addPairs, products, dims, stage_gamma,
+// stage_gamma_refined, stage_locations, set_timing, stage_times.
+//
+// Unlike the reference, adjust releases the GIL, and failures raise
+// ampcor_hip.Error instead of escaping as C++ exceptions.
+#define PY_SSIZE_T_CLEAN
+#include <Python.h>
+
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ampcor_hip.h"
+
+namespace {
+
+const char *const kWorkerCapsule = "ampcor::hip::sequential::capsule_t";
+const char *const kRasterCapsule = "ampcor::hip::raster::capsule_t";
+
+PyObject *g_error = nullptr;
+
+struct Worker {
+    ampcor_hip_plan *plan;
+    PyObject *rasters;  // list: keeps every raster the plan points into alive
+    size_t pairs;
+};
+
+PyObject *raise(int status)
+{
+    PyErr_Format(status == AMPCOR_HIP_EINVAL ? PyExc_ValueError : g_error, "%s (status %d)",
+                 ampcor_hip_last_error(), status);
+    return nullptr;
+}
+
+void free_worker(PyObject *capsule)
+{
+    if (!PyCapsule_IsValid(capsule, kWorkerCapsule)) return;
+    auto *w = static_cast<Worker *>(PyCapsule_GetPointer(capsule, kWorkerCapsule));
+    ampcor_hip_plan_destroy(w->plan);
+    Py_XDECREF(w->rasters);
+    delete w;
+}
+
+void free_raster(PyObject *capsule)
+{
+    if (!PyCapsule_IsValid(capsule, kRasterCapsule)) return;
+    ampcor_hip_raster_destroy(static_cast<ampcor_hip_raster *>(PyCapsule_GetPointer(capsule, kRasterCapsule)));
+}
+
+Worker *get_worker(PyObject *o)
+{
+    if (!PyCapsule_IsValid(o, kWorkerCapsule)) {
+        PyErr_SetString(PyExc_TypeError, "invalid HIP Sequential worker capsule");
+        return nullptr;
+    }
+    return static_cast<Worker *>(PyCapsule_GetPointer(o, kWorkerCapsule));
+}
+
+ampcor_hip_raster *get_raster(PyObject *o)
+{
+    if (!PyCapsule_IsValid(o, kRasterCapsule)) {
+        PyErr_SetString(PyExc_TypeError, "invalid raster capsule");
+        return nullptr;
+    }
+    return static_cast<ampcor_hip_raster *>(PyCapsule_GetPointer(o, kRasterCapsule));
+}
+
+int parse_dtype(const char *name, int *dtype)
+{
+    if (!strcmp(name, "complex64")) *dtype = AMPCOR_HIP_COMPLEX64;
+    else if (!strcmp(name, "float32")) *dtype = AMPCOR_HIP_FLOAT32;
+    else {
+        PyErr_Format(PyExc_ValueError, "unknown raster pixel type '%s'", name);
+        return -1;
+    }
+    return 0;
+}
+
+int parse_precision(const char *name, int *precision)
+{
+    if (!strcmp(name, "fp32")) *precision = AMPCOR_HIP_FP32;
+    else if (!strcmp(name, "bf16")) *precision = AMPCOR_HIP_BF16;
+    else if (!strcmp(name, "fp16")) *precision = AMPCOR_HIP_FP16;
+    else {
+        PyErr_Format(PyExc_ValueError, "unknown precision '%s'", name);
+        return -1;
+    }
+    return 0;
+}
+
+// ---- rasters -----------------------------------------------------------------
+
+PyObject *slc_pixelSize(PyObject *, PyObject *) { return PyLong_FromLong(8); }
+
+PyObject *raster(PyObject *, PyObject *args, PyObject *kwds)
+{
+    Py_buffer view;
+    unsigned long lines = 0, samples = 0;
+    const char *dtype_name = "complex64";
+    int device = 0;
+    static const char *names[] = {"data", "lines", "samples", "dtype", "device", nullptr};
+    if (!PyArg_ParseTupleAndKeywords(args, kwds, "y*kk|si:raster", const_cast<char **>(names),
+                                     &view, &lines, &samples, &dtype_name, &device))
+        return nullptr;
+    int dtype;
+    if (parse_dtype(dtype_name, &dtype)) {
+        PyBuffer_Release(&view);
+        return nullptr;
+    }
+    const size_t need = (size_t)lines * samples * (dtype == AMPCOR_HIP_COMPLEX64 ? 8 : 4);
+    if ((size_t)view.len != need) {
+        PyBuffer_Release(&view);
+        PyErr_Format(PyExc_ValueError, "raster buffer holds %zd bytes, shape needs %zu", view.len, need);
+        return nullptr;
+    }
+    ampcor_hip_raster *r = nullptr;
+    int rc;
+    Py_BEGIN_ALLOW_THREADS
+    rc = ampcor_hip_raster_create(device, dtype, view.buf, lines, samples, &r);
+    Py_END_ALLOW_THREADS
+    PyBuffer_Release(&view);
+    if (rc) return raise(rc);
+    return PyCapsule_New(r, kRasterCapsule, free_raster);
+}
+
+// memory-map a raw complex64 raster file and upload it (ext/ampcor/slc.cc:50-93)
+PyObject *slc_map(PyObject *, PyObject *args, PyObject *kwds)
+{
+    const char *filename;
+    unsigned long samples = 0, lines = 0;
+    int device = 0;
+    static const char *names[] = {"filename", "samples", "lines", "device", nullptr};
+    if (!PyArg_ParseTupleAndKeywords(args, kwds, "skk|i:slc_map", const_cast<char **>(names),
+                                     &filename, &samples, &lines, &device))
+        return nullptr;
+    const size_t bytes = (size_t)lines * samples * 8;
+    int fd = open(filename, O_RDONLY);
+    if (fd < 0) return PyErr_SetFromErrnoWithFilename(PyExc_OSError, filename);
+    struct stat st;
+    if (fstat(fd, &st) != 0 || (size_t)st.st_size < bytes) {
+        close(fd);
+        PyErr_Format(PyExc_ValueError, "'%s' is smaller than %lu x %lu complex64 pixels", filename,
+                     lines, samples);
+        return nullptr;
+    }
+    void *map = bytes ? mmap(nullptr, bytes, PROT_READ, MAP_PRIVATE, fd, 0) : MAP_FAILED;
+    close(fd);
+    if (map == MAP_FAILED) return PyErr_SetFromErrnoWithFilename(PyExc_OSError, filename);
+    ampcor_hip_raster *r = nullptr;
+    int rc;
+    Py_BEGIN_ALLOW_THREADS
+    rc = ampcor_hip_raster_create(device, AMPCOR_HIP_COMPLEX64, map, lines, samples, &r);
+    Py_END_ALLOW_THREADS
+    munmap(map, bytes);
+    if (rc) return raise(rc);
+    return PyCapsule_New(r, kRasterCapsule, free_raster);
+}
+
+// ---- the worker ----------------------------------------------------------------
+
+PyObject *sequential(PyObject *, PyObject *args, PyObject *kwds)
+{
+    unsigned long pairs, refLines, refSamples, tgtLines, tgtSamples;
+    unsigned long refineMargin, refineFactor, zoomFactor;
+    const char *precision_name = "fp32";
+    int device = 0;
+    static const char *names[] = {"pairs", "chip", "window", "refineMargin", "refineFactor",
+                                  "zoomFactor", "precision", "device", nullptr};
+    if (!PyArg_ParseTupleAndKeywords(args, kwds, "k(kk)(kk)kkk|si:sequential",
+                                     const_cast<char **>(names), &pairs, &refLines, &refSamples,
+                                     &tgtLines, &tgtSamples, &refineMargin, &refineFactor,
+                                     &zoomFactor, &precision_name, &device))
+        return nullptr;
+    int precision;
+    if (parse_precision(precision_name, &precision)) return nullptr;
+    ampcor_hip_plan *plan = nullptr;
+    int rc = ampcor_hip_plan_create(device, pairs, refLines, refSamples, tgtLines, tgtSamples,
+                                    refineMargin, refineFactor, zoomFactor, precision, &plan);
+    if (rc) return raise(rc);
+    auto *w = new Worker{plan, PyList_New(0), pairs};
+    return PyCapsule_New(w, kWorkerCapsule, free_worker);
+}
+
+PyObject *add_tile(PyObject *args, bool reference)
+{
+    PyObject *pyWorker, *pyRaster;
+    unsigned long idx, bl, bs, el, es;
+    if (!PyArg_ParseTuple(args, reference ? "O!O!k(kk)(kk):addReference" : "O!O!k(kk)(kk):addTarget",
+                          &PyCapsule_Type, &pyWorker, &PyCapsule_Type, &pyRaster, &idx, &bl, &bs,
+                          &el, &es))
+        return nullptr;
+    Worker *w = get_worker(pyWorker);
+    if (!w) return nullptr;
+    ampcor_hip_raster *r = get_raster(pyRaster);
+    if (!r) return nullptr;
+    int rc = reference ? ampcor_hip_add_reference(w->plan, r, idx, bl, bs, el, es)
+                       : ampcor_hip_add_target(w->plan, r, idx, bl, bs, el, es);
+    if (rc) return raise(rc);
+    int found = PySequence_Contains(w->rasters, pyRaster);
+    if (found < 0) return nullptr;
+    if (!found && PyList_Append(w->rasters, pyRaster)) return nullptr;
+    Py_RETURN_NONE;
+}
+
+PyObject *addReference(PyObject *, PyObject *args) { return add_tile(args, true); }
+PyObject *addTarget(PyObject *, PyObject *args) { return add_tile(args, false); }
+
+// addPairs(worker, refRaster, tgtRaster, first, refBegins, tgtBegins): the two
+// corner arrays are buffers of int64 (line, sample) pairs
+PyObject *addPairs(PyObject *, PyObject *args)
+{
+    PyObject *pyWorker, *pyRef, *pyTgt;
+    unsigned long first;
+    Py_buffer rb, tb;
+    if (!PyArg_ParseTuple(args, "O!O!O!ky*y*:addPairs", &PyCapsule_Type, &pyWorker, &PyCapsule_Type,
+                          &pyRef, &PyCapsule_Type, &pyTgt, &first, &rb, &tb))
+        return nullptr;
+    PyObject *result = nullptr;
+    Worker *w = get_worker(pyWorker);
+    ampcor_hip_raster *ref = w ? get_raster(pyRef) : nullptr;
+    ampcor_hip_raster *tgt = ref ? get_raster(pyTgt) : nullptr;
+    if (tgt) {
+        if (rb.len != tb.len || rb.len % 16) {
+            PyErr_SetString(PyExc_ValueError, "corner buffers must hold equal counts of int64 pairs");
+        } else {
+            int rc = ampcor_hip_add_pairs(w->plan, ref, tgt, first, rb.len / 16,
+                                          static_cast<const int64_t *>(rb.buf),
+                                          static_cast<const int64_t *>(tb.buf));
+            if (rc) raise(rc);
+            else {
+                bool ok = true;
+                for (PyObject *o : {pyRef, pyTgt}) {
+                    int found = PySequence_Contains(w->rasters, o);
+                    if (found < 0 || (!found && PyList_Append(w->rasters, o))) ok = false;
+                }
+                if (ok) {
+                    Py_INCREF(Py_None);
+                    result = Py_None;
+                }
+            }
+        }
+    }
+    PyBuffer_Release(&rb);
+    PyBuffer_Release(&tb);
+    return result;
+}
+
+PyObject *adjust(PyObject *, PyObject *args)
+{
+    PyObject *pyWorker;
+    if (!PyArg_ParseTuple(args, "O!:adjust", &PyCapsule_Type, &pyWorker)) return nullptr;
+    Worker *w = get_worker(pyWorker);
+    if (!w) return nullptr;
+    std::vector<float> offsets(2 * w->pairs);
+    int rc;
+    Py_BEGIN_ALLOW_THREADS
+    rc = ampcor_hip_adjust(w->plan, offsets.data());
+    Py_END_ALLOW_THREADS
+    if (rc) return raise(rc);
+    PyObject *results = PyTuple_New(w->pairs);
+    if (!results) return nullptr;
+    for (size_t p = 0; p < w->pairs; ++p) {
+        PyObject *shift = Py_BuildValue("(dd)", (double)offsets[2 * p], (double)offsets[2 * p + 1]);
+        if (!shift) {
+            Py_DECREF(results);
+            return nullptr;
+        }
+        PyTuple_SET_ITEM(results, p, shift);
+    }
+    return results;
+}
+
+PyObject *bytes_of(size_t n)
+{
+    return PyBytes_FromStringAndSize(nullptr, (Py_ssize_t)n);
+}
+
+PyObject *products(PyObject *, PyObject *args)
+{
+    PyObject *pyWorker;
+    if (!PyArg_ParseTuple(args, "O!:products", &PyCapsule_Type, &pyWorker)) return nullptr;
+    Worker *w = get_worker(pyWorker);
+    if (!w) return nullptr;
+    const size_t N = w->pairs;
+    PyObject *off = bytes_of(8 * N), *cp = bytes_of(4 * N), *rp = bytes_of(4 * N), *sn = bytes_of(4 * N);
+    if (!off || !cp || !rp || !sn) {
+        Py_XDECREF(off); Py_XDECREF(cp); Py_XDECREF(rp); Py_XDECREF(sn);
+        return nullptr;
+    }
+    int rc = ampcor_hip_products(w->plan, reinterpret_cast<float *>(PyBytes_AS_STRING(off)),
+                                 reinterpret_cast<float *>(PyBytes_AS_STRING(cp)),
+                                 reinterpret_cast<float *>(PyBytes_AS_STRING(rp)),
+                                 reinterpret_cast<float *>(PyBytes_AS_STRING(sn)));
+    if (rc) {
+        Py_DECREF(off); Py_DECREF(cp); Py_DECREF(rp); Py_DECREF(sn);
+        return raise(rc);
+    }
+    return Py_BuildValue("{sNsNsNsN}", "offsets", off, "coarse_peak", cp, "refined_peak", rp, "snr", sn);
+}
+
+PyObject *dims(PyObject *, PyObject *args)
+{
+    PyObject *pyWorker;
+    if (!PyArg_ParseTuple(args, "O!:dims", &PyCapsule_Type, &pyWorker)) return nullptr;
+    Worker *w = get_worker(pyWorker);
+    if (!w) return nullptr;
+    int P, R, Z;
+    if (int rc = ampcor_hip_plan_dims(w->plan, &P, &R, &Z)) return raise(rc);
+    return Py_BuildValue("(iii)", P, R, Z);
+}
+
+PyObject *stage_gamma_impl(PyObject *args, bool refined)
+{
+    PyObject *pyWorker;
+    if (!PyArg_ParseTuple(args, "O!", &PyCapsule_Type, &pyWorker)) return nullptr;
+    Worker *w = get_worker(pyWorker);
+    if (!w) return nullptr;
+    int P, R, Z;
+    if (int rc = ampcor_hip_plan_dims(w->plan, &P, &R, &Z)) return raise(rc);
+    const size_t d = refined ? R : P;
+    PyObject *out = bytes_of(4 * w->pairs * d * d);
+    if (!out) return nullptr;
+    float *dst = reinterpret_cast<float *>(PyBytes_AS_STRING(out));
+    int rc = refined ? ampcor_hip_stage_gamma_refined(w->plan, dst) : ampcor_hip_stage_gamma(w->plan, dst);
+    if (rc) {
+        Py_DECREF(out);
+        return raise(rc);
+    }
+    return out;
+}
+
+PyObject *stage_gamma(PyObject *, PyObject *args) { return stage_gamma_impl(args, false); }
+PyObject *stage_gamma_refined(PyObject *, PyObject *args) { return stage_gamma_impl(args, true); }
+
+PyObject *stage_locations(PyObject *, PyObject *args)
+{
+    PyObject *pyWorker;
+    if (!PyArg_ParseTuple(args, "O!:stage_locations", &PyCapsule_Type, &pyWorker)) return nullptr;
+    Worker *w = get_worker(pyWorker);
+    if (!w) return nullptr;
+    const size_t n = 8 * w->pairs;
+    PyObject *c = bytes_of(n), *wi = bytes_of(n), *z = bytes_of(n);
+    if (!c || !wi || !z) {
+        Py_XDECREF(c); Py_XDECREF(wi); Py_XDECREF(z);
+        return nullptr;
+    }
+    int rc = ampcor_hip_stage_locations(w->plan, reinterpret_cast<int32_t *>(PyBytes_AS_STRING(c)),
+                                        reinterpret_cast<int32_t *>(PyBytes_AS_STRING(wi)),
+                                        reinterpret_cast<int32_t *>(PyBytes_AS_STRING(z)));
+    if (rc) {
+        Py_DECREF(c); Py_DECREF(wi); Py_DECREF(z);
+        return raise(rc);
+    }
+    return Py_BuildValue("{sNsNsN}", "coarse", c, "window", wi, "zoomed", z);
+}
+
+PyObject *set_timing(PyObject *, PyObject *args)
+{
+    PyObject *pyWorker;
+    int flag;
+    if (!PyArg_ParseTuple(args, "O!p:set_timing", &PyCapsule_Type, &pyWorker, &flag)) return nullptr;
+    Worker *w = get_worker(pyWorker);
+    if (!w) return nullptr;
+    if (int rc = ampcor_hip_set_timing(w->plan, flag)) return raise(rc);
+    Py_RETURN_NONE;
+}
+
+PyObject *stage_times(PyObject *, PyObject *args)
+{
+    PyObject *pyWorker;
+    if (!PyArg_ParseTuple(args, "O!:stage_times", &PyCapsule_Type, &pyWorker)) return nullptr;
+    Worker *w = get_worker(pyWorker);
+    if (!w) return nullptr;
+    float ms[64];
+    const char *names[64];
+    int count = 0;
+    if (int rc = ampcor_hip_stage_times(w->plan, 64, &count, ms, names)) return raise(rc);
+    PyObject *d = PyDict_New();
+    for (int k = 0; k < count && k < 64; ++k) {
+        PyObject *v = PyFloat_FromDouble(ms[k]);
+        PyDict_SetItemString(d, names[k], v);
+        Py_DECREF(v);
+    }
+    return d;
+}
+
+PyObject *device_count(PyObject *, PyObject *)
+{
+    int n = 0;
+    if (int rc = ampcor_hip_device_count(&n)) return raise(rc);
+    return PyLong_FromLong(n);
+}
+
+PyMethodDef methods[] = {
+    {"sequential", (PyCFunction)(void (*)(void))sequential, METH_VARARGS | METH_KEYWORDS,
+     "instantiate a new sequential correlation worker"},
+    {"addReference", addReference, METH_VARARGS, "process a reference tile"},
+    {"addTarget", addTarget, METH_VARARGS, "process a target tile"},
+    {"addPairs", addPairs, METH_VARARGS, "register a run of pairs from int64 corner arrays"},
+    {"adjust", adjust, METH_VARARGS, "perform pixel adjustments to the registration map"},
+    {"products", products, METH_VARARGS, "offsets, correlation peaks and SNR of the last adjust"},
+    {"dims", dims, METH_VARARGS, "(coarse, refined, zoomed) surface edges"},
+    {"stage_gamma", stage_gamma, METH_VARARGS, "coarse correlation surfaces of the last adjust"},
+    {"stage_gamma_refined", stage_gamma_refined, METH_VARARGS, "refined correlation surfaces"},
+    {"stage_locations", stage_locations, METH_VARARGS, "coarse, window and zoomed peak indices"},
+    {"set_timing", set_timing, METH_VARARGS, "enable per-stage device timing"},
+    {"stage_times", stage_times, METH_VARARGS, "per-stage device milliseconds of the last adjust"},
+    {"raster", (PyCFunction)(void (*)(void))raster, METH_VARARGS | METH_KEYWORDS,
+     "upload a raster from a buffer"},
+    {"slc_map", (PyCFunction)(void (*)(void))slc_map, METH_VARARGS | METH_KEYWORDS,
+     "map an SLC raster file of a given shape onto the device"},
+    {"slc_pixelSize", slc_pixelSize, METH_NOARGS, "the size of an SLC pixel"},
+    {"device_count", device_count, METH_NOARGS, "number of visible HIP devices"},
+    {nullptr, nullptr, 0, nullptr},
+};
+
+PyModuleDef module_def = {
+    PyModuleDef_HEAD_INIT, "ampcor_hip",
+    "ampcor_hip: the MI355X batched tile correlator", -1, methods,
+};
+
+}  // namespace
+
+PyMODINIT_FUNC PyInit_ampcor_hip(void)
+{
+    PyObject *m = PyModule_Create(&module_def);
+    if (!m) return nullptr;
+    g_error = PyErr_NewException("ampcor_hip.Error", nullptr, nullptr);
+    Py_XINCREF(g_error);
+    if (PyModule_AddObject(m, "Error", g_error) < 0) {
+        Py_XDECREF(g_error);
+        Py_DECREF(m);
+        return nullptr;
+    }
+    return m;
+}

@@ -1,0 +1,60 @@
+// kernels.h -- device kernels of the gfx950 tile correlator and their launchers.
+//
+// Every launcher enqueues on the given stream and returns the launch status;
+// none synchronizes.  Buffer layouts are pair-major and row-major throughout.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ampcor_hip {
+
+// one tile pair: where its reference chip and target window start in their rasters
+struct PairDesc {
+    const void *ref;     // reference raster base (device)
+    const void *tgt;     // target raster base (device)
+    int32_t ref_stride;  // samples per raster line
+    int32_t tgt_stride;
+    int32_t ref_line, ref_sample;  // upper-left corner of the C x C chip
+    int32_t tgt_line, tgt_sample;  // upper-left corner of the W x W window
+};
+
+// tiling of one correlation launch: placements [u0, u0+ub) x [v0, v0+vb) of one
+// pair, chip columns split into `nsplit` ranges, each its own workgroup
+struct CorrelateShape {
+    int bu;      // placement rows per lane (register block)
+    int ub, vb;  // placement tile extent
+    int tiles_u, tiles_v;
+    int nsplit;  // chip-column ranges (partial numerators summed by the normalizer)
+    int threads;
+    size_t lds_bytes;
+};
+
+// chooses the tiling for chip C over P x P placements; exact = keep each
+// placement's whole contraction in one lane (the FP32 drop-in order)
+CorrelateShape choose_correlate_shape(int C, int P, bool exact);
+
+hipError_t launch_gather_detect(hipStream_t s, const PairDesc *desc, int pairs, int dtype, int C,
+                                int W, float *ref_amp, float *tgt_amp);
+hipError_t launch_chip_stats(hipStream_t s, const float *amp, int pairs, int C, float *rT,
+                             double2 *stats);
+hipError_t launch_sat(hipStream_t s, const float *amp, int pairs, int W, double *S1, double *S2);
+hipError_t launch_correlate(hipStream_t s, const CorrelateShape &shape, const float *rT,
+                            const float *amp, int pairs, int C, int W, float *partial);
+hipError_t launch_normalize(hipStream_t s, const float *partial, int nsplit, const double *S1,
+                            const double *S2, const double2 *stats, int pairs, int C, int W,
+                            float *gamma);
+hipError_t launch_coarse_peak(hipStream_t s, const float *gamma, int pairs, int P, int W, int E,
+                              int refine_margin, int exclude, int2 *loc, int2 *window,
+                              float *peak, float *snr);
+// band-limited upsampling of complex tiles straight from the rasters:
+// role 0 = reference chips (N = C, origin = chip corner), role 1 = target
+// windows (N = E, origin = window corner + nudged (top, left))
+hipError_t launch_upsample(hipStream_t s, const PairDesc *desc, int pairs, int dtype, int role,
+                           const int2 *window, int N, int rf, const float *U, const float *Ut,
+                           float2 *work, float *amp);
+hipError_t launch_zoom(hipStream_t s, const float *gamma_refined, int pairs, int R, int zf,
+                       const double *dirichlet, const int2 *window, const float *coarse_peak,
+                       int margin, int rf, float *offsets, float *refined_peak, int2 *zoom_loc);
+
+}  // namespace ampcor_hip

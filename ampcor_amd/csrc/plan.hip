@@ -1,0 +1,644 @@
+// plan.hip -- the C ABI (include/ampcor_hip.h): rasters, plans, adjust.
+//
+// A plan mirrors the reference's Sequential worker
+// (lib/libampcor_cuda/correlators/Sequential.{h,icc}) without its host arena:
+// pairs are registered as descriptors (raster, upper-left corner) and adjust
+// gathers the tiles straight from device-resident rasters.  All device
+// buffers are allocated once, at plan creation, and every stage runs on the
+// plan's own stream with no intermediate synchronization.
+#include "ampcor_hip.h"
+#include "kernels.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+using namespace ampcor_hip;
+
+struct ampcor_hip_raster {
+    int device;
+    int dtype;
+    const void *data;
+    size_t lines, samples;
+    bool owned;
+};
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string &message)
+{
+    g_error = message;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what)
+{
+    return fail(AMPCOR_HIP_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(call, what)                                  \
+    do {                                                     \
+        hipError_t _e = (call);                              \
+        if (_e != hipSuccess) return hip_fail(_e, (what));   \
+    } while (0)
+
+// one entry of the band-limited upsampling kernel (closed form):
+// t = p/rf - n; odd N: sin(pi t) / (N sin(pi t / N));
+// even N (Nyquist bin split): sin(pi t) cot(pi t / N) / N
+double upsample_kernel(int N, int rf, int p, int n)
+{
+    if (p % rf == 0) return (p / rf == n) ? 1.0 : 0.0;
+    const double t = (double)(p - rf * n) / rf;
+    const double a = M_PI * t;
+    if (N % 2) return std::sin(a) / (N * std::sin(a / N));
+    return std::sin(a) * std::cos(a / N) / (N * std::sin(a / N));
+}
+
+// periodic Dirichlet kernel of an odd-size R grid at x = d / zf
+double dirichlet(int R, int zf, int d)
+{
+    if (d % zf == 0) return ((d / zf) % R == 0) ? 1.0 : 0.0;
+    const double x = (double)d / zf;
+    return std::sin(M_PI * x) / (R * std::sin(M_PI * x / R));
+}
+
+template <typename T>
+struct DeviceBuffer {
+    T *ptr = nullptr;
+    size_t count = 0;
+    hipError_t alloc(size_t n)
+    {
+        count = n;
+        return hipMalloc((void **)&ptr, n * sizeof(T) > 0 ? n * sizeof(T) : 1);
+    }
+    void release()
+    {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+    }
+};
+
+const char *const kStageNames[] = {
+    "gather_detect", "chip_stats", "sat", "correlate", "normalize", "coarse_peak",
+    "upsample_ref", "upsample_tgt", "chip_stats_refined", "sat_refined", "correlate_refined",
+    "normalize_refined", "zoom",
+};
+constexpr int kStages = sizeof(kStageNames) / sizeof(kStageNames[0]);
+
+}  // namespace
+
+struct ampcor_hip_plan {
+    int device = 0;
+    int precision = AMPCOR_HIP_FP32;
+    int dtype = -1;
+    size_t pairs = 0;
+    int C = 0, W = 0, m = 0, P = 0, rf = 0, rm = 0, E = 0, Cr = 0, Wr = 0, R = 0, zf = 0, Z = 0;
+    hipStream_t stream = nullptr;
+
+    std::vector<PairDesc> desc;
+    std::vector<unsigned char> have_ref, have_tgt;
+    bool desc_dirty = true;
+    bool adjusted = false;
+
+    CorrelateShape coarse_shape{}, refined_shape{};
+
+    DeviceBuffer<PairDesc> d_desc;
+    DeviceBuffer<float> amp_ref, amp_tgt, rT, partial, gamma;
+    DeviceBuffer<double> S1, S2;
+    DeviceBuffer<double2> stats, stats_r;
+    DeviceBuffer<int2> loc, window, zoom_loc;
+    DeviceBuffer<float> peak, snr, peak_r, offsets;
+    DeviceBuffer<float2> work;
+    DeviceBuffer<float> amp_ref_r, amp_tgt_r, rT_r, partial_r, gamma_r;
+    DeviceBuffer<float> U_ref, Ut_ref, U_tgt, Ut_tgt;
+    DeviceBuffer<double> D;
+
+    bool timing = false;
+    hipEvent_t ev[kStages + 1] = {};
+    float stage_ms[kStages] = {};
+
+    void release()
+    {
+        DeviceBuffer<float> *fb[] = {&amp_ref, &amp_tgt, &rT, &partial, &gamma, &peak, &snr,
+                                     &peak_r, &offsets, &amp_ref_r, &amp_tgt_r, &rT_r,
+                                     &partial_r, &gamma_r, &U_ref, &Ut_ref, &U_tgt, &Ut_tgt};
+        for (auto *b : fb) b->release();
+        S1.release();
+        S2.release();
+        D.release();
+        stats.release();
+        stats_r.release();
+        loc.release();
+        window.release();
+        zoom_loc.release();
+        work.release();
+        d_desc.release();
+        for (auto &e : ev)
+            if (e) {
+                (void)hipEventDestroy(e);
+                e = nullptr;
+            }
+        if (stream) (void)hipStreamDestroy(stream);
+        stream = nullptr;
+    }
+};
+
+namespace {
+
+// makes the plan's device current for the duration of a call
+struct DeviceGuard {
+    int previous = -1;
+    explicit DeviceGuard(int device)
+    {
+        if (hipGetDevice(&previous) != hipSuccess) previous = -1;
+        if (previous != device) (void)hipSetDevice(device);
+    }
+    ~DeviceGuard()
+    {
+        if (previous >= 0) (void)hipSetDevice(previous);
+    }
+};
+
+int check_device(int device)
+{
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) return fail(AMPCOR_HIP_EDEVICE, "no HIP device available");
+    if (device < 0 || device >= count)
+        return fail(AMPCOR_HIP_EINVAL, "device index " + std::to_string(device) + " out of range");
+    return AMPCOR_HIP_OK;
+}
+
+int upload_matrix(DeviceBuffer<float> &buf, const std::vector<float> &host)
+{
+    HIP_TRY(buf.alloc(host.size()), "allocating an upsampling matrix");
+    HIP_TRY(hipMemcpy(buf.ptr, host.data(), host.size() * sizeof(float), hipMemcpyHostToDevice),
+            "uploading an upsampling matrix");
+    return AMPCOR_HIP_OK;
+}
+
+int upload_upsampler(DeviceBuffer<float> &U, DeviceBuffer<float> &Ut, int N, int rf)
+{
+    const int M = rf * N;
+    std::vector<float> u((size_t)M * N), ut((size_t)N * M);
+    for (int p = 0; p < M; ++p)
+        for (int n = 0; n < N; ++n) {
+            const float v = (float)upsample_kernel(N, rf, p, n);
+            u[(size_t)p * N + n] = v;
+            ut[(size_t)n * M + p] = v;
+        }
+    int rc = upload_matrix(U, u);
+    if (rc) return rc;
+    return upload_matrix(Ut, ut);
+}
+
+int register_tile(ampcor_hip_plan *plan, const ampcor_hip_raster *raster, size_t idx,
+                  size_t bl, size_t bs, size_t el, size_t es, bool reference)
+{
+    if (!plan || !raster) return fail(AMPCOR_HIP_EINVAL, "null plan or raster");
+    if (idx >= plan->pairs)
+        return fail(AMPCOR_HIP_EINVAL, "pair index " + std::to_string(idx) + " out of range [0, " +
+                                           std::to_string(plan->pairs) + ")");
+    if (raster->device != plan->device)
+        return fail(AMPCOR_HIP_EINVAL, "raster lives on another device than the plan");
+    if (plan->dtype >= 0 && raster->dtype != plan->dtype)
+        return fail(AMPCOR_HIP_EINVAL, "all rasters of a plan must share one pixel type");
+    const size_t edge = reference ? plan->C : plan->W;
+    if (el < bl || es < bs || el - bl != edge || es - bs != edge)
+        return fail(AMPCOR_HIP_EINVAL, std::string(reference ? "reference" : "target") +
+                                           " slice shape does not match the plan's " +
+                                           (reference ? "chip" : "window"));
+    if (el > raster->lines || es > raster->samples)
+        return fail(AMPCOR_HIP_EINVAL, "slice extends beyond the raster");
+    if (raster->samples > 0x7fffffff || raster->lines > 0x7fffffff)
+        return fail(AMPCOR_HIP_EINVAL, "raster too large");
+    plan->dtype = raster->dtype;
+    PairDesc &d = plan->desc[idx];
+    if (reference) {
+        d.ref = raster->data;
+        d.ref_stride = (int32_t)raster->samples;
+        d.ref_line = (int32_t)bl;
+        d.ref_sample = (int32_t)bs;
+        plan->have_ref[idx] = 1;
+    } else {
+        d.tgt = raster->data;
+        d.tgt_stride = (int32_t)raster->samples;
+        d.tgt_line = (int32_t)bl;
+        d.tgt_sample = (int32_t)bs;
+        plan->have_tgt[idx] = 1;
+    }
+    plan->desc_dirty = true;
+    plan->adjusted = false;
+    return AMPCOR_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *ampcor_hip_last_error(void) { return g_error.c_str(); }
+
+int ampcor_hip_device_count(int *count)
+{
+    if (!count) return fail(AMPCOR_HIP_EINVAL, "null count");
+    *count = 0;
+    hipError_t e = hipGetDeviceCount(count);
+    if (e != hipSuccess) {
+        *count = 0;
+        if (e == hipErrorNoDevice) return AMPCOR_HIP_OK;
+        return hip_fail(e, "hipGetDeviceCount");
+    }
+    return AMPCOR_HIP_OK;
+}
+
+int ampcor_hip_raster_create(int device, int dtype, const void *host, size_t lines, size_t samples,
+                             ampcor_hip_raster **out)
+{
+    if (!out || !host) return fail(AMPCOR_HIP_EINVAL, "null argument");
+    *out = nullptr;
+    if (dtype != AMPCOR_HIP_COMPLEX64 && dtype != AMPCOR_HIP_FLOAT32)
+        return fail(AMPCOR_HIP_EINVAL, "unknown raster pixel type");
+    if (lines == 0 || samples == 0) return fail(AMPCOR_HIP_EINVAL, "empty raster");
+    if (int rc = check_device(device)) return rc;
+    DeviceGuard guard(device);
+    const size_t bytes = lines * samples * (dtype == AMPCOR_HIP_COMPLEX64 ? 8 : 4);
+    void *d = nullptr;
+    hipError_t e = hipMalloc(&d, bytes);
+    if (e != hipSuccess)
+        return fail(AMPCOR_HIP_ENOMEM, "allocating " + std::to_string(bytes) + " bytes for a raster");
+    e = hipMemcpy(d, host, bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(d);
+        return hip_fail(e, "uploading a raster");
+    }
+    auto *r = new (std::nothrow) ampcor_hip_raster{device, dtype, d, lines, samples, true};
+    if (!r) {
+        (void)hipFree(d);
+        return fail(AMPCOR_HIP_ENOMEM, "raster handle");
+    }
+    *out = r;
+    return AMPCOR_HIP_OK;
+}
+
+int ampcor_hip_raster_wrap(int device, int dtype, const void *device_ptr, size_t lines,
+                           size_t samples, ampcor_hip_raster **out)
+{
+    if (!out || !device_ptr) return fail(AMPCOR_HIP_EINVAL, "null argument");
+    *out = nullptr;
+    if (dtype != AMPCOR_HIP_COMPLEX64 && dtype != AMPCOR_HIP_FLOAT32)
+        return fail(AMPCOR_HIP_EINVAL, "unknown raster pixel type");
+    if (lines == 0 || samples == 0) return fail(AMPCOR_HIP_EINVAL, "empty raster");
+    if (int rc = check_device(device)) return rc;
+    auto *r = new (std::nothrow) ampcor_hip_raster{device, dtype, device_ptr, lines, samples, false};
+    if (!r) return fail(AMPCOR_HIP_ENOMEM, "raster handle");
+    *out = r;
+    return AMPCOR_HIP_OK;
+}
+
+int ampcor_hip_raster_shape(const ampcor_hip_raster *raster, size_t *lines, size_t *samples)
+{
+    if (!raster || !lines || !samples) return fail(AMPCOR_HIP_EINVAL, "null argument");
+    *lines = raster->lines;
+    *samples = raster->samples;
+    return AMPCOR_HIP_OK;
+}
+
+int ampcor_hip_raster_destroy(ampcor_hip_raster *raster)
+{
+    if (!raster) return AMPCOR_HIP_OK;
+    if (raster->owned && raster->data) {
+        DeviceGuard guard(raster->device);
+        (void)hipFree(const_cast<void *>(raster->data));
+    }
+    delete raster;
+    return AMPCOR_HIP_OK;
+}
+
+int ampcor_hip_plan_create(int device, size_t pairs, size_t ref_lines, size_t ref_samples,
+                           size_t tgt_lines, size_t tgt_samples, size_t refine_margin,
+                           size_t refine_factor, size_t zoom_factor, int precision,
+                           ampcor_hip_plan **out)
+{
+    if (!out) return fail(AMPCOR_HIP_EINVAL, "null output pointer");
+    *out = nullptr;
+    if (pairs == 0) return fail(AMPCOR_HIP_EINVAL, "a plan needs at least one pair");
+    if (pairs > (size_t)1 << 24) return fail(AMPCOR_HIP_EINVAL, "too many pairs for one plan");
+    // the kernels assume square tiles (Sequential.icc:56-60 makes the same assumption)
+    if (ref_lines != ref_samples || tgt_lines != tgt_samples)
+        return fail(AMPCOR_HIP_EINVAL, "chips and windows must be square");
+    const size_t C = ref_lines, W = tgt_lines;
+    if (C < 2 || C > 512) return fail(AMPCOR_HIP_EINVAL, "chip edge must be in [2, 512]");
+    if (W < C || (W - C) % 2) return fail(AMPCOR_HIP_EINVAL, "window - chip must be even and >= 0");
+    if (W > 512) return fail(AMPCOR_HIP_EINVAL, "window edge must be <= 512");
+    const size_t m = (W - C) / 2;
+    if (refine_margin > m)
+        return fail(AMPCOR_HIP_EINVAL, "refine margin exceeds the search margin");
+    if (refine_factor < 1 || refine_factor > 8) return fail(AMPCOR_HIP_EINVAL, "refine factor must be in [1, 8]");
+    if (zoom_factor < 1 || zoom_factor > 31) return fail(AMPCOR_HIP_EINVAL, "zoom factor must be in [1, 31]");
+    const size_t E = C + 2 * refine_margin, Cr = refine_factor * C, Wr = refine_factor * E;
+    const size_t R = Wr - Cr + 1;
+    if (Wr > 512) return fail(AMPCOR_HIP_EINVAL, "refined window edge must be <= 512");
+    if (R > 129) return fail(AMPCOR_HIP_EINVAL, "refined surface too large (2 rf rm + 1 > 129)");
+    if (precision != AMPCOR_HIP_FP32)
+        return fail(AMPCOR_HIP_EINVAL, "only the FP32 form is available in this build");
+    if (int rc = check_device(device)) return rc;
+
+    auto *p = new (std::nothrow) ampcor_hip_plan();
+    if (!p) return fail(AMPCOR_HIP_ENOMEM, "plan");
+    p->device = device;
+    p->precision = precision;
+    p->pairs = pairs;
+    p->C = (int)C;
+    p->W = (int)W;
+    p->m = (int)m;
+    p->P = (int)(W - C + 1);
+    p->rf = (int)refine_factor;
+    p->rm = (int)refine_margin;
+    p->E = (int)E;
+    p->Cr = (int)Cr;
+    p->Wr = (int)Wr;
+    p->R = (int)R;
+    p->zf = (int)zoom_factor;
+    p->Z = p->zf * p->R;
+    p->desc.assign(pairs, PairDesc{});
+    p->have_ref.assign(pairs, 0);
+    p->have_tgt.assign(pairs, 0);
+    p->coarse_shape = choose_correlate_shape(p->C, p->P, true);
+    p->refined_shape = choose_correlate_shape(p->Cr, p->R, false);
+
+    DeviceGuard guard(device);
+    auto cleanup = [&](int rc) {
+        p->release();
+        delete p;
+        return rc;
+    };
+    hipError_t e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return cleanup(hip_fail(e, "creating the plan stream"));
+
+    const size_t N = pairs;
+    const size_t Ws = std::max(W, Wr) + 1;
+    struct {
+        hipError_t e;
+    } r{hipSuccess};
+    auto A = [&](hipError_t x) {
+        if (r.e == hipSuccess) r.e = x;
+    };
+    A(p->d_desc.alloc(N));
+    A(p->amp_ref.alloc(N * C * C));
+    A(p->amp_tgt.alloc(N * W * W));
+    A(p->rT.alloc(N * C * C));
+    A(p->stats.alloc(N));
+    A(p->S1.alloc(N * Ws * Ws));
+    A(p->S2.alloc(N * Ws * Ws));
+    A(p->partial.alloc(N * p->coarse_shape.nsplit * p->P * p->P));
+    A(p->gamma.alloc(N * p->P * p->P));
+    A(p->loc.alloc(N));
+    A(p->window.alloc(N));
+    A(p->peak.alloc(N));
+    A(p->snr.alloc(N));
+    A(p->work.alloc(N * std::max(C * Cr, E * Wr)));
+    A(p->amp_ref_r.alloc(N * Cr * Cr));
+    A(p->amp_tgt_r.alloc(N * Wr * Wr));
+    A(p->rT_r.alloc(N * Cr * Cr));
+    A(p->stats_r.alloc(N));
+    A(p->partial_r.alloc(N * p->refined_shape.nsplit * R * R));
+    A(p->gamma_r.alloc(N * R * R));
+    A(p->offsets.alloc(2 * N));
+    A(p->peak_r.alloc(N));
+    A(p->zoom_loc.alloc(N));
+    if (r.e != hipSuccess) {
+        std::string msg = std::string("allocating plan buffers for ") + std::to_string(N) +
+                          " pairs: " + hipGetErrorString(r.e);
+        return cleanup(fail(AMPCOR_HIP_ENOMEM, msg));
+    }
+    if (int rc = upload_upsampler(p->U_ref, p->Ut_ref, p->C, p->rf)) return cleanup(rc);
+    if (int rc = upload_upsampler(p->U_tgt, p->Ut_tgt, p->E, p->rf)) return cleanup(rc);
+    {
+        std::vector<double> d(2 * p->Z + 1);
+        for (int k = -p->Z; k <= p->Z; ++k) d[k + p->Z] = dirichlet(p->R, p->zf, k);
+        e = p->D.alloc(d.size());
+        if (e == hipSuccess)
+            e = hipMemcpy(p->D.ptr, d.data(), d.size() * sizeof(double), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return cleanup(hip_fail(e, "uploading the zoom kernel table"));
+    }
+    for (auto &ev : p->ev) {
+        e = hipEventCreate(&ev);
+        if (e != hipSuccess) return cleanup(hip_fail(e, "creating timing events"));
+    }
+    // the contraction's LDS use can exceed the 64 KiB default
+    *out = p;
+    return AMPCOR_HIP_OK;
+}
+
+int ampcor_hip_plan_destroy(ampcor_hip_plan *plan)
+{
+    if (!plan) return AMPCOR_HIP_OK;
+    {
+        DeviceGuard guard(plan->device);
+        if (plan->stream) (void)hipStreamSynchronize(plan->stream);
+        plan->release();
+    }
+    delete plan;
+    return AMPCOR_HIP_OK;
+}
+
+int ampcor_hip_plan_pairs(const ampcor_hip_plan *plan, size_t *pairs)
+{
+    if (!plan || !pairs) return fail(AMPCOR_HIP_EINVAL, "null argument");
+    *pairs = plan->pairs;
+    return AMPCOR_HIP_OK;
+}
+
+int ampcor_hip_plan_dims(const ampcor_hip_plan *plan, int *coarse_dim, int *refined_dim,
+                         int *zoomed_dim)
+{
+    if (!plan) return fail(AMPCOR_HIP_EINVAL, "null plan");
+    if (coarse_dim) *coarse_dim = plan->P;
+    if (refined_dim) *refined_dim = plan->R;
+    if (zoomed_dim) *zoomed_dim = plan->Z;
+    return AMPCOR_HIP_OK;
+}
+
+int ampcor_hip_add_reference(ampcor_hip_plan *plan, const ampcor_hip_raster *raster, size_t idx,
+                             size_t begin_line, size_t begin_sample, size_t end_line,
+                             size_t end_sample)
+{
+    return register_tile(plan, raster, idx, begin_line, begin_sample, end_line, end_sample, true);
+}
+
+int ampcor_hip_add_target(ampcor_hip_plan *plan, const ampcor_hip_raster *raster, size_t idx,
+                          size_t begin_line, size_t begin_sample, size_t end_line,
+                          size_t end_sample)
+{
+    return register_tile(plan, raster, idx, begin_line, begin_sample, end_line, end_sample, false);
+}
+
+int ampcor_hip_add_pairs(ampcor_hip_plan *plan, const ampcor_hip_raster *reference,
+                         const ampcor_hip_raster *target, size_t first, size_t count,
+                         const int64_t *ref_begin, const int64_t *tgt_begin)
+{
+    if (!plan || !reference || !target || (count && (!ref_begin || !tgt_begin)))
+        return fail(AMPCOR_HIP_EINVAL, "null argument");
+    if (first > plan->pairs || count > plan->pairs - first)
+        return fail(AMPCOR_HIP_EINVAL, "pair range out of bounds");
+    for (size_t k = 0; k < count; ++k) {
+        const int64_t rl = ref_begin[2 * k], rs = ref_begin[2 * k + 1];
+        const int64_t tl = tgt_begin[2 * k], ts = tgt_begin[2 * k + 1];
+        if (rl < 0 || rs < 0 || tl < 0 || ts < 0)
+            return fail(AMPCOR_HIP_EINVAL, "negative slice origin at pair " + std::to_string(first + k));
+        int rc = register_tile(plan, reference, first + k, rl, rs, rl + plan->C, rs + plan->C, true);
+        if (rc) return rc;
+        rc = register_tile(plan, target, first + k, tl, ts, tl + plan->W, ts + plan->W, false);
+        if (rc) return rc;
+    }
+    return AMPCOR_HIP_OK;
+}
+
+int ampcor_hip_set_timing(ampcor_hip_plan *plan, int enabled)
+{
+    if (!plan) return fail(AMPCOR_HIP_EINVAL, "null plan");
+    plan->timing = enabled != 0;
+    return AMPCOR_HIP_OK;
+}
+
+int ampcor_hip_adjust(ampcor_hip_plan *plan, float *offsets)
+{
+    if (!plan) return fail(AMPCOR_HIP_EINVAL, "null plan");
+    for (size_t k = 0; k < plan->pairs; ++k)
+        if (!plan->have_ref[k] || !plan->have_tgt[k])
+            return fail(AMPCOR_HIP_ESTATE, "pair " + std::to_string(k) +
+                                               " has no reference or target tile registered");
+    DeviceGuard guard(plan->device);
+    hipStream_t s = plan->stream;
+    const int N = (int)plan->pairs;
+    if (plan->desc_dirty) {
+        HIP_TRY(hipMemcpyAsync(plan->d_desc.ptr, plan->desc.data(), N * sizeof(PairDesc),
+                               hipMemcpyHostToDevice, s),
+                "uploading pair descriptors");
+        plan->desc_dirty = false;
+    }
+    int stage = 0;
+    auto mark = [&]() -> hipError_t {
+        if (!plan->timing) return hipSuccess;
+        return hipEventRecord(plan->ev[stage++], s);
+    };
+#define STEP(call, what)                        \
+    do {                                        \
+        HIP_TRY(mark(), "recording an event");  \
+        HIP_TRY((call), (what));                \
+    } while (0)
+
+    auto *p = plan;
+    STEP(launch_gather_detect(s, p->d_desc.ptr, N, p->dtype, p->C, p->W, p->amp_ref.ptr, p->amp_tgt.ptr),
+         "gather_detect");
+    STEP(launch_chip_stats(s, p->amp_ref.ptr, N, p->C, p->rT.ptr, p->stats.ptr), "chip_stats");
+    STEP(launch_sat(s, p->amp_tgt.ptr, N, p->W, p->S1.ptr, p->S2.ptr), "sat");
+    STEP(launch_correlate(s, p->coarse_shape, p->rT.ptr, p->amp_tgt.ptr, N, p->C, p->W, p->partial.ptr),
+         "correlate");
+    STEP(launch_normalize(s, p->partial.ptr, p->coarse_shape.nsplit, p->S1.ptr, p->S2.ptr, p->stats.ptr,
+                          N, p->C, p->W, p->gamma.ptr),
+         "normalize");
+    STEP(launch_coarse_peak(s, p->gamma.ptr, N, p->P, p->W, p->E, p->rm, p->rf, p->loc.ptr,
+                            p->window.ptr, p->peak.ptr, p->snr.ptr),
+         "coarse_peak");
+    STEP(launch_upsample(s, p->d_desc.ptr, N, p->dtype, 0, p->window.ptr, p->C, p->rf, p->U_ref.ptr,
+                         p->Ut_ref.ptr, p->work.ptr, p->amp_ref_r.ptr),
+         "upsample_ref");
+    STEP(launch_upsample(s, p->d_desc.ptr, N, p->dtype, 1, p->window.ptr, p->E, p->rf, p->U_tgt.ptr,
+                         p->Ut_tgt.ptr, p->work.ptr, p->amp_tgt_r.ptr),
+         "upsample_tgt");
+    STEP(launch_chip_stats(s, p->amp_ref_r.ptr, N, p->Cr, p->rT_r.ptr, p->stats_r.ptr),
+         "chip_stats_refined");
+    STEP(launch_sat(s, p->amp_tgt_r.ptr, N, p->Wr, p->S1.ptr, p->S2.ptr), "sat_refined");
+    STEP(launch_correlate(s, p->refined_shape, p->rT_r.ptr, p->amp_tgt_r.ptr, N, p->Cr, p->Wr,
+                          p->partial_r.ptr),
+         "correlate_refined");
+    STEP(launch_normalize(s, p->partial_r.ptr, p->refined_shape.nsplit, p->S1.ptr, p->S2.ptr,
+                          p->stats_r.ptr, N, p->Cr, p->Wr, p->gamma_r.ptr),
+         "normalize_refined");
+    STEP(launch_zoom(s, p->gamma_r.ptr, N, p->R, p->zf, p->D.ptr, p->window.ptr, p->peak.ptr, p->m,
+                     p->rf, p->offsets.ptr, p->peak_r.ptr, p->zoom_loc.ptr),
+         "zoom");
+#undef STEP
+    HIP_TRY(mark(), "recording an event");
+    if (offsets)
+        HIP_TRY(hipMemcpyAsync(offsets, p->offsets.ptr, 2 * (size_t)N * sizeof(float),
+                               hipMemcpyDeviceToHost, s),
+                "copying offsets to the host");
+    HIP_TRY(hipStreamSynchronize(s), "adjust");
+    if (plan->timing)
+        for (int k = 0; k < kStages; ++k)
+            HIP_TRY(hipEventElapsedTime(&plan->stage_ms[k], plan->ev[k], plan->ev[k + 1]),
+                    "reading stage times");
+    plan->adjusted = true;
+    return AMPCOR_HIP_OK;
+}
+
+int ampcor_hip_stage_times(ampcor_hip_plan *plan, int capacity, int *count, float *ms,
+                           const char **names)
+{
+    if (!plan || !count) return fail(AMPCOR_HIP_EINVAL, "null argument");
+    *count = kStages;
+    if (!plan->timing || !plan->adjusted)
+        return fail(AMPCOR_HIP_ESTATE, "no timed adjust has run on this plan");
+    for (int k = 0; k < kStages && k < capacity; ++k) {
+        if (ms) ms[k] = plan->stage_ms[k];
+        if (names) names[k] = kStageNames[k];
+    }
+    return AMPCOR_HIP_OK;
+}
+
+static int copy_out(ampcor_hip_plan *plan, void *dst, const void *src, size_t bytes)
+{
+    if (!dst) return AMPCOR_HIP_OK;
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "copying a product to the host");
+    return AMPCOR_HIP_OK;
+}
+
+int ampcor_hip_products(ampcor_hip_plan *plan, float *offsets, float *coarse_peak,
+                        float *refined_peak, float *snr)
+{
+    if (!plan) return fail(AMPCOR_HIP_EINVAL, "null plan");
+    if (!plan->adjusted) return fail(AMPCOR_HIP_ESTATE, "products requested before adjust");
+    DeviceGuard guard(plan->device);
+    const size_t N = plan->pairs;
+    if (int rc = copy_out(plan, offsets, plan->offsets.ptr, 2 * N * sizeof(float))) return rc;
+    if (int rc = copy_out(plan, coarse_peak, plan->peak.ptr, N * sizeof(float))) return rc;
+    if (int rc = copy_out(plan, refined_peak, plan->peak_r.ptr, N * sizeof(float))) return rc;
+    return copy_out(plan, snr, plan->snr.ptr, N * sizeof(float));
+}
+
+int ampcor_hip_stage_gamma(ampcor_hip_plan *plan, float *gamma)
+{
+    if (!plan || !gamma) return fail(AMPCOR_HIP_EINVAL, "null argument");
+    if (!plan->adjusted) return fail(AMPCOR_HIP_ESTATE, "stage requested before adjust");
+    DeviceGuard guard(plan->device);
+    return copy_out(plan, gamma, plan->gamma.ptr, plan->pairs * plan->P * plan->P * sizeof(float));
+}
+
+int ampcor_hip_stage_gamma_refined(ampcor_hip_plan *plan, float *gamma)
+{
+    if (!plan || !gamma) return fail(AMPCOR_HIP_EINVAL, "null argument");
+    if (!plan->adjusted) return fail(AMPCOR_HIP_ESTATE, "stage requested before adjust");
+    DeviceGuard guard(plan->device);
+    return copy_out(plan, gamma, plan->gamma_r.ptr, plan->pairs * plan->R * plan->R * sizeof(float));
+}
+
+int ampcor_hip_stage_locations(ampcor_hip_plan *plan, int32_t *coarse, int32_t *window,
+                               int32_t *zoomed)
+{
+    if (!plan) return fail(AMPCOR_HIP_EINVAL, "null plan");
+    if (!plan->adjusted) return fail(AMPCOR_HIP_ESTATE, "stage requested before adjust");
+    DeviceGuard guard(plan->device);
+    const size_t bytes = plan->pairs * sizeof(int2);
+    if (int rc = copy_out(plan, coarse, plan->loc.ptr, bytes)) return rc;
+    if (int rc = copy_out(plan, window, plan->window.ptr, bytes)) return rc;
+    return copy_out(plan, zoomed, plan->zoom_loc.ptr, bytes);
+}
+
+}  // extern "C"

@@ -1,0 +1,143 @@
+/*
+ * ampcor_hip.h -- C ABI of the MI355X (gfx950) batched tile correlator.
+ *
+ * This is the boundary a HIP correlator plugs into beside ampcor's Sequential
+ * and CUDA workers.  Every entry point returns an int status (0 = success) and
+ * never throws; ampcor_hip_last_error() describes the most recent failure on
+ * the calling thread.  No torch or C++ types cross the boundary.
+ *
+ * Mapping to the reference interface (paths relative to the ampcor tree):
+ *
+ *   ampcor_hip_plan_create   <- ampcor_cuda.sequential(pairs, (refLines, refSamples),
+ *                               (tgtLines, tgtSamples), refineMargin, refineFactor,
+ *                               zoomFactor): ext/ampcor_cuda/sequential.cc:40-73,
+ *                               worker ctor lib/libampcor_cuda/correlators/Sequential.icc:173-234
+ *   ampcor_hip_add_reference <- ampcor_cuda.addReference(worker, slc, idx, begin, end):
+ *                               ext/ampcor_cuda/sequential.cc:85-143,
+ *                               Sequential::addReferenceTile Sequential.icc:15-28
+ *   ampcor_hip_add_target    <- ampcor_cuda.addTarget(...): ext/ampcor_cuda/sequential.cc:155-213,
+ *                               Sequential::addTargetTile Sequential.icc:31-44
+ *   ampcor_hip_add_pairs     <- the per-pair loop of pkg/ampcor/correlators/CUDA.py:50-54,
+ *                               batched into one call
+ *   ampcor_hip_adjust        <- ampcor_cuda.adjust(worker): ext/ampcor_cuda/sequential.cc:225-271,
+ *                               Sequential::adjust Sequential.icc:47-141
+ *   ampcor_hip_plan_destroy  <- the worker capsule destructor ext/ampcor_cuda/sequential.cc:274-286
+ *   ampcor_hip_raster_*      <- the SLC raster the tiles are sliced from:
+ *                               lib/libampcor/dom/SLC.h:18-80, ampcor.slc_map ext/ampcor/slc.cc:50-93;
+ *                               here the raster is uploaded once and stays in device memory
+ *   ampcor_hip_products,
+ *   ampcor_hip_stage_*       no counterpart: the reference returns only the offsets and
+ *                            exposes its stages as C++ methods to its tests
+ *                            (lib/libampcor_cuda/correlators/Sequential.h:69-113)
+ */
+#ifndef AMPCOR_HIP_H
+#define AMPCOR_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define AMPCOR_HIP_OK 0
+#define AMPCOR_HIP_EINVAL 1  /* bad argument (shape, index, slice, null pointer) */
+#define AMPCOR_HIP_ENOMEM 2  /* device or host allocation failed */
+#define AMPCOR_HIP_EDEVICE 3 /* HIP runtime error, or no device */
+#define AMPCOR_HIP_ESTATE 4  /* call out of order (e.g. products before adjust) */
+
+/* arithmetic of the correlation contraction */
+#define AMPCOR_HIP_FP32 0 /* plain FP32: the drop-in form */
+#define AMPCOR_HIP_BF16 1 /* mixed: BF16 MFMA contraction, FP32 accumulate */
+#define AMPCOR_HIP_FP16 2 /* mixed: FP16 MFMA contraction, FP32 accumulate */
+
+/* raster pixel types */
+#define AMPCOR_HIP_COMPLEX64 0 /* SLC: interleaved float re, im */
+#define AMPCOR_HIP_FLOAT32 1   /* real amplitude */
+
+typedef struct ampcor_hip_raster ampcor_hip_raster;
+typedef struct ampcor_hip_plan ampcor_hip_plan;
+
+/* description of the most recent failure on this thread ("" if none) */
+const char *ampcor_hip_last_error(void);
+/* number of visible devices */
+int ampcor_hip_device_count(int *count);
+
+/* -- rasters --------------------------------------------------------------- */
+
+/* copy a host raster (lines x samples, row-major) into device memory */
+int ampcor_hip_raster_create(int device, int dtype, const void *host, size_t lines,
+                             size_t samples, ampcor_hip_raster **out);
+/* wrap device memory the caller owns and keeps alive; nothing is copied */
+int ampcor_hip_raster_wrap(int device, int dtype, const void *device_ptr, size_t lines,
+                           size_t samples, ampcor_hip_raster **out);
+int ampcor_hip_raster_shape(const ampcor_hip_raster *raster, size_t *lines, size_t *samples);
+int ampcor_hip_raster_destroy(ampcor_hip_raster *raster);
+
+/* -- plans ----------------------------------------------------------------- */
+
+/*
+ * A plan for `pairs` tile pairs: reference chips of ref_lines x ref_samples,
+ * target windows of tgt_lines x tgt_samples (both square, window - chip even,
+ * (window - chip) / 2 >= refine_margin).  Device buffers are allocated here
+ * and reused by every adjust.  A plan is not thread-safe.
+ */
+int ampcor_hip_plan_create(int device, size_t pairs, size_t ref_lines, size_t ref_samples,
+                           size_t tgt_lines, size_t tgt_samples, size_t refine_margin,
+                           size_t refine_factor, size_t zoom_factor, int precision,
+                           ampcor_hip_plan **out);
+int ampcor_hip_plan_destroy(ampcor_hip_plan *plan);
+/* number of pairs the plan was created for */
+int ampcor_hip_plan_pairs(const ampcor_hip_plan *plan, size_t *pairs);
+
+/* register the reference chip of pair idx: raster rows [begin_line, end_line),
+ * columns [begin_sample, end_sample); the extent must equal the chip shape */
+int ampcor_hip_add_reference(ampcor_hip_plan *plan, const ampcor_hip_raster *raster, size_t idx,
+                             size_t begin_line, size_t begin_sample, size_t end_line,
+                             size_t end_sample);
+/* register the target search window of pair idx */
+int ampcor_hip_add_target(ampcor_hip_plan *plan, const ampcor_hip_raster *raster, size_t idx,
+                          size_t begin_line, size_t begin_sample, size_t end_line,
+                          size_t end_sample);
+/* register pairs [first, first + count) at once: ref_begin / tgt_begin hold
+ * count (line, sample) upper-left corners */
+int ampcor_hip_add_pairs(ampcor_hip_plan *plan, const ampcor_hip_raster *reference,
+                         const ampcor_hip_raster *target, size_t first, size_t count,
+                         const int64_t *ref_begin, const int64_t *tgt_begin);
+
+/* correlate every registered pair; offsets (host, 2 * pairs floats, pair-major
+ * (dLine, dSample)) may be NULL to leave the products on the device */
+int ampcor_hip_adjust(ampcor_hip_plan *plan, float *offsets);
+
+/* copy the products of the last adjust to host arrays (any may be NULL):
+ * offsets [pairs][2], coarse correlation peak [pairs], zoomed refined peak
+ * [pairs], coarse SNR [pairs] */
+int ampcor_hip_products(ampcor_hip_plan *plan, float *offsets, float *coarse_peak,
+                        float *refined_peak, float *snr);
+
+/* -- stage access, for tests and profiling ---------------------------------- */
+
+/* surface dimensions: coarse P = window - chip + 1, refined R = 2 rf rm + 1 */
+int ampcor_hip_plan_dims(const ampcor_hip_plan *plan, int *coarse_dim, int *refined_dim,
+                         int *zoomed_dim);
+/* coarse correlation surfaces [pairs][P][P] */
+int ampcor_hip_stage_gamma(ampcor_hip_plan *plan, float *gamma);
+/* refined correlation surfaces [pairs][R][R] */
+int ampcor_hip_stage_gamma_refined(ampcor_hip_plan *plan, float *gamma);
+/* integer locations [pairs][2]: coarse argmax (row, col), nudged window
+ * (top, left), zoomed argmax (p, q) */
+int ampcor_hip_stage_locations(ampcor_hip_plan *plan, int32_t *coarse, int32_t *window,
+                               int32_t *zoomed);
+/* per-stage device time of the last adjust, milliseconds, measured with HIP
+ * events on the plan's stream; names[i] point to static strings.  Timing is
+ * off until enabled, because events serialize the stream. */
+int ampcor_hip_set_timing(ampcor_hip_plan *plan, int enabled);
+int ampcor_hip_stage_times(ampcor_hip_plan *plan, int capacity, int *count, float *ms,
+                           const char **names);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

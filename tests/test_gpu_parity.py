@@ -1,0 +1,111 @@
+"""Parity of the HIP correlator with the CPU oracle (runs on the GPU).
+
+Bar: coarse surfaces, coarse peaks and nudged windows bit-identical to the
+oracle's FP32 twin; refined offsets within 1e-3 px, peaks and SNR within 1e-5
+relative of the FP64 oracle.
+"""
+
+import numpy as np
+import pytest
+
+from ampcor_amd import synth
+from oracle import pyoracle
+from tests.gpu_helpers import run_hip
+
+pytestmark = pytest.mark.gpu
+
+OFFSET_TOL = 1e-3   # px, refined offsets vs the FP64 oracle
+REL_TOL = 1e-5      # correlation peaks and SNR vs the FP64 oracle
+
+
+def oracles(s, reference=None, target=None):
+    reference = s.reference if reference is None else reference
+    target = s.target if target is None else target
+    args = (reference, target, s.ref_begin, s.tgt_begin, s.chip, s.window, s.refine_factor,
+            s.refine_margin, s.zoom_factor)
+    return (pyoracle.adjust(*args, form="fp32", surfaces=True),
+            pyoracle.adjust(*args, form="fp64", surfaces=True))
+
+
+def check_against_oracles(hip, twin, exact):
+    # coarse stage: bit-identical to the FP32 twin
+    assert np.array_equal(hip["gamma"].view(np.uint32), twin["gamma"].view(np.uint32))
+    assert np.array_equal(hip["coarse_loc"], twin["coarse_loc"])
+    assert np.array_equal(hip["window_loc"], twin["window_loc"])
+    assert np.array_equal(hip["coarse_peak"].view(np.uint32), twin["coarse_peak"].view(np.uint32))
+    # refinement and products: FP64 oracle
+    np.testing.assert_allclose(hip["offsets"], exact["offsets"], rtol=0, atol=OFFSET_TOL)
+    np.testing.assert_allclose(hip["coarse_peak"], exact["coarse_peak"], rtol=REL_TOL)
+    np.testing.assert_allclose(hip["refined_peak"], exact["refined_peak"], rtol=REL_TOL)
+    np.testing.assert_allclose(hip["snr"], exact["snr"], rtol=REL_TOL)
+    # the tuple adjust returns is the offsets product
+    assert np.array_equal(hip["tuple_offsets"], hip["offsets"])
+
+
+def test_shifted_speckle_matches_oracles(lib):
+    s = synth.shifted_scene()
+    print(f"shifted scene: {s.pairs} pairs", flush=True)
+    hip = run_hip(lib, s.reference, s.target, s.ref_begin, s.tgt_begin, s.chip, s.window,
+                  s.refine_factor, s.refine_margin, s.zoom_factor)
+    twin, exact = oracles(s)
+    check_against_oracles(hip, twin, exact)
+    # and the known shift is recovered
+    assert np.max(np.abs(hip["offsets"] - s.truth)) < 0.05
+
+
+def test_per_pair_calls_equal_batched(lib):
+    s = synth.shifted_scene(seed=3, size=256, grid=(4, 4))
+    a = run_hip(lib, s.reference, s.target, s.ref_begin, s.tgt_begin, s.chip, s.window,
+                s.refine_factor, s.refine_margin, s.zoom_factor, per_pair=True)
+    b = run_hip(lib, s.reference, s.target, s.ref_begin, s.tgt_begin, s.chip, s.window,
+                s.refine_factor, s.refine_margin, s.zoom_factor)
+    for k in ("offsets", "gamma", "gamma_refined", "snr"):
+        assert np.array_equal(a[k].view(np.uint32), b[k].view(np.uint32)), k
+
+
+@pytest.mark.parametrize("chip,margin", [(16, 4), (32, 8)])
+def test_planted_chip(lib, chip, margin):
+    """tests/libampcor_cuda/correlate.cc:283-306 and maxcor.cc:296-315: gamma = 1
+    at the planted placement (10 eps) and the argmax is exactly that placement."""
+    ref, tgt, rb, tb = synth.planted(chip, margin, seed=chip)
+    P = 2 * margin + 1
+    rm = min(margin, 4)
+    hip = run_hip(lib, ref, tgt, rb, tb, chip, chip + 2 * margin, 2, rm, 4)
+    eps = np.finfo(np.float32).eps
+    for i in range(P):
+        for j in range(P):
+            k = i * P + j
+            assert abs(hip["gamma"][k, i, j] - 1.0) <= 10 * eps
+            assert tuple(hip["coarse_loc"][k]) == (i, j)
+    # nudge bounds (nudge.cc:201-227)
+    E = chip + 2 * rm
+    assert np.all(hip["window_loc"] >= 0) and np.all(hip["window_loc"] <= chip + 2 * margin - E)
+    # refined offsets agree with the FP64 oracle and land on the planted placement
+    exact = pyoracle.adjust(ref, tgt, rb, tb, chip, chip + 2 * margin, 2, rm, 4, form="fp64")
+    np.testing.assert_allclose(hip["offsets"], exact["offsets"], rtol=0, atol=OFFSET_TOL)
+    truth = np.array([(i - margin, j - margin) for i in range(P) for j in range(P)], np.float32)
+    assert np.max(np.abs(hip["offsets"] - truth)) < 0.1
+
+
+def test_field_scene_subset(lib):
+    """Config-2 geometry (chip 64, window 128, zoom 8) on a smaller raster."""
+    s = synth.field_scene(seed=5, lines=1024, samples=1024, grid=(12, 12), range_ramp=3.0,
+                          azimuth_quadratic=1.5)
+    print(f"field subset: {s.pairs} pairs", flush=True)
+    hip = run_hip(lib, s.reference, s.target, s.ref_begin, s.tgt_begin, s.chip, s.window,
+                  s.refine_factor, s.refine_margin, s.zoom_factor)
+    twin, exact = oracles(s)
+    check_against_oracles(hip, twin, exact)
+    assert np.max(np.abs(hip["offsets"] - s.truth)) < 0.1
+
+
+def test_float32_amplitude_rasters(lib):
+    rng = np.random.default_rng(11)
+    ref = synth.lognormal_texture((384, 384), 0.5, rng)
+    tgt = np.real(synth.fourier_shift(ref, 1.5, 2.25)).astype(np.float32)
+    tgt = tgt * (1 + 0.05 * rng.standard_normal(tgt.shape)).astype(np.float32)
+    s = synth.Scene("amp", ref, tgt, 32, 16, 2, 8, 8, (4, 4))
+    s.plan()
+    hip = run_hip(lib, ref, tgt, s.ref_begin, s.tgt_begin, s.chip, s.window, 2, 8, 8)
+    twin, exact = oracles(s)
+    check_against_oracles(hip, twin, exact)

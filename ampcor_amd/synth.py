@@ -7,6 +7,7 @@ Also the plan geometry of the standard workloads (a uniform grid of tile
 centers, constant zero coarse shift), restated as arithmetic.
 """
 
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -51,18 +52,38 @@ def _sinc_weights(frac, taps):
     return (w / w.sum(axis=1, keepdims=True)).astype(np.float32), k.astype(np.int64)
 
 
-def resample_axis(x, shift, axis, taps=16):
-    """out[..., i, ...] = x interpolated at i - shift[i] along ``axis`` (edges clamped)."""
+def resample_axis(x, shift, axis, taps=16, threads=None):
+    """out[..., i, ...] = x interpolated at i - shift[i] along ``axis`` (edges clamped).
+
+    Runs over bands of the other axis in a thread pool (NumPy releases the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+
     n = x.shape[axis]
     pos = np.arange(n, dtype=np.float64) - shift
     base = np.floor(pos).astype(np.int64)
     w, k = _sinc_weights(pos - base, taps)
-    xm = np.moveaxis(x, axis, 0)
-    out = np.zeros_like(xm)
-    for t in range(len(k)):
-        idx = np.clip(base + k[t], 0, n - 1)
-        out += w[:, t].reshape((-1,) + (1,) * (xm.ndim - 1)) * xm[idx]
-    return np.moveaxis(out, 0, axis)
+    idx = [np.clip(base + k[t], 0, n - 1) for t in range(len(k))]
+    out = np.empty_like(x)
+    other = x.shape[1 - axis]
+    threads = threads or min(16, len(os.sched_getaffinity(0)))
+    step = max(64, -(-other // (4 * threads)))
+
+    def band(lo):
+        hi = min(lo + step, other)
+        if axis == 0:
+            xs, acc = x[:, lo:hi], np.zeros((n, hi - lo), dtype=x.dtype)
+            for t in range(len(k)):
+                acc += w[:, t, None] * xs[idx[t]]
+            out[:, lo:hi] = acc
+        else:
+            xs, acc = x[lo:hi], np.zeros((hi - lo, n), dtype=x.dtype)
+            for t in range(len(k)):
+                acc += w[None, :, t] * xs[:, idx[t]]
+            out[lo:hi] = acc
+
+    with ThreadPoolExecutor(threads) as pool:
+        list(pool.map(band, range(0, other, step)))
+    return out
 
 
 def lognormal_texture(shape, sigma, rng):

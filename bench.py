@@ -1,0 +1,156 @@
+"""Throughput of the HIP tile correlator on one MI355X.
+
+Workload: the scene-sized FP32 case -- 8192 x 4096 complex64 speckle, the
+secondary resampled by a smooth field (range ramp 0-5 px, azimuth quadratic
+up to 2 px) plus 15 dB noise; a 120 x 60 grid of pairs (6844 valid under the
+plan geometry), chip 64, window 128, refine factor 2, refine margin 8, zoom 8.
+A step is one adjust over all pairs with both rasters already resident in
+device memory: gather + detect, chip statistics, FP64 SATs, the contraction,
+normalization, peak/SNR/nudge, upsampling, refined correlation, zoom,
+sub-pixel offsets, and the copy of the offsets to the host.
+
+Prints one JSON line.  The roofline figure is for the dominant kernel, timed
+with HIP events on the plan's stream during the timed steps; the CPU baseline
+is the FP64 oracle on a bounded sample of the same pairs.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "tile pairs/sec @64²/128² zoom8, mixed vs FP32 at ≤0.01px; p99 job latency"
+FP32_PEAK_TFLOPS = 157.3   # MI355X dense FP32 (matrix) peak
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--precision", default="fp32")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="target duration of the CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    return ap.parse_args()
+
+
+def cpu_baseline(scene, seconds):
+    """FP64 oracle on the host cores over a bounded sample of the same pairs."""
+    from oracle import pyoracle
+
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    args = (scene.reference, scene.target)
+    geo = (scene.chip, scene.window, scene.refine_factor, scene.refine_margin, scene.zoom_factor)
+
+    def run(n):
+        idx = np.linspace(0, scene.pairs - 1, n).astype(int)
+        t0 = time.perf_counter()
+        pyoracle.adjust(*args, scene.ref_begin[idx], scene.tgt_begin[idx], *geo, form="fp64",
+                        threads=cores)
+        return time.perf_counter() - t0
+
+    n = 2 * cores
+    dt = run(n)
+    n2 = int(min(scene.pairs, max(n, n * seconds / max(dt, 1e-3))))
+    n2 = max(cores, (n2 // cores) * cores)
+    dt = run(n2)
+    return {"value": n2 / dt, "unit": "tile pairs/sec", "cores": cores, "kind": "port",
+            "sample": f"FP64 oracle (oracle/ampcor_oracle.c, OpenMP over pairs) on {n2} of the "
+                      f"{scene.pairs} pairs, evenly spaced, {dt:.1f} s"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {a.gpus}); pairs are independent, so "
+              "several GPUs would each run this workload on their share of the tile grid",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from ampcor_amd import ext, synth
+
+    lib = ext.require()
+    if lib.device_count() < 1:
+        raise RuntimeError("no HIP device visible")
+
+    t0 = time.perf_counter()
+    scene = synth.field_scene()
+    print(f"[bench] scene ready in {time.perf_counter() - t0:.1f} s: {scene.pairs} pairs",
+          file=sys.stderr, flush=True)
+    N, C, W = scene.pairs, scene.chip, scene.window
+    worker = lib.sequential(N, (C, C), (W, W), scene.refine_margin, scene.refine_factor,
+                            scene.zoom_factor, precision=a.precision)
+    ref = lib.raster(scene.reference, *scene.reference.shape)
+    tgt = lib.raster(scene.target, *scene.target.shape)
+    lib.addPairs(worker, ref, tgt, 0, scene.ref_begin.tobytes(), scene.tgt_begin.tobytes())
+
+    for _ in range(a.warmup):
+        lib.adjust(worker)
+    lib.set_timing(worker, True)
+    stage_sum = {}
+    elapsed = 0.0
+    for _ in range(a.steps):
+        t = time.perf_counter()
+        lib.adjust(worker)               # synchronizes the plan stream before returning
+        elapsed += time.perf_counter() - t
+        for k, v in lib.stage_times(worker).items():
+            stage_sum[k] = stage_sum.get(k, 0.0) + v
+    stages = {k: v / a.steps for k, v in stage_sum.items()}
+    ms_per_step = 1e3 * elapsed / a.steps
+    value = N / (elapsed / a.steps)
+    print("[bench] stage ms/step: " + ", ".join(f"{k} {v:.3f}" for k, v in stages.items()),
+          file=sys.stderr, flush=True)
+
+    # accuracy of this run against the truth field (informational)
+    off = np.frombuffer(lib.products(worker)["offsets"], np.float32).reshape(N, 2)
+    err = np.abs(off - scene.truth)
+    print(f"[bench] |offset - true field|: median {np.median(err):.4f} px, p99 "
+          f"{np.percentile(err, 99):.4f} px", file=sys.stderr, flush=True)
+
+    # dominant kernel: the larger of the two contractions
+    P, R = W - C + 1, 2 * scene.refine_factor * scene.refine_margin + 1
+    Cr = scene.refine_factor * C
+    flops = {"correlate": 2.0 * C * C * P * P * N, "correlate_refined": 2.0 * Cr * Cr * R * R * N}
+    dom = max(flops, key=lambda k: stages[k])
+    achieved = flops[dom] / (stages[dom] * 1e-3) / 1e12
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"{dom}_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as fh:
+            traffic = json.load(fh).get("bytes_per_launch")
+    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+                "traffic": traffic, "kernel": dom, "kernel_ms": round(stages[dom], 4)}
+
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(scene, a.cpu_seconds)
+        print(f"[bench] cpu baseline: {cpu}", file=sys.stderr, flush=True)
+
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "tile pairs/sec", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if a.precision == "fp32" else a.precision, "data": "synthetic",
+        "config": {"workload": "scene 8192x4096 complex64 speckle, smooth-field secondary +15 dB, "
+                               "120x60 grid (6844 valid pairs), chip 64, window 128, refine x2 "
+                               "margin 8, zoom 8",
+                   "pairs": N, "chip": C, "window": W, "refine_factor": scene.refine_factor,
+                   "refine_margin": scene.refine_margin, "zoom_factor": scene.zoom_factor,
+                   "precision": a.precision},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -14,8 +14,9 @@ def run_hip(lib, reference, target, ref_begin, tgt_begin, chip, window, rf, rm, 
     tb = np.ascontiguousarray(tgt_begin, dtype=np.int64)
     if per_pair:
         for k in range(n):
-            lib.addReference(worker, ref, k, tuple(rb[k]), tuple(rb[k] + chip))
-            lib.addTarget(worker, tgt, k, tuple(tb[k]), tuple(tb[k] + window))
+            r0, t0 = tuple(int(v) for v in rb[k]), tuple(int(v) for v in tb[k])
+            lib.addReference(worker, ref, k, r0, (r0[0] + chip, r0[1] + chip))
+            lib.addTarget(worker, tgt, k, t0, (t0[0] + window, t0[1] + window))
     else:
         lib.addPairs(worker, ref, tgt, 0, rb.tobytes(), tb.tobytes())
     offsets = np.array(lib.adjust(worker), dtype=np.float32).reshape(n, 2)

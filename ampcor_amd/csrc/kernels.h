@@ -31,28 +31,32 @@ struct CorrelateShape {
 };
 
 // chooses the tiling for chip C over P x P placements; exact = keep each
-// placement's whole contraction in one lane (the FP32 drop-in order)
-CorrelateShape choose_correlate_shape(int C, int P, bool exact);
+// placement's whole contraction in one lane (the FP32 drop-in order);
+// bu / nsplit > 0 request a register block / chip-column split (tuning)
+CorrelateShape choose_correlate_shape(int C, int P, bool exact, int bu = 0, int nsplit = 0);
 
 hipError_t launch_gather_detect(hipStream_t s, const PairDesc *desc, int pairs, int dtype, int C,
                                 int W, float *ref_amp, float *tgt_amp);
 hipError_t launch_chip_stats(hipStream_t s, const float *amp, int pairs, int C, float *rT,
                              double2 *stats);
-hipError_t launch_sat(hipStream_t s, const float *amp, int pairs, int W, double *S1, double *S2);
+// FP64 sums of t and t^2 over every C x C placement of each W x W window: [pairs][P][P]
+hipError_t launch_window_sums(hipStream_t s, const float *amp, int pairs, int W, int C,
+                              double2 *sums);
 hipError_t launch_correlate(hipStream_t s, const CorrelateShape &shape, const float *rT,
                             const float *amp, int pairs, int C, int W, float *partial);
-hipError_t launch_normalize(hipStream_t s, const float *partial, int nsplit, const double *S1,
-                            const double *S2, const double2 *stats, int pairs, int C, int W,
-                            float *gamma);
+hipError_t launch_normalize(hipStream_t s, const float *partial, int nsplit, const double2 *sums,
+                            const double2 *stats, int pairs, int C, int W, float *gamma);
 hipError_t launch_coarse_peak(hipStream_t s, const float *gamma, int pairs, int P, int W, int E,
                               int refine_margin, int exclude, int2 *loc, int2 *window,
                               float *peak, float *snr);
-// band-limited upsampling of complex tiles straight from the rasters:
+// band-limited upsampling of complex tiles straight from the rasters, detected:
 // role 0 = reference chips (N = C, origin = chip corner), role 1 = target
-// windows (N = E, origin = window corner + nudged (top, left))
+// windows (N = E, origin = window corner + nudged (top, left)); U is the
+// (rf N) x N upsampling matrix stored padded to multiples of 16, zero-filled;
+// g (rf = 2 only) is its half-sample row, U[2a + 1][n] = g((a - n) mod N)
 hipError_t launch_upsample(hipStream_t s, const PairDesc *desc, int pairs, int dtype, int role,
-                           const int2 *window, int N, int rf, const float *U, const float *Ut,
-                           float2 *work, float *amp);
+                           const int2 *window, int N, int rf, const float *U, const float *g,
+                           float *amp);
 hipError_t launch_zoom(hipStream_t s, const float *gamma_refined, int pairs, int R, int zf,
                        const double *dirichlet, const int2 *window, const float *coarse_peak,
                        int margin, int rf, float *offsets, float *refined_peak, int2 *zoom_loc);

@@ -1,10 +1,11 @@
 // kernels.hip -- gfx950 kernels of the batched tile correlator.
 //
-// The coarse stage (gather/detect, chip statistics, SAT, contraction,
+// The coarse stage (gather/detect, chip statistics, window sums, contraction,
 // normalization, peak) rounds exactly where the CPU oracle's FP32 twin does, so
 // its correlation surfaces are bit-identical: FP64 sums run in fixed orders
-// (a 256-way strided pass plus a halving tree for chips, a per-lane chunked
-// Kogge-Stone scan for SATs), the contraction is a fused multiply-add chain in
+// (a 256-way strided pass plus a halving tree for chips, sliding column boxes
+// and a chunked Kogge-Stone row scan for windows), the contraction is a fused
+// multiply-add chain in
 // a fixed (chip column outer, chip row inner) order, and nothing is contracted
 // implicitly (the library is built with -ffp-contract=off).
 //
@@ -86,6 +87,7 @@ __global__ __launch_bounds__(kStatThreads) void k_chip_stats(const float *__rest
 {
     __shared__ double p1[kStatThreads];
     __shared__ double p2[kStatThreads];
+    __shared__ float tile[32][33];
     const int pair = blockIdx.x;
     const int t = threadIdx.x;
     const int cells = C * C;
@@ -98,14 +100,10 @@ __global__ __launch_bounds__(kStatThreads) void k_chip_stats(const float *__rest
 
     double a1 = 0.0, a2 = 0.0;
     for (int k = t; k < cells; k += kStatThreads) {
-        const float r = (float)((double)a[k] - mean);
-        const int i = k / C, j = k - i * C;
-        out[(size_t)j * C + i] = r;
-        const double dr = (double)r;
+        const double dr = (double)(float)((double)a[k] - mean);
         a1 = a1 + dr;
         a2 = a2 + dr * dr;
     }
-    // both trees share the barrier sequence; p2 is a second buffer
     p2[t] = a2;
     const double s1 = tree256(p1, a1);
     for (int s = kStatThreads / 2; s > 0; s >>= 1) {
@@ -113,64 +111,108 @@ __global__ __launch_bounds__(kStatThreads) void k_chip_stats(const float *__rest
         __syncthreads();
     }
     if (t == 0) stats[pair] = make_double2(s1, sqrt(p2[0]));
+
+    // the zero-mean chip, transposed through LDS in 32 x 32 blocks
+    const int c = t & 31, r0 = t >> 5;
+    for (int bi = 0; bi < C; bi += 32)
+        for (int bj = 0; bj < C; bj += 32) {
+            for (int r = r0; r < 32; r += kStatThreads / 32) {
+                const int i = bi + r, j = bj + c;
+                tile[r][c] = (i < C && j < C) ? (float)((double)a[(size_t)i * C + j] - mean) : 0.0f;
+            }
+            __syncthreads();
+            for (int r = r0; r < 32; r += kStatThreads / 32) {
+                const int j = bj + r, i = bi + c;
+                if (i < C && j < C) out[(size_t)j * C + i] = tile[c][r];
+            }
+            __syncthreads();
+        }
 }
 
 // ---------------------------------------------------------------------------
-// FP64 summed-area tables of t and t^2, zero-bordered (W+1)^2, one wavefront
-// per pair: each row is scanned across lanes (Kogge-Stone, 64-column chunks
-// with a running carry), and each lane accumulates its columns downward
+// window sums of t and t^2 over every C x C placement, FP64, one wavefront per
+// pair.  Each lane keeps the vertical box sums V(u, x) = sum_{i<C} t(u+i, x) of
+// its columns and slides them down one row per placement row u
+// (V <- (V + t(u+C-1, x)) - t(u-1, x)); each row of V then gets an inclusive
+// Kogge-Stone scan across lanes (64-column chunks with a running carry), i.e.
+// one row of a summed-area table, and the window sum is the difference of two
+// of its entries.
 // ---------------------------------------------------------------------------
-template <int Q>
-__global__ __launch_bounds__(256) void k_sat(const float *__restrict__ amp, int W, int pairs,
-                                             double *__restrict__ S1, double *__restrict__ S2)
-{
-    const int lane = threadIdx.x & (kWave - 1);
-    const int pair = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (pair >= pairs) return;
-    const int sc = W + 1;
-    const size_t ss = (size_t)sc * sc;
-    const float *t = amp + (size_t)pair * W * W;
-    double *s1 = S1 + pair * ss;
-    double *s2 = S2 + pair * ss;
-    for (int x = lane; x < sc; x += kWave) {
-        s1[x] = 0.0;
-        s2[x] = 0.0;
-    }
-    double c1[Q], c2[Q];
-#pragma unroll
-    for (int q = 0; q < Q; ++q) c1[q] = c2[q] = 0.0;
+constexpr int kMaxWindow = 512;
 
-    for (int k = 0; k < W; ++k) {
-        if (lane == 0) {
-            s1[(size_t)(k + 1) * sc] = 0.0;
-            s2[(size_t)(k + 1) * sc] = 0.0;
+__device__ inline void wave_inclusive_scan(double &v1, double &v2, int lane)
+{
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const double o1 = __shfl_up(v1, d, kWave);
+        const double o2 = __shfl_up(v2, d, kWave);
+        if (lane >= d) {
+            v1 = o1 + v1;
+            v2 = o2 + v2;
+        }
+    }
+}
+
+template <int Q>
+__global__ __launch_bounds__(kWave) void k_window_sums(const float *__restrict__ amp, int W, int C,
+                                                       double2 *__restrict__ sums)
+{
+    __shared__ double sx1[kMaxWindow];
+    __shared__ double sx2[kMaxWindow];
+    const int lane = threadIdx.x;
+    const int pair = blockIdx.x;
+    const int P = W - C + 1;
+    const float *t = amp + (size_t)pair * W * W;
+    double2 *out = sums + (size_t)pair * P * P;
+
+    double V1[Q], V2[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int x = lane + kWave * q;
+        V1[q] = V2[q] = 0.0;
+        if (x < W)
+            for (int i = 0; i < C; ++i) {
+                const double d = (double)t[(size_t)i * W + x];
+                V1[q] = V1[q] + d;
+                V2[q] = V2[q] + d * d;
+            }
+    }
+    for (int u = 0; u < P; ++u) {
+        if (u > 0) {
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const int x = lane + kWave * q;
+                if (x < W) {
+                    const double a = (double)t[(size_t)(u + C - 1) * W + x];
+                    const double b = (double)t[(size_t)(u - 1) * W + x];
+                    V1[q] = (V1[q] + a) - b;
+                    V2[q] = (V2[q] + a * a) - b * b;
+                }
+            }
         }
         double carry1 = 0.0, carry2 = 0.0;
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             const int x = lane + kWave * q;
-            double v1 = (x < W) ? (double)t[(size_t)k * W + x] : 0.0;
-            double v2 = v1 * v1;
-#pragma unroll
-            for (int d = 1; d < kWave; d <<= 1) {
-                const double o1 = __shfl_up(v1, d, kWave);
-                const double o2 = __shfl_up(v2, d, kWave);
-                if (lane >= d) {
-                    v1 = o1 + v1;
-                    v2 = o2 + v2;
-                }
-            }
+            double v1 = (x < W) ? V1[q] : 0.0;
+            double v2 = (x < W) ? V2[q] : 0.0;
+            wave_inclusive_scan(v1, v2, lane);
             v1 = carry1 + v1;
             v2 = carry2 + v2;
             carry1 = __shfl(v1, kWave - 1, kWave);
             carry2 = __shfl(v2, kWave - 1, kWave);
-            c1[q] = c1[q] + v1;
-            c2[q] = c2[q] + v2;
             if (x < W) {
-                s1[(size_t)(k + 1) * sc + x + 1] = c1[q];
-                s2[(size_t)(k + 1) * sc + x + 1] = c2[q];
+                sx1[x] = v1;
+                sx2[x] = v2;
             }
         }
+        __syncthreads();
+        for (int v = lane; v < P; v += kWave) {
+            const double s1 = sx1[v + C - 1] - (v > 0 ? sx1[v - 1] : 0.0);
+            const double s2 = sx2[v + C - 1] - (v > 0 ? sx2[v - 1] : 0.0);
+            out[(size_t)u * P + v] = make_double2(s1, s2);
+        }
+        __syncthreads();
     }
 }
 
@@ -256,11 +298,9 @@ __global__ __launch_bounds__(1024) void k_correlate(const float *__restrict__ rT
 // from the FP64 SATs; NaN where the window or the chip has no variance
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_normalize(const float *__restrict__ partial, int nsplit,
-                                                   const double *__restrict__ S1,
-                                                   const double *__restrict__ S2,
+                                                   const double2 *__restrict__ sums,
                                                    const double2 *__restrict__ stats, int C,
-                                                   int W, int P, int pairs,
-                                                   float *__restrict__ gamma)
+                                                   int P, int pairs, float *__restrict__ gamma)
 {
     const size_t total = (size_t)pairs * P * P;
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -268,7 +308,6 @@ __global__ __launch_bounds__(256) void k_normalize(const float *__restrict__ par
     const int pp = P * P;
     const int pair = (int)(idx / pp);
     const int c = (int)(idx - (size_t)pair * pp);
-    const int u = c / P, v = c - u * P;
 
     // fixed pairwise tree over the chip-column partials
     float part[16];
@@ -277,16 +316,10 @@ __global__ __launch_bounds__(256) void k_normalize(const float *__restrict__ par
         for (int k = 0; k < w / 2; ++k) part[k] = part[2 * k] + part[2 * k + 1];
     const float num = part[0];
 
-    const int sc = W + 1;
-    const size_t base = (size_t)pair * sc * sc;
-    const double *a1 = S1 + base, *a2 = S2 + base;
-    const size_t tl = (size_t)u * sc + v, tr = (size_t)u * sc + v + C;
-    const size_t bl = (size_t)(u + C) * sc + v, br = (size_t)(u + C) * sc + v + C;
-    const double s1 = ((a1[br] - a1[tr]) - a1[bl]) + a1[tl];
-    const double s2 = ((a2[br] - a2[tr]) - a2[bl]) + a2[tl];
+    const double2 ws = sums[idx];
     const double n = (double)C * C;
-    const double mu = s1 / n;
-    const double var = s2 - mu * s1;
+    const double mu = ws.x / n;
+    const double var = ws.y - mu * ws.x;
     const double2 st = stats[pair];
     const double A = mu * st.x;
     const double B = (var > 0.0 && st.y > 0.0) ? st.y * sqrt(var) : 0.0;
@@ -376,27 +409,31 @@ __global__ __launch_bounds__(256) void k_coarse_peak(const float *__restrict__ g
 }
 
 // ---------------------------------------------------------------------------
-// band-limited upsampling y = U x U^T of complex N x N tiles (U real,
-// M = rf N rows; rows rf*n are the identity and are copied)
-// pass 1: T = x U^T, a workgroup per (pair, 8 tile rows), read from the raster
-// pass 2: y = U T, a workgroup per (pair, 32 columns), detected on the way out
+// band-limited upsampling y = U x U^T of complex N x N tiles on the matrix
+// cores (FP32 MFMA 16x16x4).  U is real, M x N with M = rf N, stored padded to
+// Mp x Np (multiples of 16, zero-filled).  One workgroup per pair; each wave
+// owns 16-row strips of the output: T = U[strip, :] x (from the raster, both
+// planes) into LDS, then y[strip, :] = T U^T, detected on the way out.
+//
+// MFMA 16x16x4 f32 operand layout (lane l): A[m = l % 16][k = l / 16],
+// B[k = l / 16][n = l % 16], D[m = 4 (l / 16) + i][n = l % 16], i = 0..3.
 // ---------------------------------------------------------------------------
-constexpr int kUpRows = 8;
-constexpr int kUpCols = 32;
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+constexpr int kUpWaves = 4;
 
-__global__ __launch_bounds__(256) void k_upsample_rows(const PairDesc *__restrict__ desc,
-                                                       int dtype, int role,
-                                                       const int2 *__restrict__ window, int N,
-                                                       int rf, const float *__restrict__ Ut,
-                                                       float2 *__restrict__ work)
+template <int DTYPE>
+__global__ __launch_bounds__(kUpWaves * kWave) void k_upsample(
+    const PairDesc *__restrict__ desc, int role, const int2 *__restrict__ window, int N, int Np,
+    int M, int Mp, const float *__restrict__ U, float *__restrict__ amp)
 {
-    extern __shared__ float2 xs[];  // kUpRows x N
-    const int pair = blockIdx.y;
-    const int n0 = blockIdx.x * kUpRows;
-    const int M = rf * N;
+    extern __shared__ float tstrip[];  // [wave][plane][16][Np]
+    const int pair = blockIdx.x;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x >> 6;
+    const int lm = lane & 15, lk = lane >> 4;
     const PairDesc d = desc[pair];
     const void *base = role == 0 ? d.ref : d.tgt;
-    const int stride = role == 0 ? d.ref_stride : d.tgt_stride;
+    const size_t stride = role == 0 ? d.ref_stride : d.tgt_stride;
     int line = role == 0 ? d.ref_line : d.tgt_line;
     int sample = role == 0 ? d.ref_sample : d.tgt_sample;
     if (role == 1) {
@@ -404,71 +441,181 @@ __global__ __launch_bounds__(256) void k_upsample_rows(const PairDesc *__restric
         line += tl.x;
         sample += tl.y;
     }
-    const int nrows = min(kUpRows, N - n0);
-    for (int k = threadIdx.x; k < nrows * N; k += blockDim.x) {
-        const int r = k / N, c = k - r * N;
-        const size_t off = (size_t)(line + n0 + r) * stride + sample + c;
-        float2 z;
-        if (dtype == 0) z = static_cast<const float2 *>(base)[off];
-        else z = make_float2(static_cast<const float *>(base)[off], 0.0f);
-        xs[k] = z;
-    }
-    __syncthreads();
-    float2 *out = work + (size_t)pair * N * M;
-    for (int k = threadIdx.x; k < nrows * M; k += blockDim.x) {
-        const int r = k / M, q = k - r * M;
-        float2 acc;
-        if (q % rf == 0) {
-            acc = xs[r * N + q / rf];
-        } else {
-            float re = 0.0f, im = 0.0f;
-            for (int m = 0; m < N; ++m) {
-                const float u = Ut[(size_t)m * M + q];
-                const float2 x = xs[r * N + m];
-                re = fmaf(x.x, u, re);
-                im = fmaf(x.y, u, im);
+    float *Tr = tstrip + (size_t)wave * 2 * 16 * Np;
+    float *Ti = Tr + 16 * Np;
+    float *out = amp + (size_t)pair * M * M;
+
+    for (int strip = wave; strip < Mp / 16; strip += kUpWaves) {
+        const int p0 = 16 * strip;
+        // stage 1: T (16 x Np) = U[p0 : p0 + 16, :] x
+        for (int nt = 0; nt < Np / 16; ++nt) {
+            floatx4 accR = {0.0f, 0.0f, 0.0f, 0.0f}, accI = {0.0f, 0.0f, 0.0f, 0.0f};
+            const int n = nt * 16 + lm;
+            for (int k0 = 0; k0 < Np; k0 += 4) {
+                const int k = k0 + lk;
+                const float a = U[(size_t)(p0 + lm) * Np + k];
+                float br = 0.0f, bi = 0.0f;
+                if (k < N && n < N) {
+                    const size_t off = (size_t)(line + k) * stride + sample + n;
+                    if (DTYPE == 0) {
+                        const float2 z = static_cast<const float2 *>(base)[off];
+                        br = z.x;
+                        bi = z.y;
+                    } else {
+                        br = static_cast<const float *>(base)[off];
+                    }
+                }
+                accR = __builtin_amdgcn_mfma_f32_16x16x4f32(a, br, accR, 0, 0, 0);
+                if (DTYPE == 0) accI = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bi, accI, 0, 0, 0);
             }
-            acc = make_float2(re, im);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                Tr[(4 * lk + i) * Np + n] = accR[i];
+                Ti[(4 * lk + i) * Np + n] = (DTYPE == 0) ? accI[i] : 0.0f;
+            }
         }
-        out[(size_t)(n0 + r) * M + q] = acc;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // stage 2: y[p0 : p0 + 16, :] = T U^T
+        for (int qt = 0; qt < Mp / 16; ++qt) {
+            floatx4 accR = {0.0f, 0.0f, 0.0f, 0.0f}, accI = {0.0f, 0.0f, 0.0f, 0.0f};
+            const int q = qt * 16 + lm;
+            for (int k0 = 0; k0 < Np; k0 += 4) {
+                const int k = k0 + lk;
+                const float b = U[(size_t)q * Np + k];
+                accR = __builtin_amdgcn_mfma_f32_16x16x4f32(Tr[lm * Np + k], b, accR, 0, 0, 0);
+                accI = __builtin_amdgcn_mfma_f32_16x16x4f32(Ti[lm * Np + k], b, accI, 0, 0, 0);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int p = p0 + 4 * lk + i;
+                if (p < M && q < M) {
+                    const double re = accR[i], im = accI[i];
+                    out[(size_t)p * M + q] = (float)sqrt(re * re + im * im);
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
-__global__ __launch_bounds__(256) void k_upsample_cols(const float2 *__restrict__ work, int N,
-                                                       int rf, const float *__restrict__ U,
-                                                       float *__restrict__ amp)
+// ---------------------------------------------------------------------------
+// x2 upsampling with the tile resident in LDS (N <= 96).  With rf = 2 the
+// refined rows 2a are the coarse rows and rows 2a + 1 are a half-sample shift,
+// a circulant H[a][n] = g((a - n) mod N), so
+//   y[2a][2b] = x,  y[2a+1][2b] = (H x),  y[2a+s][2b+1] = (T_s H^T),
+// with T_0 = x, T_1 = H x: three N^3 products instead of rf^3 N^3 / ... of
+// the dense form.  8 waves share the output tiles.
+// ---------------------------------------------------------------------------
+constexpr int kUp2Waves = 8;
+constexpr int kUp2MaxN = 96;
+
+__device__ inline float amplitude(float re, float im)
 {
-    extern __shared__ float2 ts[];  // N x kUpCols
-    const int pair = blockIdx.y;
-    const int M = rf * N;
-    const int q0 = blockIdx.x * kUpCols;
-    const int ncols = min(kUpCols, M - q0);
-    const float2 *in = work + (size_t)pair * N * M;
-    for (int k = threadIdx.x; k < N * kUpCols; k += blockDim.x) {
-        const int n = k / kUpCols, c = k - n * kUpCols;
-        ts[k] = (c < ncols) ? in[(size_t)n * M + q0 + c] : make_float2(0.0f, 0.0f);
+    const double r = re, i = im;
+    return (float)sqrt(r * r + i * i);
+}
+
+template <int DTYPE>
+__global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2(
+    const PairDesc *__restrict__ desc, int role, const int2 *__restrict__ window, int N, int Np,
+    const float *__restrict__ g, float *__restrict__ amp)
+{
+    extern __shared__ float lds[];
+    const int ls = Np + 4;
+    float *xr = lds, *xi = xr + Np * ls, *tr = xi + Np * ls, *ti = tr + Np * ls;
+    float *gs = ti + Np * ls;
+    const int pair = blockIdx.x;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x >> 6;
+    const int lm = lane & 15, lk = lane >> 4;
+    const PairDesc d = desc[pair];
+    const void *base = role == 0 ? d.ref : d.tgt;
+    const size_t stride = role == 0 ? d.ref_stride : d.tgt_stride;
+    int line = role == 0 ? d.ref_line : d.tgt_line;
+    int sample = role == 0 ? d.ref_sample : d.tgt_sample;
+    if (role == 1) {
+        const int2 tl = window[pair];
+        line += tl.x;
+        sample += tl.y;
+    }
+    for (int k = threadIdx.x; k < Np * Np; k += blockDim.x) {
+        const int r = k / Np, c = k - r * Np;
+        float re = 0.0f, im = 0.0f;
+        if (r < N && c < N) {
+            const size_t off = (size_t)(line + r) * stride + sample + c;
+            if (DTYPE == 0) {
+                const float2 z = static_cast<const float2 *>(base)[off];
+                re = z.x;
+                im = z.y;
+            } else {
+                re = static_cast<const float *>(base)[off];
+            }
+        }
+        xr[r * ls + c] = re;
+        xi[r * ls + c] = im;
+    }
+    for (int k = threadIdx.x; k < N; k += blockDim.x) gs[k] = g[k];
+    __syncthreads();
+
+    const int M = 2 * N;
+    float *out = amp + (size_t)pair * M * M;
+    for (int k = threadIdx.x; k < N * N; k += blockDim.x) {
+        const int a = k / N, b = k - a * N;
+        out[(size_t)(2 * a) * M + 2 * b] = amplitude(xr[a * ls + b], xi[a * ls + b]);
+    }
+
+    const int tiles = Np / 16;
+    // stage 1: T_1 = H x
+    for (int tile = wave; tile < tiles * tiles; tile += kUp2Waves) {
+        const int mt = tile / tiles, nt = tile - mt * tiles;
+        const int m = 16 * mt + lm;
+        floatx4 accR = {0.0f, 0.0f, 0.0f, 0.0f}, accI = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int k0 = 0; k0 < Np; k0 += 4) {
+            const int k = k0 + lk;
+            int dd = m - k;
+            dd += (dd < 0) ? N : 0;
+            const float a = (k < N && m < N) ? gs[dd] : 0.0f;
+            accR = __builtin_amdgcn_mfma_f32_16x16x4f32(a, xr[k * ls + 16 * nt + lm], accR, 0, 0, 0);
+            if (DTYPE == 0)
+                accI = __builtin_amdgcn_mfma_f32_16x16x4f32(a, xi[k * ls + 16 * nt + lm], accI, 0, 0, 0);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            tr[(16 * mt + 4 * lk + i) * ls + 16 * nt + lm] = accR[i];
+            ti[(16 * mt + 4 * lk + i) * ls + 16 * nt + lm] = (DTYPE == 0) ? accI[i] : 0.0f;
+        }
     }
     __syncthreads();
-    float *out = amp + (size_t)pair * M * M;
-    for (int k = threadIdx.x; k < M * kUpCols; k += blockDim.x) {
-        const int p = k / kUpCols, c = k - p * kUpCols;
-        if (c >= ncols) continue;
-        float2 y;
-        if (p % rf == 0) {
-            y = ts[(p / rf) * kUpCols + c];
-        } else {
-            const float *urow = U + (size_t)p * N;
-            float re = 0.0f, im = 0.0f;
-            for (int n = 0; n < N; ++n) {
-                const float u = urow[n];
-                const float2 t = ts[n * kUpCols + c];
-                re = fmaf(u, t.x, re);
-                im = fmaf(u, t.y, im);
-            }
-            y = make_float2(re, im);
+    for (int k = threadIdx.x; k < N * N; k += blockDim.x) {
+        const int a = k / N, b = k - a * N;
+        out[(size_t)(2 * a + 1) * M + 2 * b] = amplitude(tr[a * ls + b], ti[a * ls + b]);
+    }
+    // stage 2: odd columns, y[2a + s][2b + 1] = (T_s H^T)[a][b]
+    for (int tile = wave; tile < 2 * tiles * tiles; tile += kUp2Waves) {
+        const int sphase = tile / (tiles * tiles);
+        const int rem = tile - sphase * tiles * tiles;
+        const int mt = rem / tiles, nt = rem - mt * tiles;
+        const float *Ar = sphase ? tr : xr;
+        const float *Ai = sphase ? ti : xi;
+        const int n = 16 * nt + lm;
+        floatx4 accR = {0.0f, 0.0f, 0.0f, 0.0f}, accI = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int k0 = 0; k0 < Np; k0 += 4) {
+            const int k = k0 + lk;
+            int dd = n - k;
+            dd += (dd < 0) ? N : 0;
+            const float b = (k < N && n < N) ? gs[dd] : 0.0f;
+            accR = __builtin_amdgcn_mfma_f32_16x16x4f32(Ar[(16 * mt + lm) * ls + k], b, accR, 0, 0, 0);
+            accI = __builtin_amdgcn_mfma_f32_16x16x4f32(Ai[(16 * mt + lm) * ls + k], b, accI, 0, 0, 0);
         }
-        const double re = y.x, im = y.y;
-        out[(size_t)p * M + q0 + c] = (float)sqrt(re * re + im * im);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int a = 16 * mt + 4 * lk + i;
+            if (a < N && n < N) out[(size_t)(2 * a + sphase) * M + 2 * n + 1] = amplitude(accR[i], accI[i]);
+        }
     }
 }
 
@@ -586,19 +733,28 @@ static size_t correlate_lds(int C, int bu, int ucount, int vcount, int nsplit)
     return (size_t)(nub * bu + C - 1) * (vcount + C / nsplit - 1) * sizeof(float);
 }
 
-CorrelateShape choose_correlate_shape(int C, int P, bool exact)
+CorrelateShape choose_correlate_shape(int C, int P, bool exact, int bu, int nsplit)
 {
     CorrelateShape s{};
-    s.bu = (C % 8 == 0) ? 8 : (C % 4 == 0) ? 4 : (C % 2 == 0) ? 2 : 1;
+    if (bu > 0 && C % bu == 0 && (bu == 1 || bu == 2 || bu == 4 || bu == 8 || bu == 16))
+        s.bu = bu;
+    else if (!exact && C % 4 == 0)
+        s.bu = 4;  // measured best for the split refined contraction
+    else
+        s.bu = (C % 8 == 0) ? 8 : (C % 4 == 0) ? 4 : (C % 2 == 0) ? 2 : 1;
     const size_t budget = 96 * 1024;
     // chip-column split: the exact form keeps the whole contraction in a lane
     s.nsplit = 1;
     if (!exact) {
-        for (int ns : {8, 4, 2})
-            if (C % ns == 0 && C / ns >= 16) {
-                s.nsplit = ns;
-                break;
-            }
+        if (nsplit > 0 && (nsplit & (nsplit - 1)) == 0 && nsplit <= 16 && C % nsplit == 0) {
+            s.nsplit = nsplit;
+        } else {
+            for (int ns : {8, 4, 2})
+                if (C % ns == 0 && C / ns >= 16) {
+                    s.nsplit = ns;
+                    break;
+                }
+        }
     }
     int extent = P;
     const int candidates[] = {P, 64, 48, 32, 24, 16, 8};
@@ -638,18 +794,19 @@ hipError_t launch_chip_stats(hipStream_t s, const float *amp, int pairs, int C, 
     return hipGetLastError();
 }
 
-hipError_t launch_sat(hipStream_t s, const float *amp, int pairs, int W, double *S1, double *S2)
+hipError_t launch_window_sums(hipStream_t s, const float *amp, int pairs, int W, int C,
+                              double2 *sums)
 {
-    const dim3 grid((pairs + 3) / 4), block(256);
+    const dim3 grid(pairs), block(kWave);
     switch ((W + kWave - 1) / kWave) {
-    case 1: hipLaunchKernelGGL(k_sat<1>, grid, block, 0, s, amp, W, pairs, S1, S2); break;
-    case 2: hipLaunchKernelGGL(k_sat<2>, grid, block, 0, s, amp, W, pairs, S1, S2); break;
-    case 3: hipLaunchKernelGGL(k_sat<3>, grid, block, 0, s, amp, W, pairs, S1, S2); break;
-    case 4: hipLaunchKernelGGL(k_sat<4>, grid, block, 0, s, amp, W, pairs, S1, S2); break;
-    case 5: hipLaunchKernelGGL(k_sat<5>, grid, block, 0, s, amp, W, pairs, S1, S2); break;
-    case 6: hipLaunchKernelGGL(k_sat<6>, grid, block, 0, s, amp, W, pairs, S1, S2); break;
-    case 7: hipLaunchKernelGGL(k_sat<7>, grid, block, 0, s, amp, W, pairs, S1, S2); break;
-    case 8: hipLaunchKernelGGL(k_sat<8>, grid, block, 0, s, amp, W, pairs, S1, S2); break;
+    case 1: hipLaunchKernelGGL(k_window_sums<1>, grid, block, 0, s, amp, W, C, sums); break;
+    case 2: hipLaunchKernelGGL(k_window_sums<2>, grid, block, 0, s, amp, W, C, sums); break;
+    case 3: hipLaunchKernelGGL(k_window_sums<3>, grid, block, 0, s, amp, W, C, sums); break;
+    case 4: hipLaunchKernelGGL(k_window_sums<4>, grid, block, 0, s, amp, W, C, sums); break;
+    case 5: hipLaunchKernelGGL(k_window_sums<5>, grid, block, 0, s, amp, W, C, sums); break;
+    case 6: hipLaunchKernelGGL(k_window_sums<6>, grid, block, 0, s, amp, W, C, sums); break;
+    case 7: hipLaunchKernelGGL(k_window_sums<7>, grid, block, 0, s, amp, W, C, sums); break;
+    case 8: hipLaunchKernelGGL(k_window_sums<8>, grid, block, 0, s, amp, W, C, sums); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -664,6 +821,7 @@ hipError_t launch_correlate(hipStream_t s, const CorrelateShape &sh, const float
     hipLaunchKernelGGL(k_correlate<BUV>, grid, block, sh.lds_bytes, s, rT, amp, C, W, P, sh.ub,  \
                        sh.vb, sh.tiles_u, sh.tiles_v, sh.nsplit, partial)
     switch (sh.bu) {
+    case 16: AMPCOR_CORRELATE(16); break;
     case 8: AMPCOR_CORRELATE(8); break;
     case 4: AMPCOR_CORRELATE(4); break;
     case 2: AMPCOR_CORRELATE(2); break;
@@ -674,15 +832,14 @@ hipError_t launch_correlate(hipStream_t s, const CorrelateShape &sh, const float
     return hipGetLastError();
 }
 
-hipError_t launch_normalize(hipStream_t s, const float *partial, int nsplit, const double *S1,
-                            const double *S2, const double2 *stats, int pairs, int C, int W,
-                            float *gamma)
+hipError_t launch_normalize(hipStream_t s, const float *partial, int nsplit, const double2 *sums,
+                            const double2 *stats, int pairs, int C, int W, float *gamma)
 {
     const int P = W - C + 1;
     const size_t total = (size_t)pairs * P * P;
     const unsigned blocks = (unsigned)((total + 255) / 256);
-    hipLaunchKernelGGL(k_normalize, dim3(blocks), dim3(256), 0, s, partial, nsplit, S1, S2, stats,
-                       C, W, P, pairs, gamma);
+    hipLaunchKernelGGL(k_normalize, dim3(blocks), dim3(256), 0, s, partial, nsplit, sums, stats, C,
+                       P, pairs, gamma);
     return hipGetLastError();
 }
 
@@ -696,24 +853,30 @@ hipError_t launch_coarse_peak(hipStream_t s, const float *gamma, int pairs, int 
 }
 
 hipError_t launch_upsample(hipStream_t s, const PairDesc *desc, int pairs, int dtype, int role,
-                           const int2 *window, int N, int rf, const float *U, const float *Ut,
-                           float2 *work, float *amp)
+                           const int2 *window, int N, int rf, const float *U, const float *g,
+                           float *amp)
 {
-    const int M = rf * N;
-    for (int first = 0; first < pairs; first += kMaxGridY) {
-        const int n = pairs - first < kMaxGridY ? pairs - first : kMaxGridY;
-        hipLaunchKernelGGL(k_upsample_rows, dim3((N + kUpRows - 1) / kUpRows, n), dim3(256),
-                           kUpRows * N * sizeof(float2), s, desc + first, dtype, role,
-                           window + first, N, rf, Ut, work + (size_t)first * N * M);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_upsample_cols, dim3((M + kUpCols - 1) / kUpCols, n), dim3(256),
-                           N * kUpCols * sizeof(float2), s, work + (size_t)first * N * M, N, rf,
-                           U, amp + (size_t)first * M * M);
-        e = hipGetLastError();
-        if (e != hipSuccess) return e;
+    const int Np = (N + 15) / 16 * 16;
+    if (rf == 2 && Np <= kUp2MaxN && g) {
+        const size_t lds = (4 * (size_t)Np * (Np + 4) + Np) * sizeof(float);
+        if (dtype == 0)
+            hipLaunchKernelGGL(k_upsample2<0>, dim3(pairs), dim3(kUp2Waves * kWave), lds, s, desc,
+                               role, window, N, Np, g, amp);
+        else
+            hipLaunchKernelGGL(k_upsample2<1>, dim3(pairs), dim3(kUp2Waves * kWave), lds, s, desc,
+                               role, window, N, Np, g, amp);
+        return hipGetLastError();
     }
-    return hipSuccess;
+    const int M = rf * N;
+    const int Mp = (M + 15) / 16 * 16;
+    const size_t lds = (size_t)kUpWaves * 2 * 16 * Np * sizeof(float);
+    if (dtype == 0)
+        hipLaunchKernelGGL(k_upsample<0>, dim3(pairs), dim3(kUpWaves * kWave), lds, s, desc, role,
+                           window, N, Np, M, Mp, U, amp);
+    else
+        hipLaunchKernelGGL(k_upsample<1>, dim3(pairs), dim3(kUpWaves * kWave), lds, s, desc, role,
+                           window, N, Np, M, Mp, U, amp);
+    return hipGetLastError();
 }
 
 hipError_t launch_zoom(hipStream_t s, const float *gamma_refined, int pairs, int R, int zf,

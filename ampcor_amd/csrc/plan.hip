@@ -11,6 +11,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -84,8 +85,8 @@ struct DeviceBuffer {
 };
 
 const char *const kStageNames[] = {
-    "gather_detect", "chip_stats", "sat", "correlate", "normalize", "coarse_peak",
-    "upsample_ref", "upsample_tgt", "chip_stats_refined", "sat_refined", "correlate_refined",
+    "gather_detect", "chip_stats", "window_sums", "correlate", "normalize", "coarse_peak",
+    "upsample_ref", "upsample_tgt", "chip_stats_refined", "window_sums_refined", "correlate_refined",
     "normalize_refined", "zoom",
 };
 constexpr int kStages = sizeof(kStageNames) / sizeof(kStageNames[0]);
@@ -109,13 +110,12 @@ struct ampcor_hip_plan {
 
     DeviceBuffer<PairDesc> d_desc;
     DeviceBuffer<float> amp_ref, amp_tgt, rT, partial, gamma;
-    DeviceBuffer<double> S1, S2;
+    DeviceBuffer<double2> sums, sums_r;
     DeviceBuffer<double2> stats, stats_r;
     DeviceBuffer<int2> loc, window, zoom_loc;
     DeviceBuffer<float> peak, snr, peak_r, offsets;
-    DeviceBuffer<float2> work;
     DeviceBuffer<float> amp_ref_r, amp_tgt_r, rT_r, partial_r, gamma_r;
-    DeviceBuffer<float> U_ref, Ut_ref, U_tgt, Ut_tgt;
+    DeviceBuffer<float> U_ref, U_tgt, g_ref, g_tgt;
     DeviceBuffer<double> D;
 
     bool timing = false;
@@ -126,17 +126,16 @@ struct ampcor_hip_plan {
     {
         DeviceBuffer<float> *fb[] = {&amp_ref, &amp_tgt, &rT, &partial, &gamma, &peak, &snr,
                                      &peak_r, &offsets, &amp_ref_r, &amp_tgt_r, &rT_r,
-                                     &partial_r, &gamma_r, &U_ref, &Ut_ref, &U_tgt, &Ut_tgt};
+                                     &partial_r, &gamma_r, &U_ref, &U_tgt, &g_ref, &g_tgt};
         for (auto *b : fb) b->release();
-        S1.release();
-        S2.release();
+        sums.release();
+        sums_r.release();
         D.release();
         stats.release();
         stats_r.release();
         loc.release();
         window.release();
         zoom_loc.release();
-        work.release();
         d_desc.release();
         for (auto &e : ev)
             if (e) {
@@ -182,19 +181,21 @@ int upload_matrix(DeviceBuffer<float> &buf, const std::vector<float> &host)
     return AMPCOR_HIP_OK;
 }
 
-int upload_upsampler(DeviceBuffer<float> &U, DeviceBuffer<float> &Ut, int N, int rf)
+// the upsampling matrix U[p][n] = kernel(N, rf, p, n), padded with zeros to
+// (rf N rounded up to 16) x (N rounded up to 16) for the matrix-core kernel
+int upload_upsampler(DeviceBuffer<float> &U, DeviceBuffer<float> &g, int N, int rf)
 {
     const int M = rf * N;
-    std::vector<float> u((size_t)M * N), ut((size_t)N * M);
+    const int Np = (N + 15) / 16 * 16, Mp = (M + 15) / 16 * 16;
+    std::vector<float> u((size_t)Mp * Np, 0.0f);
     for (int p = 0; p < M; ++p)
-        for (int n = 0; n < N; ++n) {
-            const float v = (float)upsample_kernel(N, rf, p, n);
-            u[(size_t)p * N + n] = v;
-            ut[(size_t)n * M + p] = v;
-        }
+        for (int n = 0; n < N; ++n) u[(size_t)p * Np + n] = (float)upsample_kernel(N, rf, p, n);
     int rc = upload_matrix(U, u);
-    if (rc) return rc;
-    return upload_matrix(Ut, ut);
+    if (rc || rf != 2) return rc;
+    // the half-sample shift is circulant: U[2a + 1][n] = g((a - n) mod N)
+    std::vector<float> row(N);
+    for (int d = 0; d < N; ++d) row[d] = (float)upsample_kernel(N, rf, 2 * d + 1, 0);
+    return upload_matrix(g, row);
 }
 
 int register_tile(ampcor_hip_plan *plan, const ampcor_hip_raster *raster, size_t idx,
@@ -368,8 +369,14 @@ int ampcor_hip_plan_create(int device, size_t pairs, size_t ref_lines, size_t re
     p->desc.assign(pairs, PairDesc{});
     p->have_ref.assign(pairs, 0);
     p->have_tgt.assign(pairs, 0);
-    p->coarse_shape = choose_correlate_shape(p->C, p->P, true);
-    p->refined_shape = choose_correlate_shape(p->Cr, p->R, false);
+    {
+        // tuning overrides: AMPCOR_HIP_TILE_COARSE="bu", AMPCOR_HIP_TILE_REFINED="bu,nsplit"
+        int cbu = 0, rbu = 0, rns = 0;
+        if (const char *e = std::getenv("AMPCOR_HIP_TILE_COARSE")) std::sscanf(e, "%d", &cbu);
+        if (const char *e = std::getenv("AMPCOR_HIP_TILE_REFINED")) std::sscanf(e, "%d,%d", &rbu, &rns);
+        p->coarse_shape = choose_correlate_shape(p->C, p->P, true, cbu, 0);
+        p->refined_shape = choose_correlate_shape(p->Cr, p->R, false, rbu, rns);
+    }
 
     DeviceGuard guard(device);
     auto cleanup = [&](int rc) {
@@ -381,7 +388,6 @@ int ampcor_hip_plan_create(int device, size_t pairs, size_t ref_lines, size_t re
     if (e != hipSuccess) return cleanup(hip_fail(e, "creating the plan stream"));
 
     const size_t N = pairs;
-    const size_t Ws = std::max(W, Wr) + 1;
     struct {
         hipError_t e;
     } r{hipSuccess};
@@ -393,15 +399,14 @@ int ampcor_hip_plan_create(int device, size_t pairs, size_t ref_lines, size_t re
     A(p->amp_tgt.alloc(N * W * W));
     A(p->rT.alloc(N * C * C));
     A(p->stats.alloc(N));
-    A(p->S1.alloc(N * Ws * Ws));
-    A(p->S2.alloc(N * Ws * Ws));
+    A(p->sums.alloc(N * p->P * p->P));
+    A(p->sums_r.alloc(N * R * R));
     A(p->partial.alloc(N * p->coarse_shape.nsplit * p->P * p->P));
     A(p->gamma.alloc(N * p->P * p->P));
     A(p->loc.alloc(N));
     A(p->window.alloc(N));
     A(p->peak.alloc(N));
     A(p->snr.alloc(N));
-    A(p->work.alloc(N * std::max(C * Cr, E * Wr)));
     A(p->amp_ref_r.alloc(N * Cr * Cr));
     A(p->amp_tgt_r.alloc(N * Wr * Wr));
     A(p->rT_r.alloc(N * Cr * Cr));
@@ -416,8 +421,8 @@ int ampcor_hip_plan_create(int device, size_t pairs, size_t ref_lines, size_t re
                           " pairs: " + hipGetErrorString(r.e);
         return cleanup(fail(AMPCOR_HIP_ENOMEM, msg));
     }
-    if (int rc = upload_upsampler(p->U_ref, p->Ut_ref, p->C, p->rf)) return cleanup(rc);
-    if (int rc = upload_upsampler(p->U_tgt, p->Ut_tgt, p->E, p->rf)) return cleanup(rc);
+    if (int rc = upload_upsampler(p->U_ref, p->g_ref, p->C, p->rf)) return cleanup(rc);
+    if (int rc = upload_upsampler(p->U_tgt, p->g_tgt, p->E, p->rf)) return cleanup(rc);
     {
         std::vector<double> d(2 * p->Z + 1);
         for (int k = -p->Z; k <= p->Z; ++k) d[k + p->Z] = dirichlet(p->R, p->zf, k);
@@ -537,28 +542,29 @@ int ampcor_hip_adjust(ampcor_hip_plan *plan, float *offsets)
     STEP(launch_gather_detect(s, p->d_desc.ptr, N, p->dtype, p->C, p->W, p->amp_ref.ptr, p->amp_tgt.ptr),
          "gather_detect");
     STEP(launch_chip_stats(s, p->amp_ref.ptr, N, p->C, p->rT.ptr, p->stats.ptr), "chip_stats");
-    STEP(launch_sat(s, p->amp_tgt.ptr, N, p->W, p->S1.ptr, p->S2.ptr), "sat");
+    STEP(launch_window_sums(s, p->amp_tgt.ptr, N, p->W, p->C, p->sums.ptr), "window_sums");
     STEP(launch_correlate(s, p->coarse_shape, p->rT.ptr, p->amp_tgt.ptr, N, p->C, p->W, p->partial.ptr),
          "correlate");
-    STEP(launch_normalize(s, p->partial.ptr, p->coarse_shape.nsplit, p->S1.ptr, p->S2.ptr, p->stats.ptr,
-                          N, p->C, p->W, p->gamma.ptr),
+    STEP(launch_normalize(s, p->partial.ptr, p->coarse_shape.nsplit, p->sums.ptr, p->stats.ptr, N,
+                          p->C, p->W, p->gamma.ptr),
          "normalize");
     STEP(launch_coarse_peak(s, p->gamma.ptr, N, p->P, p->W, p->E, p->rm, p->rf, p->loc.ptr,
                             p->window.ptr, p->peak.ptr, p->snr.ptr),
          "coarse_peak");
     STEP(launch_upsample(s, p->d_desc.ptr, N, p->dtype, 0, p->window.ptr, p->C, p->rf, p->U_ref.ptr,
-                         p->Ut_ref.ptr, p->work.ptr, p->amp_ref_r.ptr),
+                         p->g_ref.ptr, p->amp_ref_r.ptr),
          "upsample_ref");
     STEP(launch_upsample(s, p->d_desc.ptr, N, p->dtype, 1, p->window.ptr, p->E, p->rf, p->U_tgt.ptr,
-                         p->Ut_tgt.ptr, p->work.ptr, p->amp_tgt_r.ptr),
+                         p->g_tgt.ptr, p->amp_tgt_r.ptr),
          "upsample_tgt");
     STEP(launch_chip_stats(s, p->amp_ref_r.ptr, N, p->Cr, p->rT_r.ptr, p->stats_r.ptr),
          "chip_stats_refined");
-    STEP(launch_sat(s, p->amp_tgt_r.ptr, N, p->Wr, p->S1.ptr, p->S2.ptr), "sat_refined");
+    STEP(launch_window_sums(s, p->amp_tgt_r.ptr, N, p->Wr, p->Cr, p->sums_r.ptr),
+         "window_sums_refined");
     STEP(launch_correlate(s, p->refined_shape, p->rT_r.ptr, p->amp_tgt_r.ptr, N, p->Cr, p->Wr,
                           p->partial_r.ptr),
          "correlate_refined");
-    STEP(launch_normalize(s, p->partial_r.ptr, p->refined_shape.nsplit, p->S1.ptr, p->S2.ptr,
+    STEP(launch_normalize(s, p->partial_r.ptr, p->refined_shape.nsplit, p->sums_r.ptr,
                           p->stats_r.ptr, N, p->Cr, p->Wr, p->gamma_r.ptr),
          "normalize_refined");
     STEP(launch_zoom(s, p->gamma_r.ptr, N, p->R, p->zf, p->D.ptr, p->window.ptr, p->peak.ptr, p->m,

@@ -341,51 +341,67 @@ static void twin_chip_stats(float *r, int cells, double *sum_rp, double *norm)
     *norm = sqrt(twin_tree(p2));
 }
 
-/* The HIP SAT kernel gives one 64-lane wavefront to a pair.  Row k is read
- * in chunks of 64 columns; each chunk gets an inclusive Kogge-Stone scan
- * across lanes (offsets 1..32), then the running carry of the earlier chunks
- * is added; each lane keeps the column sums of its columns. */
-static void twin_sat(const float *t, int W, double *S1, double *S2)
+/* The HIP window-sum kernel gives one 64-lane wavefront to a pair.  Lane l
+ * owns columns l + 64 q.  Vertical box sums start as the sum of rows 0..C-1
+ * and slide down one row per placement row, V <- (V + t(u+C-1)) - t(u-1);
+ * each row of V then gets an inclusive Kogge-Stone scan across the 64 lanes
+ * (offsets 1..32) per 64-column chunk, plus the running carry of the earlier
+ * chunks; the window sum is the difference of two scan entries. */
+static void twin_scan_row(const double *V1, const double *V2, int W, double *s1, double *s2)
 {
-    const int sc = W + 1;
     const int Q = (W + 63) / 64;
-    double c1[64 * 8], c2[64 * 8];
     double v1[64], v2[64], o1[64], o2[64];
-    for (int k = 0; k < 64 * Q; ++k) c1[k] = c2[k] = 0.0;
-    for (int j = 0; j < sc; ++j) S1[j] = S2[j] = 0.0;
-    for (int k = 0; k < W; ++k) {
-        S1[(size_t)(k + 1) * sc] = 0.0;
-        S2[(size_t)(k + 1) * sc] = 0.0;
-        double carry1 = 0.0, carry2 = 0.0;
-        for (int q = 0; q < Q; ++q) {
-            for (int l = 0; l < 64; ++l) {
-                const int x = l + 64 * q;
-                const double d = (x < W) ? (double)t[(size_t)k * W + x] : 0.0;
-                v1[l] = d;
-                v2[l] = d * d;
+    double carry1 = 0.0, carry2 = 0.0;
+    for (int q = 0; q < Q; ++q) {
+        for (int l = 0; l < 64; ++l) {
+            const int x = l + 64 * q;
+            v1[l] = (x < W) ? V1[x] : 0.0;
+            v2[l] = (x < W) ? V2[x] : 0.0;
+        }
+        for (int d = 1; d < 64; d <<= 1) {
+            for (int l = 0; l < 64; ++l) { o1[l] = v1[l]; o2[l] = v2[l]; }
+            for (int l = d; l < 64; ++l) {
+                v1[l] = o1[l - d] + o1[l];
+                v2[l] = o2[l - d] + o2[l];
             }
-            for (int d = 1; d < 64; d <<= 1) {
-                for (int l = 0; l < 64; ++l) { o1[l] = v1[l]; o2[l] = v2[l]; }
-                for (int l = d; l < 64; ++l) {
-                    v1[l] = o1[l - d] + o1[l];
-                    v2[l] = o2[l - d] + o2[l];
-                }
+        }
+        for (int l = 0; l < 64; ++l) {
+            v1[l] = carry1 + v1[l];
+            v2[l] = carry2 + v2[l];
+        }
+        carry1 = v1[63];
+        carry2 = v2[63];
+        for (int l = 0; l < 64; ++l) {
+            const int x = l + 64 * q;
+            if (x < W) { s1[x] = v1[l]; s2[x] = v2[l]; }
+        }
+    }
+}
+
+static void twin_window_sums(const float *t, int W, int C, double *sum1, double *sum2)
+{
+    const int P = W - C + 1;
+    double V1[512], V2[512], s1[512], s2[512];
+    for (int x = 0; x < W; ++x) {
+        V1[x] = V2[x] = 0.0;
+        for (int i = 0; i < C; ++i) {
+            const double d = (double)t[(size_t)i * W + x];
+            V1[x] = V1[x] + d;
+            V2[x] = V2[x] + d * d;
+        }
+    }
+    for (int u = 0; u < P; ++u) {
+        if (u > 0)
+            for (int x = 0; x < W; ++x) {
+                const double a = (double)t[(size_t)(u + C - 1) * W + x];
+                const double b = (double)t[(size_t)(u - 1) * W + x];
+                V1[x] = (V1[x] + a) - b;
+                V2[x] = (V2[x] + a * a) - b * b;
             }
-            for (int l = 0; l < 64; ++l) {
-                v1[l] = carry1 + v1[l];
-                v2[l] = carry2 + v2[l];
-            }
-            carry1 = v1[63];
-            carry2 = v2[63];
-            for (int l = 0; l < 64; ++l) {
-                const int x = l + 64 * q;
-                c1[64 * q + l] = c1[64 * q + l] + v1[l];
-                c2[64 * q + l] = c2[64 * q + l] + v2[l];
-                if (x < W) {
-                    S1[(size_t)(k + 1) * sc + x + 1] = c1[64 * q + l];
-                    S2[(size_t)(k + 1) * sc + x + 1] = c2[64 * q + l];
-                }
-            }
+        twin_scan_row(V1, V2, W, s1, s2);
+        for (int v = 0; v < P; ++v) {
+            sum1[(size_t)u * P + v] = s1[v + C - 1] - (v > 0 ? s1[v - 1] : 0.0);
+            sum2[(size_t)u * P + v] = s2[v + C - 1] - (v > 0 ? s2[v - 1] : 0.0);
         }
     }
 }
@@ -400,9 +416,8 @@ static void twin_coarse(const float *amp_ref /* C*C, becomes r' */, const float 
     twin_chip_stats(r, C * C, &sum_rp, &norm);
     for (int i = 0; i < C; ++i)
         for (int j = 0; j < C; ++j) rT[(size_t)j * C + i] = r[(size_t)i * C + j];
-    twin_sat(amp_tgt, W, S1, S2);
+    twin_window_sums(amp_tgt, W, C, S1, S2);
     const double n = (double)C * C;
-    const int sc = W + 1;
     for (int u = 0; u < P; ++u)
         for (int v = 0; v < P; ++v) {
             float acc = 0.0f;
@@ -410,10 +425,8 @@ static void twin_coarse(const float *amp_ref /* C*C, becomes r' */, const float 
                 const float *rc = rT + (size_t)j * C;
                 for (int i = 0; i < C; ++i) acc = fmaf(rc[i], amp_tgt[(size_t)(u + i) * W + v + j], acc);
             }
-            const double s1 = ((S1[(size_t)(u + C) * sc + v + C] - S1[(size_t)u * sc + v + C])
-                               - S1[(size_t)(u + C) * sc + v]) + S1[(size_t)u * sc + v];
-            const double s2 = ((S2[(size_t)(u + C) * sc + v + C] - S2[(size_t)u * sc + v + C])
-                               - S2[(size_t)(u + C) * sc + v]) + S2[(size_t)u * sc + v];
+            const double s1 = S1[(size_t)u * P + v];
+            const double s2 = S2[(size_t)u * P + v];
             const double mu = s1 / n;
             const double var = s2 - mu * s1;
             const double A = mu * sum_rp;

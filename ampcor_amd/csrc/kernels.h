@@ -57,6 +57,32 @@ hipError_t launch_coarse_peak(hipStream_t s, const float *gamma, int pairs, int 
 hipError_t launch_upsample(hipStream_t s, const PairDesc *desc, int pairs, int dtype, int role,
                            const int2 *window, int N, int rf, const float *U, const float *g,
                            float *amp);
+// ---- mixed precision (mixed.hip) ----
+// per-pair constants of the mixed normalizer, unscaled units
+struct MixedStats {
+    double sumr;    // sum of the quantized zero-mean chip
+    double normr;   // its L2 norm
+    double inv_ab;  // 1 / (chip scale * target scale)
+    double mean_t;  // window mean removed from the quantized target
+};
+// tiling of the MFMA contraction: 16 x 16 placement tiles, nk K-steps of 16
+// target columns per tile, workgroups of wg_ut waves x wg_vt v-tiles
+struct MixedShape {
+    int nk, cs;
+    int wg_ut, wg_vt, tiles_u, tiles_v, nsplit;
+    size_t lds_bytes;
+};
+MixedShape choose_mixed_shape(int C, int P);
+hipError_t launch_prep_mixed(hipStream_t s, const float *ref_amp, const float *tgt_amp, int pairs,
+                             int C, int W, int CS, bool fp16, uint16_t *chip_q, uint16_t *tgt_q,
+                             MixedStats *stats);
+hipError_t launch_correlate_mixed(hipStream_t s, const MixedShape &shape, const uint16_t *chip_q,
+                                  const uint16_t *tgt_q, int pairs, int C, int W, bool fp16,
+                                  float *partial);
+hipError_t launch_normalize_mixed(hipStream_t s, const float *partial, int nsplit,
+                                  const double2 *sums, const MixedStats *stats, int pairs, int C,
+                                  int W, float *gamma);
+
 hipError_t launch_zoom(hipStream_t s, const float *gamma_refined, int pairs, int R, int zf,
                        const double *dirichlet, const int2 *window, const float *coarse_peak,
                        int margin, int rf, float *offsets, float *refined_peak, int2 *zoom_loc);

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <cstring>
 #include <new>
 #include <string>
@@ -90,6 +91,13 @@ const char *const kStageNames[] = {
     "normalize_refined", "zoom",
 };
 constexpr int kStages = sizeof(kStageNames) / sizeof(kStageNames[0]);
+// the mixed forms quantize instead of computing FP32 chip statistics
+const char *const kStageNamesMixed[] = {
+    "gather_detect", "prep", "window_sums", "correlate", "normalize", "coarse_peak",
+    "upsample_ref", "upsample_tgt", "prep_refined", "window_sums_refined", "correlate_refined",
+    "normalize_refined", "zoom",
+};
+static_assert(sizeof(kStageNamesMixed) == sizeof(kStageNames), "stage tables differ");
 
 }  // namespace
 
@@ -107,6 +115,8 @@ struct ampcor_hip_plan {
     bool adjusted = false;
 
     CorrelateShape coarse_shape{}, refined_shape{};
+    MixedShape mixed_coarse{}, mixed_refined{};
+    bool mixed = false;
 
     DeviceBuffer<PairDesc> d_desc;
     DeviceBuffer<float> amp_ref, amp_tgt, rT, partial, gamma;
@@ -117,6 +127,8 @@ struct ampcor_hip_plan {
     DeviceBuffer<float> amp_ref_r, amp_tgt_r, rT_r, partial_r, gamma_r;
     DeviceBuffer<float> U_ref, U_tgt, g_ref, g_tgt;
     DeviceBuffer<double> D;
+    DeviceBuffer<uint16_t> chip_q, tgt_q, chip_q_r, tgt_q_r;
+    DeviceBuffer<MixedStats> mstats, mstats_r;
 
     bool timing = false;
     hipEvent_t ev[kStages + 1] = {};
@@ -130,6 +142,12 @@ struct ampcor_hip_plan {
         for (auto *b : fb) b->release();
         sums.release();
         sums_r.release();
+        chip_q.release();
+        tgt_q.release();
+        chip_q_r.release();
+        tgt_q_r.release();
+        mstats.release();
+        mstats_r.release();
         D.release();
         stats.release();
         stats_r.release();
@@ -345,8 +363,11 @@ int ampcor_hip_plan_create(int device, size_t pairs, size_t ref_lines, size_t re
     const size_t R = Wr - Cr + 1;
     if (Wr > 512) return fail(AMPCOR_HIP_EINVAL, "refined window edge must be <= 512");
     if (R > 129) return fail(AMPCOR_HIP_EINVAL, "refined surface too large (2 rf rm + 1 > 129)");
-    if (precision != AMPCOR_HIP_FP32)
-        return fail(AMPCOR_HIP_EINVAL, "only the FP32 form is available in this build");
+    if (precision != AMPCOR_HIP_FP32 && precision != AMPCOR_HIP_BF16 && precision != AMPCOR_HIP_FP16)
+        return fail(AMPCOR_HIP_EINVAL, "unknown precision (FP32, BF16 or FP16)");
+    if (precision != AMPCOR_HIP_FP32 && (C % 16 || C * refine_factor > 256))
+        return fail(AMPCOR_HIP_EINVAL, "the mixed forms need a chip edge that is a multiple of 16 "
+                                       "and a refined chip edge <= 256");
     if (int rc = check_device(device)) return rc;
 
     auto *p = new (std::nothrow) ampcor_hip_plan();
@@ -377,6 +398,11 @@ int ampcor_hip_plan_create(int device, size_t pairs, size_t ref_lines, size_t re
         p->coarse_shape = choose_correlate_shape(p->C, p->P, true, cbu, 0);
         p->refined_shape = choose_correlate_shape(p->Cr, p->R, false, rbu, rns);
     }
+    p->mixed = precision != AMPCOR_HIP_FP32;
+    if (p->mixed) {
+        p->mixed_coarse = choose_mixed_shape(p->C, p->P);
+        p->mixed_refined = choose_mixed_shape(p->Cr, p->R);
+    }
 
     DeviceGuard guard(device);
     auto cleanup = [&](int rc) {
@@ -401,7 +427,9 @@ int ampcor_hip_plan_create(int device, size_t pairs, size_t ref_lines, size_t re
     A(p->stats.alloc(N));
     A(p->sums.alloc(N * p->P * p->P));
     A(p->sums_r.alloc(N * R * R));
-    A(p->partial.alloc(N * p->coarse_shape.nsplit * p->P * p->P));
+    const size_t ns_c = std::max(p->coarse_shape.nsplit, p->mixed ? p->mixed_coarse.nsplit : 1);
+    const size_t ns_r = std::max(p->refined_shape.nsplit, p->mixed ? p->mixed_refined.nsplit : 1);
+    A(p->partial.alloc(N * ns_c * p->P * p->P));
     A(p->gamma.alloc(N * p->P * p->P));
     A(p->loc.alloc(N));
     A(p->window.alloc(N));
@@ -411,7 +439,15 @@ int ampcor_hip_plan_create(int device, size_t pairs, size_t ref_lines, size_t re
     A(p->amp_tgt_r.alloc(N * Wr * Wr));
     A(p->rT_r.alloc(N * Cr * Cr));
     A(p->stats_r.alloc(N));
-    A(p->partial_r.alloc(N * p->refined_shape.nsplit * R * R));
+    A(p->partial_r.alloc(N * ns_r * R * R));
+    if (p->mixed) {
+        A(p->chip_q.alloc(N * C * p->mixed_coarse.cs));
+        A(p->tgt_q.alloc(N * W * W));
+        A(p->chip_q_r.alloc(N * Cr * p->mixed_refined.cs));
+        A(p->tgt_q_r.alloc(N * Wr * Wr));
+        A(p->mstats.alloc(N));
+        A(p->mstats_r.alloc(N));
+    }
     A(p->gamma_r.alloc(N * R * R));
     A(p->offsets.alloc(2 * N));
     A(p->peak_r.alloc(N));
@@ -539,15 +575,31 @@ int ampcor_hip_adjust(ampcor_hip_plan *plan, float *offsets)
     } while (0)
 
     auto *p = plan;
+    const bool fp16 = p->precision == AMPCOR_HIP_FP16;
     STEP(launch_gather_detect(s, p->d_desc.ptr, N, p->dtype, p->C, p->W, p->amp_ref.ptr, p->amp_tgt.ptr),
          "gather_detect");
-    STEP(launch_chip_stats(s, p->amp_ref.ptr, N, p->C, p->rT.ptr, p->stats.ptr), "chip_stats");
+    if (p->mixed)
+        STEP(launch_prep_mixed(s, p->amp_ref.ptr, p->amp_tgt.ptr, N, p->C, p->W, p->mixed_coarse.cs,
+                               fp16, p->chip_q.ptr, p->tgt_q.ptr, p->mstats.ptr),
+             "prep");
+    else
+        STEP(launch_chip_stats(s, p->amp_ref.ptr, N, p->C, p->rT.ptr, p->stats.ptr), "chip_stats");
     STEP(launch_window_sums(s, p->amp_tgt.ptr, N, p->W, p->C, p->sums.ptr), "window_sums");
-    STEP(launch_correlate(s, p->coarse_shape, p->rT.ptr, p->amp_tgt.ptr, N, p->C, p->W, p->partial.ptr),
-         "correlate");
-    STEP(launch_normalize(s, p->partial.ptr, p->coarse_shape.nsplit, p->sums.ptr, p->stats.ptr, N,
-                          p->C, p->W, p->gamma.ptr),
-         "normalize");
+    if (p->mixed) {
+        STEP(launch_correlate_mixed(s, p->mixed_coarse, p->chip_q.ptr, p->tgt_q.ptr, N, p->C, p->W,
+                                    fp16, p->partial.ptr),
+             "correlate");
+        STEP(launch_normalize_mixed(s, p->partial.ptr, p->mixed_coarse.nsplit, p->sums.ptr,
+                                    p->mstats.ptr, N, p->C, p->W, p->gamma.ptr),
+             "normalize");
+    } else {
+        STEP(launch_correlate(s, p->coarse_shape, p->rT.ptr, p->amp_tgt.ptr, N, p->C, p->W,
+                              p->partial.ptr),
+             "correlate");
+        STEP(launch_normalize(s, p->partial.ptr, p->coarse_shape.nsplit, p->sums.ptr, p->stats.ptr, N,
+                              p->C, p->W, p->gamma.ptr),
+             "normalize");
+    }
     STEP(launch_coarse_peak(s, p->gamma.ptr, N, p->P, p->W, p->E, p->rm, p->rf, p->loc.ptr,
                             p->window.ptr, p->peak.ptr, p->snr.ptr),
          "coarse_peak");
@@ -557,16 +609,31 @@ int ampcor_hip_adjust(ampcor_hip_plan *plan, float *offsets)
     STEP(launch_upsample(s, p->d_desc.ptr, N, p->dtype, 1, p->window.ptr, p->E, p->rf, p->U_tgt.ptr,
                          p->g_tgt.ptr, p->amp_tgt_r.ptr),
          "upsample_tgt");
-    STEP(launch_chip_stats(s, p->amp_ref_r.ptr, N, p->Cr, p->rT_r.ptr, p->stats_r.ptr),
-         "chip_stats_refined");
+    if (p->mixed)
+        STEP(launch_prep_mixed(s, p->amp_ref_r.ptr, p->amp_tgt_r.ptr, N, p->Cr, p->Wr,
+                               p->mixed_refined.cs, fp16, p->chip_q_r.ptr, p->tgt_q_r.ptr,
+                               p->mstats_r.ptr),
+             "prep_refined");
+    else
+        STEP(launch_chip_stats(s, p->amp_ref_r.ptr, N, p->Cr, p->rT_r.ptr, p->stats_r.ptr),
+             "chip_stats_refined");
     STEP(launch_window_sums(s, p->amp_tgt_r.ptr, N, p->Wr, p->Cr, p->sums_r.ptr),
          "window_sums_refined");
-    STEP(launch_correlate(s, p->refined_shape, p->rT_r.ptr, p->amp_tgt_r.ptr, N, p->Cr, p->Wr,
-                          p->partial_r.ptr),
-         "correlate_refined");
-    STEP(launch_normalize(s, p->partial_r.ptr, p->refined_shape.nsplit, p->sums_r.ptr,
-                          p->stats_r.ptr, N, p->Cr, p->Wr, p->gamma_r.ptr),
-         "normalize_refined");
+    if (p->mixed) {
+        STEP(launch_correlate_mixed(s, p->mixed_refined, p->chip_q_r.ptr, p->tgt_q_r.ptr, N, p->Cr,
+                                    p->Wr, fp16, p->partial_r.ptr),
+             "correlate_refined");
+        STEP(launch_normalize_mixed(s, p->partial_r.ptr, p->mixed_refined.nsplit, p->sums_r.ptr,
+                                    p->mstats_r.ptr, N, p->Cr, p->Wr, p->gamma_r.ptr),
+             "normalize_refined");
+    } else {
+        STEP(launch_correlate(s, p->refined_shape, p->rT_r.ptr, p->amp_tgt_r.ptr, N, p->Cr, p->Wr,
+                              p->partial_r.ptr),
+             "correlate_refined");
+        STEP(launch_normalize(s, p->partial_r.ptr, p->refined_shape.nsplit, p->sums_r.ptr,
+                              p->stats_r.ptr, N, p->Cr, p->Wr, p->gamma_r.ptr),
+             "normalize_refined");
+    }
     STEP(launch_zoom(s, p->gamma_r.ptr, N, p->R, p->zf, p->D.ptr, p->window.ptr, p->peak.ptr, p->m,
                      p->rf, p->offsets.ptr, p->peak_r.ptr, p->zoom_loc.ptr),
          "zoom");
@@ -594,7 +661,7 @@ int ampcor_hip_stage_times(ampcor_hip_plan *plan, int capacity, int *count, floa
         return fail(AMPCOR_HIP_ESTATE, "no timed adjust has run on this plan");
     for (int k = 0; k < kStages && k < capacity; ++k) {
         if (ms) ms[k] = plan->stage_ms[k];
-        if (names) names[k] = kStageNames[k];
+        if (names) names[k] = plan->mixed ? kStageNamesMixed[k] : kStageNames[k];
     }
     return AMPCOR_HIP_OK;
 }

@@ -27,6 +27,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "tile pairs/sec @64²/128² zoom8, mixed vs FP32 at ≤0.01px; p99 job latency"
 FP32_PEAK_TFLOPS = 157.3   # MI355X dense FP32 (matrix) peak
+MIXED_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 / FP16 matrix peak
 HBM_PEAK_GBS = 8000.0
 
 
@@ -35,7 +36,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--precision", default="fp32")
+    ap.add_argument("--precision", default="bf16",
+                    help="form whose throughput is the headline value (fp32, bf16, fp16); "
+                         "the FP32 drop-in form is always timed beside it")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target duration of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -51,20 +54,20 @@ def cpu_baseline(scene, seconds):
     geo = (scene.chip, scene.window, scene.refine_factor, scene.refine_margin, scene.zoom_factor)
 
     def run(n):
-        idx = np.linspace(0, scene.pairs - 1, n).astype(int)
+        idx = np.unique(np.linspace(0, scene.pairs - 1, n).astype(int))
         t0 = time.perf_counter()
-        pyoracle.adjust(*args, scene.ref_begin[idx], scene.tgt_begin[idx], *geo, form="fp64",
-                        threads=cores)
-        return time.perf_counter() - t0
+        out = pyoracle.adjust(*args, scene.ref_begin[idx], scene.tgt_begin[idx], *geo,
+                              form="fp64", threads=cores)
+        return time.perf_counter() - t0, idx, out["offsets"]
 
     n = 2 * cores
-    dt = run(n)
+    dt, _, _ = run(n)
     n2 = int(min(scene.pairs, max(n, n * seconds / max(dt, 1e-3))))
     n2 = max(cores, (n2 // cores) * cores)
-    dt = run(n2)
-    return {"value": n2 / dt, "unit": "tile pairs/sec", "cores": cores, "kind": "port",
-            "sample": f"FP64 oracle (oracle/ampcor_oracle.c, OpenMP over pairs) on {n2} of the "
-                      f"{scene.pairs} pairs, evenly spaced, {dt:.1f} s"}
+    dt, idx, offsets = run(n2)
+    return ({"value": len(idx) / dt, "unit": "tile pairs/sec", "cores": cores, "kind": "port",
+             "sample": f"FP64 oracle (oracle/ampcor_oracle.c, OpenMP over pairs) on {len(idx)} "
+                       f"of the {scene.pairs} pairs, evenly spaced, {dt:.1f} s"}, idx, offsets)
 
 
 def main():
@@ -86,34 +89,40 @@ def main():
     print(f"[bench] scene ready in {time.perf_counter() - t0:.1f} s: {scene.pairs} pairs",
           file=sys.stderr, flush=True)
     N, C, W = scene.pairs, scene.chip, scene.window
-    worker = lib.sequential(N, (C, C), (W, W), scene.refine_margin, scene.refine_factor,
-                            scene.zoom_factor, precision=a.precision)
     ref = lib.raster(scene.reference, *scene.reference.shape)
     tgt = lib.raster(scene.target, *scene.target.shape)
-    lib.addPairs(worker, ref, tgt, 0, scene.ref_begin.tobytes(), scene.tgt_begin.tobytes())
 
-    for _ in range(a.warmup):
-        lib.adjust(worker)
-    lib.set_timing(worker, True)
-    stage_sum = {}
-    elapsed = 0.0
-    for _ in range(a.steps):
-        t = time.perf_counter()
-        lib.adjust(worker)               # synchronizes the plan stream before returning
-        elapsed += time.perf_counter() - t
-        for k, v in lib.stage_times(worker).items():
-            stage_sum[k] = stage_sum.get(k, 0.0) + v
-    stages = {k: v / a.steps for k, v in stage_sum.items()}
-    ms_per_step = 1e3 * elapsed / a.steps
-    value = N / (elapsed / a.steps)
-    print("[bench] stage ms/step: " + ", ".join(f"{k} {v:.3f}" for k, v in stages.items()),
-          file=sys.stderr, flush=True)
+    def timed(precision):
+        worker = lib.sequential(N, (C, C), (W, W), scene.refine_margin, scene.refine_factor,
+                                scene.zoom_factor, precision=precision)
+        lib.addPairs(worker, ref, tgt, 0, scene.ref_begin.tobytes(), scene.tgt_begin.tobytes())
+        for _ in range(a.warmup):
+            lib.adjust(worker)
+        lib.set_timing(worker, True)
+        stage_sum = {}
+        elapsed = 0.0
+        for _ in range(a.steps):
+            t = time.perf_counter()
+            lib.adjust(worker)               # synchronizes the plan stream before returning
+            elapsed += time.perf_counter() - t
+            for k, v in lib.stage_times(worker).items():
+                stage_sum[k] = stage_sum.get(k, 0.0) + v
+        stages = {k: v / a.steps for k, v in stage_sum.items()}
+        off = np.frombuffer(lib.products(worker)["offsets"], np.float32).reshape(N, 2).copy()
+        print(f"[bench] {precision} stage ms/step: " +
+              ", ".join(f"{k} {v:.3f}" for k, v in stages.items()), file=sys.stderr, flush=True)
+        return elapsed / a.steps, stages, off
 
-    # accuracy of this run against the truth field (informational)
-    off = np.frombuffer(lib.products(worker)["offsets"], np.float32).reshape(N, 2)
+    forms = ["fp32"] if a.precision == "fp32" else ["fp32", a.precision]
+    runs = {f: timed(f) for f in forms}
+    step_s, stages, off = runs[a.precision]
+    ms_per_step = 1e3 * step_s
+    value = N / step_s
+    fp32_value = N / runs["fp32"][0]
     err = np.abs(off - scene.truth)
-    print(f"[bench] |offset - true field|: median {np.median(err):.4f} px, p99 "
-          f"{np.percentile(err, 99):.4f} px", file=sys.stderr, flush=True)
+    print(f"[bench] {a.precision}: {value:.0f} pairs/s ({ms_per_step:.3f} ms/step); FP32 form "
+          f"{fp32_value:.0f} pairs/s; |offset - true field| median {np.median(err):.4f} px, "
+          f"p99 {np.percentile(err, 99):.4f} px", file=sys.stderr, flush=True)
 
     # dominant kernel: the larger of the two contractions
     P, R = W - C + 1, 2 * scene.refine_factor * scene.refine_margin + 1
@@ -126,14 +135,20 @@ def main():
     if os.path.exists(tfile):
         with open(tfile) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
-    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+    peak = FP32_PEAK_TFLOPS if a.precision == "fp32" else MIXED_PEAK_TFLOPS
+    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
+                "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
                 "traffic": traffic, "kernel": dom, "kernel_ms": round(stages[dom], 4)}
 
     cpu = None
+    accuracy = None
     if not a.no_cpu:
-        cpu = cpu_baseline(scene, a.cpu_seconds)
+        cpu, idx, exact = cpu_baseline(scene, a.cpu_seconds)
         print(f"[bench] cpu baseline: {cpu}", file=sys.stderr, flush=True)
+        accuracy = {f"{f}_max_offset_err_px_vs_fp64": float(np.max(np.abs(runs[f][2][idx] - exact)))
+                    for f in forms}
+        accuracy["pairs_compared"] = int(len(idx))
+        print(f"[bench] accuracy vs FP64 oracle: {accuracy}", file=sys.stderr, flush=True)
 
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "tile pairs/sec", "n_gpus": 1,
@@ -145,7 +160,8 @@ def main():
                                "margin 8, zoom 8",
                    "pairs": N, "chip": C, "window": W, "refine_factor": scene.refine_factor,
                    "refine_margin": scene.refine_margin, "zoom_factor": scene.zoom_factor,
-                   "precision": a.precision},
+                   "precision": a.precision, "fp32_pairs_per_sec": round(fp32_value, 1),
+                   "speedup_vs_fp32": round(value / fp32_value, 3), "accuracy": accuracy},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

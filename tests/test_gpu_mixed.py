@@ -1,0 +1,75 @@
+"""The mixed-precision forms (BF16 / FP16 contraction on MFMA, FP32 accumulate,
+FP64 window statistics) against the FP64 oracle.
+
+Bar: refined offsets within 0.01 px of the FP64 oracle; coarse peaks on the
+FP64 oracle's placement; correlation peaks within 1e-2 relative.
+"""
+
+import numpy as np
+import pytest
+
+from ampcor_amd import synth
+from oracle import pyoracle
+from tests.gpu_helpers import run_hip
+
+pytestmark = pytest.mark.gpu
+
+MIXED_OFFSET_TOL = 0.01  # px vs the FP64 oracle
+PEAK_REL_TOL = 1e-2
+
+
+def fp64(s):
+    return pyoracle.adjust(s.reference, s.target, s.ref_begin, s.tgt_begin, s.chip, s.window,
+                           s.refine_factor, s.refine_margin, s.zoom_factor, form="fp64")
+
+
+def run(lib, s, precision):
+    return run_hip(lib, s.reference, s.target, s.ref_begin, s.tgt_begin, s.chip, s.window,
+                   s.refine_factor, s.refine_margin, s.zoom_factor, precision=precision)
+
+
+@pytest.mark.parametrize("precision", ["bf16", "fp16"])
+def test_mixed_shifted_scene(lib, precision):
+    s = synth.shifted_scene()
+    hip, exact = run(lib, s, precision), fp64(s)
+    err = np.abs(hip["offsets"] - exact["offsets"])
+    print(f"{precision}: max |offset - fp64| = {err.max():.2e} px", flush=True)
+    assert err.max() <= MIXED_OFFSET_TOL
+    assert np.array_equal(hip["coarse_loc"], exact["coarse_loc"])
+    np.testing.assert_allclose(hip["coarse_peak"], exact["coarse_peak"], rtol=PEAK_REL_TOL)
+    np.testing.assert_allclose(hip["refined_peak"], exact["refined_peak"], rtol=PEAK_REL_TOL)
+
+
+@pytest.mark.parametrize("precision", ["bf16", "fp16"])
+def test_mixed_field_scene(lib, precision):
+    s = synth.field_scene(seed=6, lines=1024, samples=1024, grid=(12, 12), range_ramp=3.0,
+                          azimuth_quadratic=1.5)
+    hip, exact = run(lib, s, precision), fp64(s)
+    err = np.abs(hip["offsets"] - exact["offsets"])
+    print(f"{precision}: {s.pairs} pairs, max |offset - fp64| = {err.max():.2e} px", flush=True)
+    assert err.max() <= MIXED_OFFSET_TOL
+    agree = np.mean(np.all(hip["coarse_loc"] == exact["coarse_loc"], axis=1))
+    assert agree >= 0.98
+
+
+@pytest.mark.parametrize("precision", ["bf16", "fp16"])
+def test_mixed_planted_chip(lib, precision):
+    ref, tgt, rb, tb = synth.planted(32, 8, seed=4)
+    hip = run_hip(lib, ref, tgt, rb, tb, 32, 48, 2, 4, 4, precision=precision)
+    P = 17
+    for i in range(P):
+        for j in range(P):
+            k = i * P + j
+            assert tuple(hip["coarse_loc"][k]) == (i, j)
+            assert abs(hip["gamma"][k, i, j] - 1.0) < 1e-2
+
+
+def test_mixed_float32_rasters(lib):
+    rng = np.random.default_rng(12)
+    ref = synth.lognormal_texture((384, 384), 0.5, rng)
+    tgt = np.real(synth.fourier_shift(ref, -2.5, 1.75)).astype(np.float32)
+    tgt = tgt * (1 + 0.05 * rng.standard_normal(tgt.shape)).astype(np.float32)
+    s = synth.Scene("amp", ref, tgt, 32, 16, 2, 8, 8, (4, 4))
+    s.plan()
+    hip, exact = run(lib, s, "fp16"), fp64(s)
+    assert np.abs(hip["offsets"] - exact["offsets"]).max() <= MIXED_OFFSET_TOL

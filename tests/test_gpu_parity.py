@@ -53,6 +53,18 @@ def test_shifted_speckle_matches_oracles(lib):
     assert np.max(np.abs(hip["offsets"] - s.truth)) < 0.05
 
 
+def test_golden_shift_end_to_end(lib):
+    """The committed fixture: every pair recovers the planted Fourier shift."""
+    import os
+    f = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "shift_e2e.npz"))
+    C, W, rf, rm, zf = (int(v) for v in f["geometry"])
+    hip = run_hip(lib, f["reference"], f["target"], f["ref_begin"], f["tgt_begin"], C, W, rf, rm, zf)
+    assert np.max(np.abs(hip["offsets"] - f["shift"])) < 0.05
+    exact = pyoracle.adjust(f["reference"], f["target"], f["ref_begin"], f["tgt_begin"], C, W, rf, rm,
+                            zf, form="fp64")
+    np.testing.assert_allclose(hip["offsets"], exact["offsets"], rtol=0, atol=OFFSET_TOL)
+
+
 def test_per_pair_calls_equal_batched(lib):
     s = synth.shifted_scene(seed=3, size=256, grid=(4, 4))
     a = run_hip(lib, s.reference, s.target, s.ref_begin, s.tgt_begin, s.chip, s.window,

@@ -19,6 +19,18 @@ struct PairDesc {
     int32_t tgt_line, tgt_sample;  // upper-left corner of the W x W window
 };
 
+// per-pair constants of the mixed normalizer, unscaled units
+struct MixedStats {
+    double sumr;     // sum of the quantized chip values (unscaled)
+    double sumr2;    // sum of their squares
+    double scale_r;  // power-of-two scale of the quantized chip
+    double scale_t;  // power-of-two scale of the quantized window
+    double mean_t;   // constant removed from the window before rounding
+};
+// quantization modes of the upsampler: detected FP32 only, or BF16 / FP16
+// operands of the mixed contraction as well
+enum UpsampleOutput { kUpFP32 = 0, kUpBF16 = 1, kUpFP16 = 2 };
+
 // tiling of one correlation launch: placements [u0, u0+ub) x [v0, v0+vb) of one
 // pair, chip columns split into `nsplit` ranges, each its own workgroup
 struct CorrelateShape {
@@ -53,18 +65,24 @@ hipError_t launch_coarse_peak(hipStream_t s, const float *gamma, int pairs, int 
 // role 0 = reference chips (N = C, origin = chip corner), role 1 = target
 // windows (N = E, origin = window corner + nudged (top, left)); U is the
 // (rf N) x N upsampling matrix stored padded to multiples of 16, zero-filled;
-// g (rf = 2 only) is its half-sample row, U[2a + 1][n] = g((a - n) mod N)
+// g (rf = 2 only) is its half-sample row, U[2a + 1][n] = g((a - n) mod N).
+// With a quantized output mode (rf = 2, N <= 96) the kernel also writes the
+// mixed-contraction operand: role 0 the chip rows q[pair][M][cs] (16 zero
+// columns on the left), role 1 the window q[pair][M][M]; and its share of
+// the pair's MixedStats.  amp may be null for role 0 then.
+bool upsample_fuses_quantization(int N, int rf);
 hipError_t launch_upsample(hipStream_t s, const PairDesc *desc, int pairs, int dtype, int role,
                            const int2 *window, int N, int rf, const float *U, const float *g,
-                           float *amp);
+                           float *amp, int qmode = kUpFP32, uint16_t *q = nullptr, int cs = 0,
+                           MixedStats *stats = nullptr);
+// mixed coarse stage in one pass when the tiles fit in LDS: gather, detect,
+// write the FP32 window (for the window sums) and both quantized operands
+bool gather_prep_fits(int C, int W);
+hipError_t launch_gather_prep_mixed(hipStream_t s, const PairDesc *desc, int pairs, int dtype,
+                                    int C, int W, int CS, bool fp16, float *tgt_amp,
+                                    uint16_t *chip_q, uint16_t *tgt_q, MixedStats *stats);
 // ---- mixed precision (mixed.hip) ----
-// per-pair constants of the mixed normalizer, unscaled units
-struct MixedStats {
-    double sumr;    // sum of the quantized zero-mean chip
-    double normr;   // its L2 norm
-    double inv_ab;  // 1 / (chip scale * target scale)
-    double mean_t;  // window mean removed from the quantized target
-};
+
 // tiling of the MFMA contraction: 16 x 16 placement tiles, nk K-steps of 16
 // target columns per tile, workgroups of wg_ut waves x wg_vt v-tiles
 struct MixedShape {

@@ -171,6 +171,7 @@ __global__ __launch_bounds__(kWave) void k_window_sums(const float *__restrict__
         const int x = lane + kWave * q;
         V1[q] = V2[q] = 0.0;
         if (x < W)
+#pragma unroll 8
             for (int i = 0; i < C; ++i) {
                 const double d = (double)t[(size_t)i * W + x];
                 V1[q] = V1[q] + d;
@@ -519,12 +520,38 @@ __device__ inline float amplitude(float re, float im)
     return (float)sqrt(r * r + i * i);
 }
 
-template <int DTYPE>
+__device__ inline uint16_t quantize(int qmode, float x)
+{
+    if (qmode == kUpFP16) return __builtin_bit_cast(uint16_t, (_Float16)x);
+    const uint32_t b = __float_as_uint(x);
+    return (uint16_t)((b + 0x7fffu + ((b >> 16) & 1u)) >> 16);
+}
+
+__device__ inline float dequantize(int qmode, uint16_t h)
+{
+    if (qmode == kUpFP16) return (float)__builtin_bit_cast(_Float16, h);
+    return __uint_as_float((uint32_t)h << 16);
+}
+
+// power of two bringing max-abs into [0.5, 1); 1 for BF16, whose range is FP32's
+__device__ inline double quant_scale(int qmode, float maxabs)
+{
+    if (qmode != kUpFP16 || !(maxabs > 0.0f)) return 1.0;
+    int e;
+    frexp((double)maxabs, &e);
+    return ldexp(1.0, -e);
+}
+
+template <int DTYPE, int QMODE>
 __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2(
     const PairDesc *__restrict__ desc, int role, const int2 *__restrict__ window, int N, int Np,
-    const float *__restrict__ g, float *__restrict__ amp)
+    const float *__restrict__ g, float *__restrict__ amp, uint16_t *__restrict__ qout, int cs,
+    MixedStats *__restrict__ stats)
 {
     extern __shared__ float lds[];
+    __shared__ double red[kUp2Waves * kWave];
+    __shared__ double red2[kUp2Waves * kWave];
+    __shared__ float qconst[2];
     const int ls = Np + 4;
     float *xr = lds, *xi = xr + Np * ls, *tr = xi + Np * ls, *ti = tr + Np * ls;
     float *gs = ti + Np * ls;
@@ -542,6 +569,8 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2(
         line += tl.x;
         sample += tl.y;
     }
+    double asum = 0.0;
+    float amax = 0.0f;
     for (int k = threadIdx.x; k < Np * Np; k += blockDim.x) {
         const int r = k / Np, c = k - r * Np;
         float re = 0.0f, im = 0.0f;
@@ -554,18 +583,64 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2(
             } else {
                 re = static_cast<const float *>(base)[off];
             }
+            if (QMODE != kUpFP32) {
+                const float a = amplitude(re, im);
+                asum += a;
+                amax = fmaxf(amax, a);
+            }
         }
         xr[r * ls + c] = re;
         xi[r * ls + c] = im;
     }
     for (int k = threadIdx.x; k < N; k += blockDim.x) gs[k] = g[k];
-    __syncthreads();
-
     const int M = 2 * N;
-    float *out = amp + (size_t)pair * M * M;
+    if (QMODE != kUpFP32) {
+        // the quantization constants come from the coarse tile: its mean
+        // amplitude is removed, and twice its peak bounds the upsampled values
+        red[threadIdx.x] = asum;
+        red2[threadIdx.x] = amax;
+        __syncthreads();
+        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+            if (threadIdx.x < s) {
+                red[threadIdx.x] += red[threadIdx.x + s];
+                red2[threadIdx.x] = fmax(red2[threadIdx.x], red2[threadIdx.x + s]);
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            qconst[0] = (float)(red[0] / ((double)N * N));
+            qconst[1] = (float)quant_scale(QMODE, 2.0f * (float)red2[0]);
+        }
+        // zero padding of the chip rows
+        if (role == 0)
+            for (int k = threadIdx.x; k < M * (cs - M); k += blockDim.x) {
+                const int p = k / (cs - M), c = k - p * (cs - M);
+                const int col = c < 16 ? c : M + c;
+                qout[((size_t)pair * M + p) * cs + col] = 0;
+            }
+    }
+    __syncthreads();
+    const float qc = (QMODE != kUpFP32) ? qconst[0] : 0.0f;
+    const float qs = (QMODE != kUpFP32) ? qconst[1] : 1.0f;
+    float *out = amp ? amp + (size_t)pair * M * M : nullptr;
+    double s1 = 0.0, s2 = 0.0;
+    auto emit = [&](int p, int q, float v) {
+        if (QMODE == kUpFP32 || role == 1) out[(size_t)p * M + q] = v;
+        if (QMODE != kUpFP32) {
+            const uint16_t h = quantize(QMODE, (v - qc) * qs);
+            if (role == 0) {
+                qout[((size_t)pair * M + p) * cs + 16 + q] = h;
+                const double back = (double)dequantize(QMODE, h) / qs;
+                s1 += back;
+                s2 += back * back;
+            } else {
+                qout[((size_t)pair * M + p) * M + q] = h;
+            }
+        }
+    };
     for (int k = threadIdx.x; k < N * N; k += blockDim.x) {
         const int a = k / N, b = k - a * N;
-        out[(size_t)(2 * a) * M + 2 * b] = amplitude(xr[a * ls + b], xi[a * ls + b]);
+        emit(2 * a, 2 * b, amplitude(xr[a * ls + b], xi[a * ls + b]));
     }
 
     const int tiles = Np / 16;
@@ -574,7 +649,9 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2(
         const int mt = tile / tiles, nt = tile - mt * tiles;
         const int m = 16 * mt + lm;
         floatx4 accR = {0.0f, 0.0f, 0.0f, 0.0f}, accI = {0.0f, 0.0f, 0.0f, 0.0f};
-        for (int k0 = 0; k0 < Np; k0 += 4) {
+        for (int k16 = 0; k16 < Np; k16 += 16)
+#pragma unroll
+        for (int k0 = k16; k0 < k16 + 16; k0 += 4) {
             const int k = k0 + lk;
             int dd = m - k;
             dd += (dd < 0) ? N : 0;
@@ -592,7 +669,7 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2(
     __syncthreads();
     for (int k = threadIdx.x; k < N * N; k += blockDim.x) {
         const int a = k / N, b = k - a * N;
-        out[(size_t)(2 * a + 1) * M + 2 * b] = amplitude(tr[a * ls + b], ti[a * ls + b]);
+        emit(2 * a + 1, 2 * b, amplitude(tr[a * ls + b], ti[a * ls + b]));
     }
     // stage 2: odd columns, y[2a + s][2b + 1] = (T_s H^T)[a][b]
     for (int tile = wave; tile < 2 * tiles * tiles; tile += kUp2Waves) {
@@ -603,7 +680,9 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2(
         const float *Ai = sphase ? ti : xi;
         const int n = 16 * nt + lm;
         floatx4 accR = {0.0f, 0.0f, 0.0f, 0.0f}, accI = {0.0f, 0.0f, 0.0f, 0.0f};
-        for (int k0 = 0; k0 < Np; k0 += 4) {
+        for (int k16 = 0; k16 < Np; k16 += 16)
+#pragma unroll
+        for (int k0 = k16; k0 < k16 + 16; k0 += 4) {
             const int k = k0 + lk;
             int dd = n - k;
             dd += (dd < 0) ? N : 0;
@@ -614,8 +693,115 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2(
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int a = 16 * mt + 4 * lk + i;
-            if (a < N && n < N) out[(size_t)(2 * a + sphase) * M + 2 * n + 1] = amplitude(accR[i], accI[i]);
+            if (a < N && n < N) emit(2 * a + sphase, 2 * n + 1, amplitude(accR[i], accI[i]));
         }
+    }
+    if (QMODE != kUpFP32) {
+        if (role == 0) {
+            __syncthreads();
+            red[threadIdx.x] = s1;
+            red2[threadIdx.x] = s2;
+            __syncthreads();
+            for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+                if (threadIdx.x < s) {
+                    red[threadIdx.x] += red[threadIdx.x + s];
+                    red2[threadIdx.x] += red2[threadIdx.x + s];
+                }
+                __syncthreads();
+            }
+            if (threadIdx.x == 0) {
+                stats[pair].sumr = red[0];
+                stats[pair].sumr2 = red2[0];
+                stats[pair].scale_r = qs;
+            }
+        } else if (threadIdx.x == 0) {
+            stats[pair].scale_t = qs;
+            stats[pair].mean_t = qc;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// mixed coarse stage in one pass: one workgroup per pair gathers and detects
+// the chip into LDS (its exact mean is removed before rounding), then streams
+// the window: FP32 amplitudes for the exact window sums, and the quantized
+// operand with the chip's mean amplitude removed -- any constant works there,
+// the normalizer takes it back exactly, and the chip's mean is close to the
+// window's for the same scene
+// ---------------------------------------------------------------------------
+constexpr int kGatherPrepThreads = 256;
+constexpr size_t kGatherPrepLds = 64 * 1024;
+
+template <int DTYPE, int QMODE>
+__global__ __launch_bounds__(kGatherPrepThreads) void k_gather_prep_mixed(
+    const PairDesc *__restrict__ desc, int C, int W, int CS, float *__restrict__ tgt_amp,
+    uint16_t *__restrict__ chip_q, uint16_t *__restrict__ tgt_q, MixedStats *__restrict__ stats)
+{
+    extern __shared__ float lds[];
+    __shared__ double red[kGatherPrepThreads];
+    __shared__ double red2[kGatherPrepThreads];
+    const int pair = blockIdx.x;
+    const int t = threadIdx.x;
+    const PairDesc d = desc[pair];
+    float *ra = lds;
+    auto reduce = [&](double &a, double &b, bool maxb) {
+        red[t] = a;
+        red2[t] = b;
+        __syncthreads();
+        for (int s = kGatherPrepThreads / 2; s > 0; s >>= 1) {
+            if (t < s) {
+                red[t] += red[t + s];
+                red2[t] = maxb ? fmax(red2[t], red2[t + s]) : red2[t] + red2[t + s];
+            }
+            __syncthreads();
+        }
+        a = red[0];
+        b = red2[0];
+        __syncthreads();
+    };
+    double rs = 0.0, rm = 0.0;
+    for (int k = t; k < C * C; k += kGatherPrepThreads) {
+        const int i = k / C, j = k - i * C;
+        const float v = detect(DTYPE, d.ref, (size_t)(d.ref_line + i) * d.ref_stride + d.ref_sample + j);
+        ra[k] = v;
+        rs += v;
+        rm = fmax(rm, (double)v);
+    }
+    reduce(rs, rm, true);
+    const float cr = (float)(rs / ((double)C * C));
+    const float ar = (float)quant_scale(QMODE, (float)rm);
+    // headroom for window values far above the chip's
+    const float at = (float)quant_scale(QMODE, 16.0f * (float)rm);
+    uint16_t *cq = chip_q + (size_t)pair * C * CS;
+    double s1 = 0.0, s2 = 0.0;
+    for (int k = t; k < C * CS; k += kGatherPrepThreads) {
+        const int i = k / CS, j = k - i * CS - 16;
+        uint16_t h = 0;
+        if (j >= 0 && j < C) {
+            h = quantize(QMODE, (ra[i * C + j] - cr) * ar);
+            const double back = (double)dequantize(QMODE, h) / ar;
+            s1 += back;
+            s2 += back * back;
+        }
+        cq[k] = h;
+    }
+    float *ga = tgt_amp + (size_t)pair * W * W;
+    uint16_t *tq = tgt_q + (size_t)pair * W * W;
+    for (int k = t; k < W * W; k += kGatherPrepThreads) {
+        const int i = k / W, j = k - i * W;
+        const float v = detect(DTYPE, d.tgt, (size_t)(d.tgt_line + i) * d.tgt_stride + d.tgt_sample + j);
+        ga[k] = v;
+        tq[k] = quantize(QMODE, (v - cr) * at);
+    }
+    reduce(s1, s2, false);
+    if (t == 0) {
+        MixedStats st;
+        st.sumr = s1;
+        st.sumr2 = s2;
+        st.scale_r = ar;
+        st.scale_t = at;
+        st.mean_t = cr;
+        stats[pair] = st;
     }
 }
 
@@ -852,21 +1038,32 @@ hipError_t launch_coarse_peak(hipStream_t s, const float *gamma, int pairs, int 
     return hipGetLastError();
 }
 
+bool upsample_fuses_quantization(int N, int rf) { return rf == 2 && (N + 15) / 16 * 16 <= kUp2MaxN; }
+
 hipError_t launch_upsample(hipStream_t s, const PairDesc *desc, int pairs, int dtype, int role,
                            const int2 *window, int N, int rf, const float *U, const float *g,
-                           float *amp)
+                           float *amp, int qmode, uint16_t *q, int cs, MixedStats *stats)
 {
     const int Np = (N + 15) / 16 * 16;
     if (rf == 2 && Np <= kUp2MaxN && g) {
         const size_t lds = (4 * (size_t)Np * (Np + 4) + Np) * sizeof(float);
-        if (dtype == 0)
-            hipLaunchKernelGGL(k_upsample2<0>, dim3(pairs), dim3(kUp2Waves * kWave), lds, s, desc,
-                               role, window, N, Np, g, amp);
-        else
-            hipLaunchKernelGGL(k_upsample2<1>, dim3(pairs), dim3(kUp2Waves * kWave), lds, s, desc,
-                               role, window, N, Np, g, amp);
+        const dim3 grid(pairs), block(kUp2Waves * kWave);
+#define AMPCOR_UP2(DT, QM)                                                                        \
+    hipLaunchKernelGGL((k_upsample2<DT, QM>), grid, block, lds, s, desc, role, window, N, Np, g, \
+                       amp, q, cs, stats)
+        if (dtype == 0) {
+            if (qmode == kUpBF16) AMPCOR_UP2(0, kUpBF16);
+            else if (qmode == kUpFP16) AMPCOR_UP2(0, kUpFP16);
+            else AMPCOR_UP2(0, kUpFP32);
+        } else {
+            if (qmode == kUpBF16) AMPCOR_UP2(1, kUpBF16);
+            else if (qmode == kUpFP16) AMPCOR_UP2(1, kUpFP16);
+            else AMPCOR_UP2(1, kUpFP32);
+        }
+#undef AMPCOR_UP2
         return hipGetLastError();
     }
+    if (qmode != kUpFP32) return hipErrorInvalidValue;  // callers quantize separately
     const int M = rf * N;
     const int Mp = (M + 15) / 16 * 16;
     const size_t lds = (size_t)kUpWaves * 2 * 16 * Np * sizeof(float);
@@ -876,6 +1073,28 @@ hipError_t launch_upsample(hipStream_t s, const PairDesc *desc, int pairs, int d
     else
         hipLaunchKernelGGL(k_upsample<1>, dim3(pairs), dim3(kUpWaves * kWave), lds, s, desc, role,
                            window, N, Np, M, Mp, U, amp);
+    return hipGetLastError();
+}
+
+bool gather_prep_fits(int C, int W) { return (size_t)C * C * sizeof(float) <= kGatherPrepLds; }
+
+hipError_t launch_gather_prep_mixed(hipStream_t s, const PairDesc *desc, int pairs, int dtype,
+                                    int C, int W, int CS, bool fp16, float *tgt_amp,
+                                    uint16_t *chip_q, uint16_t *tgt_q, MixedStats *stats)
+{
+    const size_t lds = (size_t)C * C * sizeof(float);
+    const dim3 grid(pairs), block(kGatherPrepThreads);
+#define AMPCOR_GP(DT, QM)                                                                            \
+    hipLaunchKernelGGL((k_gather_prep_mixed<DT, QM>), grid, block, lds, s, desc, C, W, CS, tgt_amp, \
+                       chip_q, tgt_q, stats)
+    if (dtype == 0) {
+        if (fp16) AMPCOR_GP(0, kUpFP16);
+        else AMPCOR_GP(0, kUpBF16);
+    } else {
+        if (fp16) AMPCOR_GP(1, kUpFP16);
+        else AMPCOR_GP(1, kUpBF16);
+    }
+#undef AMPCOR_GP
     return hipGetLastError();
 }
 

@@ -123,7 +123,7 @@ __global__ __launch_bounds__(kPrepThreads) void k_prep_mixed(
         cq[k] = q;
     }
     const double sumr = block_sum(pd, s1);
-    const double normr = sqrt(block_sum(pd, s2));
+    const double sumr2 = block_sum(pd, s2);
 
     acc = 0.0;
     for (int k = t; k < W * W; k += kPrepThreads) acc += g[k];
@@ -138,8 +138,9 @@ __global__ __launch_bounds__(kPrepThreads) void k_prep_mixed(
     if (t == 0) {
         MixedStats st;
         st.sumr = sumr;
-        st.normr = normr;
-        st.inv_ab = 1.0 / (a * b);
+        st.sumr2 = sumr2;
+        st.scale_r = a;
+        st.scale_t = b;
         st.mean_t = (double)mean_t;
         stats[pair] = st;
     }
@@ -283,8 +284,10 @@ __global__ __launch_bounds__(256) void k_normalize_mixed(const float *__restrict
     const double n = (double)C * C;
     const double mu = ws.x / n;
     const double var = ws.y - mu * ws.x;
-    const double N = (double)num * st.inv_ab + (st.mean_t - mu) * st.sumr;
-    const double B = (var > 0.0 && st.normr > 0.0) ? st.normr * sqrt(var) : 0.0;
+    // the numerator does not see a constant added to the chip, the norm is centred
+    const double N = (double)num / (st.scale_r * st.scale_t) + (st.mean_t - mu) * st.sumr;
+    const double c2 = st.sumr2 - st.sumr * st.sumr / n;
+    const double B = (var > 0.0 && c2 > 0.0) ? sqrt(c2) * sqrt(var) : 0.0;
     gamma[idx] = (B > 0.0) ? (float)(N / B) : __builtin_nanf("");
 }
 

@@ -6,8 +6,7 @@
 // gathers the tiles straight from device-resident rasters.  All device
 // buffers are allocated once, at plan creation, and every stage runs on the
 // plan's own stream with no intermediate synchronization.
-#include "ampcor_hip.h"
-#include "kernels.h"
+#include "plan_internal.h"
 
 #include <cmath>
 #include <cstdio>
@@ -20,17 +19,11 @@
 
 using namespace ampcor_hip;
 
-struct ampcor_hip_raster {
-    int device;
-    int dtype;
-    const void *data;
-    size_t lines, samples;
-    bool owned;
-};
-
 namespace {
-
 thread_local std::string g_error;
+}  // namespace
+
+namespace ampcor_hip {
 
 int fail(int code, const std::string &message)
 {
@@ -43,11 +36,9 @@ int hip_fail(hipError_t e, const char *what)
     return fail(AMPCOR_HIP_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-#define HIP_TRY(call, what)                                  \
-    do {                                                     \
-        hipError_t _e = (call);                              \
-        if (_e != hipSuccess) return hip_fail(_e, (what));   \
-    } while (0)
+}  // namespace ampcor_hip
+
+namespace {
 
 // one entry of the band-limited upsampling kernel (closed form):
 // t = p/rf - n; odd N: sin(pi t) / (N sin(pi t / N));
@@ -69,117 +60,9 @@ double dirichlet(int R, int zf, int d)
     return std::sin(M_PI * x) / (R * std::sin(M_PI * x / R));
 }
 
-template <typename T>
-struct DeviceBuffer {
-    T *ptr = nullptr;
-    size_t count = 0;
-    hipError_t alloc(size_t n)
-    {
-        count = n;
-        return hipMalloc((void **)&ptr, n * sizeof(T) > 0 ? n * sizeof(T) : 1);
-    }
-    void release()
-    {
-        if (ptr) (void)hipFree(ptr);
-        ptr = nullptr;
-    }
-};
-
-const char *const kStageNames[] = {
-    "gather_detect", "chip_stats", "window_sums", "correlate", "normalize", "coarse_peak",
-    "upsample_ref", "upsample_tgt", "chip_stats_refined", "window_sums_refined", "correlate_refined",
-    "normalize_refined", "zoom",
-};
-constexpr int kStages = sizeof(kStageNames) / sizeof(kStageNames[0]);
-// the mixed forms quantize instead of computing FP32 chip statistics
-const char *const kStageNamesMixed[] = {
-    "gather_detect", "prep", "window_sums", "correlate", "normalize", "coarse_peak",
-    "upsample_ref", "upsample_tgt", "prep_refined", "window_sums_refined", "correlate_refined",
-    "normalize_refined", "zoom",
-};
-static_assert(sizeof(kStageNamesMixed) == sizeof(kStageNames), "stage tables differ");
-
 }  // namespace
 
-struct ampcor_hip_plan {
-    int device = 0;
-    int precision = AMPCOR_HIP_FP32;
-    int dtype = -1;
-    size_t pairs = 0;
-    int C = 0, W = 0, m = 0, P = 0, rf = 0, rm = 0, E = 0, Cr = 0, Wr = 0, R = 0, zf = 0, Z = 0;
-    hipStream_t stream = nullptr;
-
-    std::vector<PairDesc> desc;
-    std::vector<unsigned char> have_ref, have_tgt;
-    bool desc_dirty = true;
-    bool adjusted = false;
-
-    CorrelateShape coarse_shape{}, refined_shape{};
-    MixedShape mixed_coarse{}, mixed_refined{};
-    bool mixed = false;
-
-    DeviceBuffer<PairDesc> d_desc;
-    DeviceBuffer<float> amp_ref, amp_tgt, rT, partial, gamma;
-    DeviceBuffer<double2> sums, sums_r;
-    DeviceBuffer<double2> stats, stats_r;
-    DeviceBuffer<int2> loc, window, zoom_loc;
-    DeviceBuffer<float> peak, snr, peak_r, offsets;
-    DeviceBuffer<float> amp_ref_r, amp_tgt_r, rT_r, partial_r, gamma_r;
-    DeviceBuffer<float> U_ref, U_tgt, g_ref, g_tgt;
-    DeviceBuffer<double> D;
-    DeviceBuffer<uint16_t> chip_q, tgt_q, chip_q_r, tgt_q_r;
-    DeviceBuffer<MixedStats> mstats, mstats_r;
-
-    bool timing = false;
-    hipEvent_t ev[kStages + 1] = {};
-    float stage_ms[kStages] = {};
-
-    void release()
-    {
-        DeviceBuffer<float> *fb[] = {&amp_ref, &amp_tgt, &rT, &partial, &gamma, &peak, &snr,
-                                     &peak_r, &offsets, &amp_ref_r, &amp_tgt_r, &rT_r,
-                                     &partial_r, &gamma_r, &U_ref, &U_tgt, &g_ref, &g_tgt};
-        for (auto *b : fb) b->release();
-        sums.release();
-        sums_r.release();
-        chip_q.release();
-        tgt_q.release();
-        chip_q_r.release();
-        tgt_q_r.release();
-        mstats.release();
-        mstats_r.release();
-        D.release();
-        stats.release();
-        stats_r.release();
-        loc.release();
-        window.release();
-        zoom_loc.release();
-        d_desc.release();
-        for (auto &e : ev)
-            if (e) {
-                (void)hipEventDestroy(e);
-                e = nullptr;
-            }
-        if (stream) (void)hipStreamDestroy(stream);
-        stream = nullptr;
-    }
-};
-
 namespace {
-
-// makes the plan's device current for the duration of a call
-struct DeviceGuard {
-    int previous = -1;
-    explicit DeviceGuard(int device)
-    {
-        if (hipGetDevice(&previous) != hipSuccess) previous = -1;
-        if (previous != device) (void)hipSetDevice(device);
-    }
-    ~DeviceGuard()
-    {
-        if (previous >= 0) (void)hipSetDevice(previous);
-    }
-};
 
 int check_device(int device)
 {
@@ -547,16 +430,13 @@ int ampcor_hip_set_timing(ampcor_hip_plan *plan, int enabled)
     return AMPCOR_HIP_OK;
 }
 
-int ampcor_hip_adjust(ampcor_hip_plan *plan, float *offsets)
+}  // extern "C"
+
+namespace ampcor_hip {
+
+int plan_enqueue(ampcor_hip_plan *plan, int N)
 {
-    if (!plan) return fail(AMPCOR_HIP_EINVAL, "null plan");
-    for (size_t k = 0; k < plan->pairs; ++k)
-        if (!plan->have_ref[k] || !plan->have_tgt[k])
-            return fail(AMPCOR_HIP_ESTATE, "pair " + std::to_string(k) +
-                                               " has no reference or target tile registered");
-    DeviceGuard guard(plan->device);
     hipStream_t s = plan->stream;
-    const int N = (int)plan->pairs;
     if (plan->desc_dirty) {
         HIP_TRY(hipMemcpyAsync(plan->d_desc.ptr, plan->desc.data(), N * sizeof(PairDesc),
                                hipMemcpyHostToDevice, s),
@@ -564,26 +444,38 @@ int ampcor_hip_adjust(ampcor_hip_plan *plan, float *offsets)
         plan->desc_dirty = false;
     }
     int stage = 0;
-    auto mark = [&]() -> hipError_t {
-        if (!plan->timing) return hipSuccess;
+    auto mark = [&](const char *name) -> hipError_t {
+        if (!plan->timing || stage >= kStages) return hipSuccess;
+        plan->stage_name[stage] = name;
         return hipEventRecord(plan->ev[stage++], s);
     };
-#define STEP(call, what)                        \
-    do {                                        \
-        HIP_TRY(mark(), "recording an event");  \
-        HIP_TRY((call), (what));                \
+#define STEP(call, what)                          \
+    do {                                          \
+        HIP_TRY(mark(what), "recording an event"); \
+        HIP_TRY((call), (what));                  \
     } while (0)
 
     auto *p = plan;
     const bool fp16 = p->precision == AMPCOR_HIP_FP16;
-    STEP(launch_gather_detect(s, p->d_desc.ptr, N, p->dtype, p->C, p->W, p->amp_ref.ptr, p->amp_tgt.ptr),
-         "gather_detect");
-    if (p->mixed)
-        STEP(launch_prep_mixed(s, p->amp_ref.ptr, p->amp_tgt.ptr, N, p->C, p->W, p->mixed_coarse.cs,
-                               fp16, p->chip_q.ptr, p->tgt_q.ptr, p->mstats.ptr),
-             "prep");
-    else
-        STEP(launch_chip_stats(s, p->amp_ref.ptr, N, p->C, p->rT.ptr, p->stats.ptr), "chip_stats");
+    const int qmode = fp16 ? kUpFP16 : kUpBF16;
+    if (p->mixed && gather_prep_fits(p->C, p->W)) {
+        STEP(launch_gather_prep_mixed(s, p->d_desc.ptr, N, p->dtype, p->C, p->W, p->mixed_coarse.cs,
+                                      fp16, p->amp_tgt.ptr, p->chip_q.ptr, p->tgt_q.ptr,
+                                      p->mstats.ptr),
+             "gather_prep");
+    } else {
+        STEP(launch_gather_detect(s, p->d_desc.ptr, N, p->dtype, p->C, p->W, p->amp_ref.ptr,
+                                  p->amp_tgt.ptr),
+             "gather_detect");
+        if (p->mixed)
+            STEP(launch_prep_mixed(s, p->amp_ref.ptr, p->amp_tgt.ptr, N, p->C, p->W,
+                                   p->mixed_coarse.cs, fp16, p->chip_q.ptr, p->tgt_q.ptr,
+                                   p->mstats.ptr),
+                 "prep");
+        else
+            STEP(launch_chip_stats(s, p->amp_ref.ptr, N, p->C, p->rT.ptr, p->stats.ptr),
+                 "chip_stats");
+    }
     STEP(launch_window_sums(s, p->amp_tgt.ptr, N, p->W, p->C, p->sums.ptr), "window_sums");
     if (p->mixed) {
         STEP(launch_correlate_mixed(s, p->mixed_coarse, p->chip_q.ptr, p->tgt_q.ptr, N, p->C, p->W,
@@ -603,20 +495,33 @@ int ampcor_hip_adjust(ampcor_hip_plan *plan, float *offsets)
     STEP(launch_coarse_peak(s, p->gamma.ptr, N, p->P, p->W, p->E, p->rm, p->rf, p->loc.ptr,
                             p->window.ptr, p->peak.ptr, p->snr.ptr),
          "coarse_peak");
-    STEP(launch_upsample(s, p->d_desc.ptr, N, p->dtype, 0, p->window.ptr, p->C, p->rf, p->U_ref.ptr,
-                         p->g_ref.ptr, p->amp_ref_r.ptr),
-         "upsample_ref");
-    STEP(launch_upsample(s, p->d_desc.ptr, N, p->dtype, 1, p->window.ptr, p->E, p->rf, p->U_tgt.ptr,
-                         p->g_tgt.ptr, p->amp_tgt_r.ptr),
-         "upsample_tgt");
-    if (p->mixed)
-        STEP(launch_prep_mixed(s, p->amp_ref_r.ptr, p->amp_tgt_r.ptr, N, p->Cr, p->Wr,
-                               p->mixed_refined.cs, fp16, p->chip_q_r.ptr, p->tgt_q_r.ptr,
-                               p->mstats_r.ptr),
-             "prep_refined");
-    else
-        STEP(launch_chip_stats(s, p->amp_ref_r.ptr, N, p->Cr, p->rT_r.ptr, p->stats_r.ptr),
-             "chip_stats_refined");
+    const bool fused = p->mixed && upsample_fuses_quantization(p->C, p->rf) &&
+                       upsample_fuses_quantization(p->E, p->rf);
+    if (fused) {
+        STEP(launch_upsample(s, p->d_desc.ptr, N, p->dtype, 0, p->window.ptr, p->C, p->rf,
+                             p->U_ref.ptr, p->g_ref.ptr, nullptr, qmode, p->chip_q_r.ptr,
+                             p->mixed_refined.cs, p->mstats_r.ptr),
+             "upsample_ref");
+        STEP(launch_upsample(s, p->d_desc.ptr, N, p->dtype, 1, p->window.ptr, p->E, p->rf,
+                             p->U_tgt.ptr, p->g_tgt.ptr, p->amp_tgt_r.ptr, qmode, p->tgt_q_r.ptr, 0,
+                             p->mstats_r.ptr),
+             "upsample_tgt");
+    } else {
+        STEP(launch_upsample(s, p->d_desc.ptr, N, p->dtype, 0, p->window.ptr, p->C, p->rf,
+                             p->U_ref.ptr, p->g_ref.ptr, p->amp_ref_r.ptr),
+             "upsample_ref");
+        STEP(launch_upsample(s, p->d_desc.ptr, N, p->dtype, 1, p->window.ptr, p->E, p->rf,
+                             p->U_tgt.ptr, p->g_tgt.ptr, p->amp_tgt_r.ptr),
+             "upsample_tgt");
+        if (p->mixed)
+            STEP(launch_prep_mixed(s, p->amp_ref_r.ptr, p->amp_tgt_r.ptr, N, p->Cr, p->Wr,
+                                   p->mixed_refined.cs, fp16, p->chip_q_r.ptr, p->tgt_q_r.ptr,
+                                   p->mstats_r.ptr),
+                 "prep_refined");
+        else
+            STEP(launch_chip_stats(s, p->amp_ref_r.ptr, N, p->Cr, p->rT_r.ptr, p->stats_r.ptr),
+                 "chip_stats_refined");
+    }
     STEP(launch_window_sums(s, p->amp_tgt_r.ptr, N, p->Wr, p->Cr, p->sums_r.ptr),
          "window_sums_refined");
     if (p->mixed) {
@@ -638,16 +543,37 @@ int ampcor_hip_adjust(ampcor_hip_plan *plan, float *offsets)
                      p->rf, p->offsets.ptr, p->peak_r.ptr, p->zoom_loc.ptr),
          "zoom");
 #undef STEP
-    HIP_TRY(mark(), "recording an event");
+    plan->stages_run = stage;
+    if (plan->timing) HIP_TRY(mark("end"), "recording an event");
+    return AMPCOR_HIP_OK;
+}
+
+}  // namespace ampcor_hip
+
+extern "C" {
+
+int ampcor_hip_adjust(ampcor_hip_plan *plan, float *offsets)
+{
+    if (!plan) return fail(AMPCOR_HIP_EINVAL, "null plan");
+    for (size_t k = 0; k < plan->pairs; ++k)
+        if (!plan->have_ref[k] || !plan->have_tgt[k])
+            return fail(AMPCOR_HIP_ESTATE, "pair " + std::to_string(k) +
+                                               " has no reference or target tile registered");
+    DeviceGuard guard(plan->device);
+    hipStream_t s = plan->stream;
+    const int N = (int)plan->pairs;
+    if (int rc = plan_enqueue(plan, N)) return rc;
+    const int stages = plan->stages_run;
     if (offsets)
-        HIP_TRY(hipMemcpyAsync(offsets, p->offsets.ptr, 2 * (size_t)N * sizeof(float),
+        HIP_TRY(hipMemcpyAsync(offsets, plan->offsets.ptr, 2 * (size_t)N * sizeof(float),
                                hipMemcpyDeviceToHost, s),
                 "copying offsets to the host");
     HIP_TRY(hipStreamSynchronize(s), "adjust");
-    if (plan->timing)
-        for (int k = 0; k < kStages; ++k)
+    if (plan->timing) {
+        for (int k = 0; k < stages; ++k)
             HIP_TRY(hipEventElapsedTime(&plan->stage_ms[k], plan->ev[k], plan->ev[k + 1]),
                     "reading stage times");
+    }
     plan->adjusted = true;
     return AMPCOR_HIP_OK;
 }
@@ -656,12 +582,12 @@ int ampcor_hip_stage_times(ampcor_hip_plan *plan, int capacity, int *count, floa
                            const char **names)
 {
     if (!plan || !count) return fail(AMPCOR_HIP_EINVAL, "null argument");
-    *count = kStages;
-    if (!plan->timing || !plan->adjusted)
+    *count = plan->stages_run;
+    if (!plan->timing || !plan->adjusted || plan->stages_run == 0)
         return fail(AMPCOR_HIP_ESTATE, "no timed adjust has run on this plan");
-    for (int k = 0; k < kStages && k < capacity; ++k) {
+    for (int k = 0; k < plan->stages_run && k < capacity; ++k) {
         if (ms) ms[k] = plan->stage_ms[k];
-        if (names) names[k] = plan->mixed ? kStageNamesMixed[k] : kStageNames[k];
+        if (names) names[k] = plan->stage_name[k];
     }
     return AMPCOR_HIP_OK;
 }

@@ -1,0 +1,138 @@
+// plan_internal.h -- plan and raster internals shared by plan.hip and server.hip.
+#pragma once
+
+#include "ampcor_hip.h"
+#include "kernels.h"
+
+#include <string>
+#include <vector>
+
+struct ampcor_hip_raster {
+    int device;
+    int dtype;
+    const void *data;
+    size_t lines, samples;
+    bool owned;
+};
+
+namespace ampcor_hip {
+
+// records a failure for ampcor_hip_last_error() and returns the code
+int fail(int code, const std::string &message);
+int hip_fail(hipError_t e, const char *what);
+
+#define HIP_TRY(call, what)                                  \
+    do {                                                     \
+        hipError_t _e = (call);                              \
+        if (_e != hipSuccess) return hip_fail(_e, (what));   \
+    } while (0)
+
+template <typename T>
+struct DeviceBuffer {
+    T *ptr = nullptr;
+    size_t count = 0;
+    hipError_t alloc(size_t n)
+    {
+        count = n;
+        return hipMalloc((void **)&ptr, n * sizeof(T) > 0 ? n * sizeof(T) : 1);
+    }
+    void release()
+    {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+    }
+};
+
+constexpr int kStages = 16;  // room for every stage of any form
+
+// makes the plan's device current for the duration of a call
+struct DeviceGuard {
+    int previous = -1;
+    explicit DeviceGuard(int device)
+    {
+        if (hipGetDevice(&previous) != hipSuccess) previous = -1;
+        if (previous != device) (void)hipSetDevice(device);
+    }
+    ~DeviceGuard()
+    {
+        if (previous >= 0) (void)hipSetDevice(previous);
+    }
+};
+
+}  // namespace ampcor_hip
+
+struct ampcor_hip_plan {
+    using PairDesc = ampcor_hip::PairDesc;
+    int device = 0;
+    int precision = AMPCOR_HIP_FP32;
+    int dtype = -1;
+    size_t pairs = 0;
+    int C = 0, W = 0, m = 0, P = 0, rf = 0, rm = 0, E = 0, Cr = 0, Wr = 0, R = 0, zf = 0, Z = 0;
+    hipStream_t stream = nullptr;
+
+    std::vector<PairDesc> desc;
+    std::vector<unsigned char> have_ref, have_tgt;
+    bool desc_dirty = true;
+    bool adjusted = false;
+
+    ampcor_hip::CorrelateShape coarse_shape{}, refined_shape{};
+    ampcor_hip::MixedShape mixed_coarse{}, mixed_refined{};
+    bool mixed = false;
+
+    ampcor_hip::DeviceBuffer<PairDesc> d_desc;
+    ampcor_hip::DeviceBuffer<float> amp_ref, amp_tgt, rT, partial, gamma;
+    ampcor_hip::DeviceBuffer<double2> sums, sums_r;
+    ampcor_hip::DeviceBuffer<double2> stats, stats_r;
+    ampcor_hip::DeviceBuffer<int2> loc, window, zoom_loc;
+    ampcor_hip::DeviceBuffer<float> peak, snr, peak_r, offsets;
+    ampcor_hip::DeviceBuffer<float> amp_ref_r, amp_tgt_r, rT_r, partial_r, gamma_r;
+    ampcor_hip::DeviceBuffer<float> U_ref, U_tgt, g_ref, g_tgt;
+    ampcor_hip::DeviceBuffer<double> D;
+    ampcor_hip::DeviceBuffer<uint16_t> chip_q, tgt_q, chip_q_r, tgt_q_r;
+    ampcor_hip::DeviceBuffer<ampcor_hip::MixedStats> mstats, mstats_r;
+
+    bool timing = false;
+    hipEvent_t ev[ampcor_hip::kStages + 1] = {};
+    float stage_ms[ampcor_hip::kStages] = {};
+    const char *stage_name[ampcor_hip::kStages] = {};
+    int stages_run = 0;
+
+    void release()
+    {
+        ampcor_hip::DeviceBuffer<float> *fb[] = {&amp_ref, &amp_tgt, &rT, &partial, &gamma, &peak, &snr,
+                                     &peak_r, &offsets, &amp_ref_r, &amp_tgt_r, &rT_r,
+                                     &partial_r, &gamma_r, &U_ref, &U_tgt, &g_ref, &g_tgt};
+        for (auto *b : fb) b->release();
+        sums.release();
+        sums_r.release();
+        chip_q.release();
+        tgt_q.release();
+        chip_q_r.release();
+        tgt_q_r.release();
+        mstats.release();
+        mstats_r.release();
+        D.release();
+        stats.release();
+        stats_r.release();
+        loc.release();
+        window.release();
+        zoom_loc.release();
+        d_desc.release();
+        for (auto &e : ev)
+            if (e) {
+                (void)hipEventDestroy(e);
+                e = nullptr;
+            }
+        if (stream) (void)hipStreamDestroy(stream);
+        stream = nullptr;
+    }
+};
+
+
+namespace ampcor_hip {
+// enqueue every stage for the first n registered pairs on the plan's stream;
+// uploads the descriptors first when they changed; no synchronization
+int plan_enqueue(ampcor_hip_plan *plan, int n);
+// device bytes a plan of this geometry allocates per pair (for admission)
+size_t plan_bytes_per_pair(int C, int W, int rf, int rm, int precision);
+}  // namespace ampcor_hip

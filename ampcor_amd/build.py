@@ -20,7 +20,7 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 ARCH = os.environ.get("AMPCOR_HIP_ARCH", "gfx950")
 
-HIP_SOURCES = ["kernels.hip", "mixed.hip", "plan.hip"]
+HIP_SOURCES = ["kernels.hip", "mixed.hip", "plan.hip", "server.hip"]
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -25,6 +25,8 @@
 
 #include <cstring>
 #include <string>
+#include <unordered_map>
+#include <utility>
 #include <vector>
 
 #include "ampcor_hip.h"
@@ -33,6 +35,7 @@ namespace {
 
 const char *const kWorkerCapsule = "ampcor::hip::sequential::capsule_t";
 const char *const kRasterCapsule = "ampcor::hip::raster::capsule_t";
+const char *const kServerCapsule = "ampcor::hip::server::capsule_t";
 
 PyObject *g_error = nullptr;
 
@@ -401,6 +404,164 @@ PyObject *stage_times(PyObject *, PyObject *args)
     return d;
 }
 
+
+// ---- the serving loop ------------------------------------------------------------
+
+struct Server {
+    ampcor_hip_server *server;
+    // every job holds its two rasters alive until it is waited for
+    std::unordered_map<uint64_t, std::pair<PyObject *, PyObject *>> held;
+};
+
+void free_server(PyObject *capsule)
+{
+    if (!PyCapsule_IsValid(capsule, kServerCapsule)) return;
+    auto *s = static_cast<Server *>(PyCapsule_GetPointer(capsule, kServerCapsule));
+    Py_BEGIN_ALLOW_THREADS
+    ampcor_hip_server_destroy(s->server);
+    Py_END_ALLOW_THREADS
+    for (auto &kv : s->held) {
+        Py_XDECREF(kv.second.first);
+        Py_XDECREF(kv.second.second);
+    }
+    delete s;
+}
+
+Server *get_server(PyObject *o)
+{
+    if (!PyCapsule_IsValid(o, kServerCapsule)) {
+        PyErr_SetString(PyExc_TypeError, "invalid server capsule");
+        return nullptr;
+    }
+    return static_cast<Server *>(PyCapsule_GetPointer(o, kServerCapsule));
+}
+
+PyObject *server(PyObject *, PyObject *args, PyObject *kwds)
+{
+    int device = 0, slots = 2, graphs = 1;
+    double budget_gb = 64.0;
+    unsigned long batch = 4096;
+    static const char *names[] = {"device", "budget_gb", "batch", "slots", "graphs", nullptr};
+    if (!PyArg_ParseTupleAndKeywords(args, kwds, "|idkip:server", const_cast<char **>(names), &device,
+                                     &budget_gb, &batch, &slots, &graphs))
+        return nullptr;
+    ampcor_hip_server *sv = nullptr;
+    int rc = ampcor_hip_server_create(device, (size_t)(budget_gb * (1ull << 30)), batch, slots, graphs, &sv);
+    if (rc) return raise(rc);
+    return PyCapsule_New(new Server{sv, {}}, kServerCapsule, free_server);
+}
+
+PyObject *submit(PyObject *, PyObject *args, PyObject *kwds)
+{
+    PyObject *pyServer, *pyRef, *pyTgt;
+    Py_buffer rb, tb;
+    unsigned long chip, window, rm, rf, zf;
+    const char *precision_name = "fp32";
+    static const char *names[] = {"server", "reference", "target", "refBegin", "tgtBegin", "chip",
+                                  "window", "refineMargin", "refineFactor", "zoomFactor",
+                                  "precision", nullptr};
+    if (!PyArg_ParseTupleAndKeywords(args, kwds, "O!O!O!y*y*kkkkk|s:submit", const_cast<char **>(names),
+                                     &PyCapsule_Type, &pyServer, &PyCapsule_Type, &pyRef,
+                                     &PyCapsule_Type, &pyTgt, &rb, &tb, &chip, &window, &rm, &rf,
+                                     &zf, &precision_name))
+        return nullptr;
+    PyObject *result = nullptr;
+    Server *s = get_server(pyServer);
+    ampcor_hip_raster *ref = s ? get_raster(pyRef) : nullptr;
+    ampcor_hip_raster *tgt = ref ? get_raster(pyTgt) : nullptr;
+    int precision;
+    if (tgt && !parse_precision(precision_name, &precision)) {
+        if (rb.len != tb.len || rb.len % 16 || rb.len == 0) {
+            PyErr_SetString(PyExc_ValueError, "corner buffers must hold equal, nonzero counts of int64 pairs");
+        } else {
+            uint64_t id = 0;
+            int rc = ampcor_hip_server_submit(s->server, ref, tgt, rb.len / 16,
+                                              static_cast<const int64_t *>(rb.buf),
+                                              static_cast<const int64_t *>(tb.buf), chip, window, rm,
+                                              rf, zf, precision, &id);
+            if (rc) raise(rc);
+            else {
+                Py_INCREF(pyRef);
+                Py_INCREF(pyTgt);
+                s->held[id] = {pyRef, pyTgt};
+                result = PyLong_FromUnsignedLongLong(id);
+            }
+        }
+    }
+    PyBuffer_Release(&rb);
+    PyBuffer_Release(&tb);
+    return result;
+}
+
+PyObject *wait(PyObject *, PyObject *args, PyObject *kwds)
+{
+    PyObject *pyServer;
+    unsigned long long id;
+    unsigned long pairs;
+    int timeout_ms = -1;
+    static const char *names[] = {"server", "job", "pairs", "timeout_ms", nullptr};
+    if (!PyArg_ParseTupleAndKeywords(args, kwds, "O!Kk|i:wait", const_cast<char **>(names),
+                                     &PyCapsule_Type, &pyServer, &id, &pairs, &timeout_ms))
+        return nullptr;
+    Server *s = get_server(pyServer);
+    if (!s) return nullptr;
+    PyObject *off = bytes_of(8 * (size_t)pairs), *cp = bytes_of(4 * (size_t)pairs),
+             *rp = bytes_of(4 * (size_t)pairs), *sn = bytes_of(4 * (size_t)pairs);
+    if (!off || !cp || !rp || !sn) {
+        Py_XDECREF(off); Py_XDECREF(cp); Py_XDECREF(rp); Py_XDECREF(sn);
+        return nullptr;
+    }
+    double latency = 0.0;
+    int rc;
+    Py_BEGIN_ALLOW_THREADS
+    rc = ampcor_hip_server_wait(s->server, id, timeout_ms,
+                                reinterpret_cast<float *>(PyBytes_AS_STRING(off)),
+                                reinterpret_cast<float *>(PyBytes_AS_STRING(cp)),
+                                reinterpret_cast<float *>(PyBytes_AS_STRING(rp)),
+                                reinterpret_cast<float *>(PyBytes_AS_STRING(sn)), &latency);
+    Py_END_ALLOW_THREADS
+    if (rc) {
+        Py_DECREF(off); Py_DECREF(cp); Py_DECREF(rp); Py_DECREF(sn);
+        return raise(rc);
+    }
+    auto it = s->held.find(id);
+    if (it != s->held.end()) {
+        Py_XDECREF(it->second.first);
+        Py_XDECREF(it->second.second);
+        s->held.erase(it);
+    }
+    return Py_BuildValue("{sNsNsNsNsd}", "offsets", off, "coarse_peak", cp, "refined_peak", rp,
+                         "snr", sn, "latency_ms", latency);
+}
+
+PyObject *server_stats(PyObject *, PyObject *args)
+{
+    PyObject *pyServer;
+    if (!PyArg_ParseTuple(args, "O!:server_stats", &PyCapsule_Type, &pyServer)) return nullptr;
+    Server *s = get_server(pyServer);
+    if (!s) return nullptr;
+    uint64_t pairs, batches, graphs, lanes, tunes;
+    if (int rc = ampcor_hip_server_stats(s->server, &pairs, &batches, &graphs, &lanes, &tunes)) return raise(rc);
+    return Py_BuildValue("{sKsKsKsKsK}", "pairs", (unsigned long long)pairs, "batches",
+                         (unsigned long long)batches, "graph_launches", (unsigned long long)graphs,
+                         "lanes", (unsigned long long)lanes, "tunes", (unsigned long long)tunes);
+}
+
+PyObject *autotune(PyObject *, PyObject *args)
+{
+    PyObject *pyWorker;
+    int reps = 3;
+    if (!PyArg_ParseTuple(args, "O!|i:autotune", &PyCapsule_Type, &pyWorker, &reps)) return nullptr;
+    Worker *w = get_worker(pyWorker);
+    if (!w) return nullptr;
+    int rc;
+    Py_BEGIN_ALLOW_THREADS
+    rc = ampcor_hip_plan_autotune(w->plan, reps);
+    Py_END_ALLOW_THREADS
+    if (rc) return raise(rc);
+    Py_RETURN_NONE;
+}
+
 PyObject *device_count(PyObject *, PyObject *)
 {
     int n = 0;
@@ -428,6 +589,14 @@ PyMethodDef methods[] = {
      "map an SLC raster file of a given shape onto the device"},
     {"slc_pixelSize", slc_pixelSize, METH_NOARGS, "the size of an SLC pixel"},
     {"device_count", device_count, METH_NOARGS, "number of visible HIP devices"},
+    {"autotune", autotune, METH_VARARGS, "time candidate tilings on the last adjust's data"},
+    {"server", (PyCFunction)(void (*)(void))server, METH_VARARGS | METH_KEYWORDS,
+     "start a continuous-batching server"},
+    {"submit", (PyCFunction)(void (*)(void))submit, METH_VARARGS | METH_KEYWORDS,
+     "submit an offset-tracking job; returns its id"},
+    {"wait", (PyCFunction)(void (*)(void))wait, METH_VARARGS | METH_KEYWORDS,
+     "wait for a job; returns its products and latency"},
+    {"server_stats", server_stats, METH_VARARGS, "pairs, batches, graph launches, lanes, tunes"},
     {nullptr, nullptr, 0, nullptr},
 };
 

@@ -90,7 +90,8 @@ struct MixedShape {
     int wg_ut, wg_vt, tiles_u, tiles_v, nsplit;
     size_t lds_bytes;
 };
-MixedShape choose_mixed_shape(int C, int P);
+// nsplit > 0 asks for at least that many chip-row ranges (tuning)
+MixedShape choose_mixed_shape(int C, int P, int nsplit = 0);
 hipError_t launch_prep_mixed(hipStream_t s, const float *ref_amp, const float *tgt_amp, int pairs,
                              int C, int W, int CS, bool fp16, uint16_t *chip_q, uint16_t *tgt_q,
                              MixedStats *stats);

@@ -293,7 +293,7 @@ __global__ __launch_bounds__(256) void k_normalize_mixed(const float *__restrict
 
 }  // namespace
 
-MixedShape choose_mixed_shape(int C, int P)
+MixedShape choose_mixed_shape(int C, int P, int nsplit)
 {
     MixedShape s{};
     s.nk = (C + 15) / 16 + 1;
@@ -316,6 +316,9 @@ MixedShape choose_mixed_shape(int C, int P)
     }
     // more independent workgroups for small surfaces
     if (tiles <= 3 && C % 2 == 0 && C / 2 >= 32 && s.nsplit == 1) s.nsplit = 2;
+    // a requested split (tuning) that keeps at least 16 chip rows per range
+    if (nsplit > s.nsplit && (nsplit & (nsplit - 1)) == 0 && C % nsplit == 0 && C / nsplit >= 16)
+        s.nsplit = nsplit;
     s.tiles_u = (tiles + s.wg_ut - 1) / s.wg_ut;
     s.tiles_v = (tiles + s.wg_vt - 1) / s.wg_vt;
     s.lds_bytes = lds(s.wg_ut, s.wg_vt, s.nsplit);

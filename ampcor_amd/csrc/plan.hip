@@ -64,6 +64,9 @@ double dirichlet(int R, int zf, int d)
 
 namespace {
 
+constexpr int kTuneMaxSplitCoarse = 4;
+constexpr int kTuneMaxSplitRefined = 16;
+
 int check_device(int device)
 {
     int count = 0;
@@ -310,8 +313,11 @@ int ampcor_hip_plan_create(int device, size_t pairs, size_t ref_lines, size_t re
     A(p->stats.alloc(N));
     A(p->sums.alloc(N * p->P * p->P));
     A(p->sums_r.alloc(N * R * R));
-    const size_t ns_c = std::max(p->coarse_shape.nsplit, p->mixed ? p->mixed_coarse.nsplit : 1);
-    const size_t ns_r = std::max(p->refined_shape.nsplit, p->mixed ? p->mixed_refined.nsplit : 1);
+    // partial buffers leave room for the autotuner's largest candidate splits
+    const size_t ns_c = std::max({p->coarse_shape.nsplit, p->mixed ? p->mixed_coarse.nsplit : 1,
+                                  p->mixed ? kTuneMaxSplitCoarse : 1});
+    const size_t ns_r = std::max({p->refined_shape.nsplit, p->mixed ? p->mixed_refined.nsplit : 1,
+                                  kTuneMaxSplitRefined});
     A(p->partial.alloc(N * ns_c * p->P * p->P));
     A(p->gamma.alloc(N * p->P * p->P));
     A(p->loc.alloc(N));
@@ -437,6 +443,7 @@ namespace ampcor_hip {
 int plan_enqueue(ampcor_hip_plan *plan, int N)
 {
     hipStream_t s = plan->stream;
+    plan->last_n = N;
     if (plan->desc_dirty) {
         HIP_TRY(hipMemcpyAsync(plan->d_desc.ptr, plan->desc.data(), N * sizeof(PairDesc),
                                hipMemcpyHostToDevice, s),
@@ -640,4 +647,108 @@ int ampcor_hip_stage_locations(ampcor_hip_plan *plan, int32_t *coarse, int32_t *
     return copy_out(plan, zoomed, plan->zoom_loc.ptr, bytes);
 }
 
+// Times candidate tilings of the two contractions on the data of the last
+// adjust and keeps the fastest.  The FP32 coarse contraction only varies its
+// register block, which leaves its arithmetic order (and results) unchanged.
+int ampcor_hip_plan_autotune(ampcor_hip_plan *plan, int reps)
+{
+    if (!plan) return fail(AMPCOR_HIP_EINVAL, "null plan");
+    if (!plan->adjusted) return fail(AMPCOR_HIP_ESTATE, "autotune needs a prior adjust");
+    if (reps < 1) reps = 1;
+    DeviceGuard guard(plan->device);
+    hipStream_t s = plan->stream;
+    const int N = plan->last_n > 0 ? plan->last_n : (int)plan->pairs;
+    const bool fp16 = plan->precision == AMPCOR_HIP_FP16;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0), "creating tuning events");
+    HIP_TRY(hipEventCreate(&e1), "creating tuning events");
+    auto timeit = [&](auto launch, float *ms) -> int {
+        HIP_TRY(launch(), "tuning warm-up");
+        HIP_TRY(hipEventRecord(e0, s), "tuning");
+        for (int r = 0; r < reps; ++r) HIP_TRY(launch(), "tuning launch");
+        HIP_TRY(hipEventRecord(e1, s), "tuning");
+        HIP_TRY(hipEventSynchronize(e1), "tuning");
+        HIP_TRY(hipEventElapsedTime(ms, e0, e1), "tuning");
+        return AMPCOR_HIP_OK;
+    };
+    int rc = AMPCOR_HIP_OK;
+    if (!plan->mixed) {
+        float best = 1e30f;
+        CorrelateShape pick = plan->coarse_shape;
+        for (int bu : {4, 8, 16}) {
+            if (plan->C % bu) continue;
+            CorrelateShape sh = choose_correlate_shape(plan->C, plan->P, true, bu, 0);
+            float ms;
+            if ((rc = timeit([&] { return launch_correlate(s, sh, plan->rT.ptr, plan->amp_tgt.ptr, N,
+                                                           plan->C, plan->W, plan->partial.ptr); },
+                             &ms)))
+                break;
+            if (ms < best) best = ms, pick = sh;
+        }
+        plan->coarse_shape = pick;
+        best = 1e30f;
+        pick = plan->refined_shape;
+        for (int bu : {4, 8, 16})
+            for (int ns : {4, 8, 16}) {
+                if (rc || plan->Cr % bu || plan->Cr % ns || plan->Cr / ns < 8) continue;
+                CorrelateShape sh = choose_correlate_shape(plan->Cr, plan->R, false, bu, ns);
+                if (sh.lds_bytes > 150 * 1024) continue;
+                float ms;
+                if ((rc = timeit([&] { return launch_correlate(s, sh, plan->rT_r.ptr,
+                                                               plan->amp_tgt_r.ptr, N, plan->Cr,
+                                                               plan->Wr, plan->partial_r.ptr); },
+                                 &ms)))
+                    break;
+                if (ms < best) best = ms, pick = sh;
+            }
+        plan->refined_shape = pick;
+    } else {
+        struct Stage {
+            MixedShape *shape;
+            int C, W, P, maxsplit;
+            uint16_t *cq, *tq;
+            float *partial;
+        } stages[] = {{&plan->mixed_coarse, plan->C, plan->W, plan->P, kTuneMaxSplitCoarse,
+                       plan->chip_q.ptr, plan->tgt_q.ptr, plan->partial.ptr},
+                      {&plan->mixed_refined, plan->Cr, plan->Wr, plan->R, kTuneMaxSplitRefined,
+                       plan->chip_q_r.ptr, plan->tgt_q_r.ptr, plan->partial_r.ptr}};
+        for (auto &st : stages) {
+            float best = 1e30f;
+            MixedShape pick = *st.shape;
+            for (int ns = 1; ns <= st.maxsplit && !rc; ns *= 2) {
+                MixedShape sh = choose_mixed_shape(st.C, st.P, ns);
+                if (sh.nsplit > st.maxsplit || sh.lds_bytes > 150 * 1024) continue;
+                float ms;
+                if ((rc = timeit([&] { return launch_correlate_mixed(s, sh, st.cq, st.tq, N, st.C,
+                                                                     st.W, fp16, st.partial); },
+                                 &ms)))
+                    break;
+                if (ms < best) best = ms, pick = sh;
+            }
+            *st.shape = pick;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    plan->adjusted = false;  // the tuned stages overwrote intermediate buffers
+    return rc;
+}
+
 }  // extern "C"
+
+namespace ampcor_hip {
+
+size_t plan_bytes_per_pair(int C, int W, int rf, int rm, int precision)
+{
+    const size_t P = W - C + 1, E = C + 2 * rm, Cr = rf * C, Wr = rf * E, R = Wr - Cr + 1;
+    size_t b = sizeof(PairDesc) + 4 * (size_t)(C * C) * 2 + 4 * (size_t)W * W + 16 + 16 * P * P +
+               4 * P * P * kTuneMaxSplitCoarse + 4 * P * P + 3 * sizeof(int2) + 4 * 4 + 8 +
+               4 * Cr * Cr * 2 + 4 * Wr * Wr + 16 + 16 * R * R + 4 * R * R * kTuneMaxSplitRefined +
+               4 * R * R;
+    if (precision != AMPCOR_HIP_FP32)
+        b += 2 * (size_t)C * (C + 48) + 2 * (size_t)W * W + 2 * Cr * (Cr + 48) + 2 * Wr * Wr +
+             2 * sizeof(MixedStats);
+    return b;
+}
+
+}  // namespace ampcor_hip

@@ -96,6 +96,7 @@ struct ampcor_hip_plan {
     float stage_ms[ampcor_hip::kStages] = {};
     const char *stage_name[ampcor_hip::kStages] = {};
     int stages_run = 0;
+    int last_n = 0;  // pairs of the last enqueue
 
     void release()
     {

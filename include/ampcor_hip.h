@@ -133,8 +133,45 @@ int ampcor_hip_stage_locations(ampcor_hip_plan *plan, int32_t *coarse, int32_t *
  * events on the plan's stream; names[i] point to static strings.  Timing is
  * off until enabled, because events serialize the stream. */
 int ampcor_hip_set_timing(ampcor_hip_plan *plan, int enabled);
+/* times candidate tilings of both contractions on the data of the last
+ * adjust (reps launches each) and keeps the fastest; the next adjust uses
+ * them.  No counterpart in the reference, whose launch shapes are fixed
+ * templates (lib/libampcor_cuda/correlators/correlate.cu:69-135). */
+int ampcor_hip_plan_autotune(ampcor_hip_plan *plan, int reps);
 int ampcor_hip_stage_times(ampcor_hip_plan *plan, int capacity, int *count, float *ms,
                            const char **names);
+
+/* -- serving: continuous batching of offset-tracking jobs --------------------
+ *
+ * No counterpart in the reference, which correlates one plan at a time
+ * (pkg/ampcor/correlators/MGA.py:52-98).  A server owns a scheduler thread.
+ * Jobs are admitted under an HBM budget, queued per geometry (chip, window,
+ * refine margin and factor, zoom, precision, pixel type), and their pairs are
+ * batched -- across jobs and rasters -- into launches of up to batch_pairs
+ * pairs on slots_per_lane concurrent plans per geometry.  With use_graphs each
+ * batch replays a HIP graph captured for its size bucket.  The first batch of
+ * a geometry runs the plan autotuner.  Rasters must outlive their jobs.
+ */
+typedef struct ampcor_hip_server ampcor_hip_server;
+
+int ampcor_hip_server_create(int device, size_t hbm_budget_bytes, size_t batch_pairs,
+                             int slots_per_lane, int use_graphs, ampcor_hip_server **out);
+/* enqueue a job of `pairs` tile pairs (int64 (line, sample) corners); returns its id */
+int ampcor_hip_server_submit(ampcor_hip_server *server, const ampcor_hip_raster *reference,
+                             const ampcor_hip_raster *target, size_t pairs,
+                             const int64_t *ref_begin, const int64_t *tgt_begin, size_t chip,
+                             size_t window, size_t refine_margin, size_t refine_factor,
+                             size_t zoom_factor, int precision, uint64_t *job);
+/* wait up to timeout_ms (< 0: forever) for a job; copies its products (any
+ * pointer may be NULL) and its submit-to-finish latency, then forgets it.
+ * AMPCOR_HIP_ESTATE if it is still running at the timeout. */
+int ampcor_hip_server_wait(ampcor_hip_server *server, uint64_t job, int timeout_ms,
+                           float *offsets, float *coarse_peak, float *refined_peak, float *snr,
+                           double *latency_ms);
+int ampcor_hip_server_stats(ampcor_hip_server *server, uint64_t *pairs, uint64_t *batches,
+                            uint64_t *graph_launches, uint64_t *lanes, uint64_t *tunes);
+/* stops the scheduler after the batches in flight finish */
+int ampcor_hip_server_destroy(ampcor_hip_server *server);
 
 #ifdef __cplusplus
 }

@@ -79,3 +79,24 @@ def test_raster_buffer_size_is_checked():
     lib = ext.require()
     with pytest.raises(ValueError, match="bytes"):
         lib.raster(np.zeros((4, 4), np.complex64), 4, 5)
+
+
+def test_server_needs_a_device_and_valid_arguments():
+    lib = ext.require()
+    with pytest.raises(ValueError):
+        lib.server(batch=0)
+    if lib.device_count() == 0:
+        with pytest.raises(lib.Error, match="no HIP device"):
+            lib.server()
+
+
+def test_serving_jobs_stay_inside_their_rasters():
+    from ampcor_amd import serving
+    pool = [(np.zeros((300, 260), np.complex64), None), (np.zeros((512, 512), np.complex64), None)]
+    for j in serving.make_jobs(pool, count=40, rate=50.0, seed=1, min_pairs=8, max_pairs=64):
+        lines, samples = pool[j.raster][0].shape
+        assert np.all(j.tgt_begin >= 0)
+        assert np.all(j.tgt_begin[:, 0] + j.window <= lines)
+        assert np.all(j.tgt_begin[:, 1] + j.window <= samples)
+        assert np.all(j.ref_begin - j.tgt_begin == j.margin)
+        assert 8 <= j.pairs <= 64

@@ -11,7 +11,10 @@ sub-pixel offsets, and the copy of the offsets to the host.
 
 Prints one JSON line.  The roofline figure is for the dominant kernel, timed
 with HIP events on the plan's stream during the timed steps; the CPU baseline
-is the FP64 oracle on a bounded sample of the same pairs.
+is the FP64 oracle on a bounded sample of the same pairs.  The "serving"
+object reports the continuous-batching server on a Poisson stream of 256 jobs
+of mixed geometries (ampcor_amd/serving.py): p50/p99 job latency and pairs/s
+at a fixed arrival rate, for each form.
 """
 
 import argparse
@@ -42,6 +45,10 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target duration of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--serve-jobs", type=int, default=256,
+                    help="jobs in the serving stream (0 skips it)")
+    ap.add_argument("--serve-rate", type=float, default=100.0,
+                    help="Poisson arrival rate of the serving stream, jobs/s")
     return ap.parse_args()
 
 
@@ -150,6 +157,19 @@ def main():
         accuracy["pairs_compared"] = int(len(idx))
         print(f"[bench] accuracy vs FP64 oracle: {accuracy}", file=sys.stderr, flush=True)
 
+    serving = None
+    if a.serve_jobs > 0:
+        from ampcor_amd import serving as serve
+        log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)
+        t = time.perf_counter()
+        pool = serve.raster_pool()
+        log(f"serving raster pool ready in {time.perf_counter() - t:.1f} s")
+        serving = {}
+        for f in forms:
+            serving[f] = serve.serve_benchmark(precision=f, count=a.serve_jobs, rate=a.serve_rate,
+                                               pool=pool, log=log)
+            log(f"serving {f}: {serving[f]}")
+
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "tile pairs/sec", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms_per_step, 4),
@@ -164,6 +184,7 @@ def main():
                    "speedup_vs_fp32": round(value / fp32_value, 3), "accuracy": accuracy},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
 

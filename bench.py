@@ -137,11 +137,13 @@ def main():
     flops = {"correlate": 2.0 * C * C * P * P * N, "correlate_refined": 2.0 * Cr * Cr * R * R * N}
     dom = max(flops, key=lambda k: stages[k])
     achieved = flops[dom] / (stages[dom] * 1e-3) / 1e12
+    # HBM bytes per launch of that kernel, from the committed rocprofv3 --pmc
+    # FETCH_SIZE / WRITE_SIZE passes (copy-calibrated) on this same workload
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"{dom}_traffic.json")
+    tfile = os.path.join(ROOT, "profiles", "traffic_scene8192x4096.json")
     if os.path.exists(tfile):
         with open(tfile) as fh:
-            traffic = json.load(fh).get("bytes_per_launch")
+            traffic = json.load(fh).get("bytes_per_launch", {}).get(a.precision, {}).get(dom)
     peak = FP32_PEAK_TFLOPS if a.precision == "fp32" else MIXED_PEAK_TFLOPS
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
                 "unit": "TFLOP/s", "frac": round(achieved / peak, 4),

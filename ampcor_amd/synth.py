@@ -189,3 +189,23 @@ def field_scene(seed=2, lines=8192, samples=4096, grid=(120, 60), chip=64, margi
     centers = s.ref_begin + chip // 2
     s.truth = np.stack([dl[centers[:, 0]], ds[centers[:, 1]]], axis=1)
     return s
+
+
+def amplitude_scene(seed=4, size=8192, grid=(64, 32), chip=128, margin=64, refine_factor=2,
+                    refine_margin=8, zoom_factor=16, max_shift=12.0, sigma=0.5, noise=0.1, taps=8):
+    """Real amplitude rasters: lognormal texture (sigma) smoothed by a 5 x 5
+    Gaussian; the secondary is resampled by a random smooth separable field of
+    up to max_shift px and multiplied by (1 + noise N(0, 1))."""
+    rng = np.random.default_rng(seed)
+    ref = lognormal_texture((size, size), sigma, rng)
+    a, b = rng.uniform(0.5, 1.0, size=2) * max_shift * rng.choice([-1.0, 1.0], size=2)
+    x = np.arange(size) / (size - 1)
+    dl = a * np.sin(0.5 * np.pi * x)          # smooth, monotone, up to |a| px
+    ds = b * x ** 2                           # smooth, up to |b| px
+    tgt = resample_axis(resample_axis(ref, dl, axis=0, taps=taps), ds, axis=1, taps=taps)
+    tgt = (tgt * (1.0 + noise * rng.standard_normal(tgt.shape, dtype=np.float32))).astype(np.float32)
+    s = Scene("amplitude", ref, tgt, chip, margin, refine_factor, refine_margin, zoom_factor, grid)
+    s.plan()
+    centers = s.ref_begin + chip // 2
+    s.truth = np.stack([dl[centers[:, 0]], ds[centers[:, 1]]], axis=1)
+    return s

@@ -32,6 +32,13 @@ METRIC = "tile pairs/sec @64²/128² zoom8, mixed vs FP32 at ≤0.01px; p99 job 
 FP32_PEAK_TFLOPS = 157.3   # MI355X dense FP32 (matrix) peak
 MIXED_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 / FP16 matrix peak
 HBM_PEAK_GBS = 8000.0
+WORKLOADS = {
+    "scene": "scene 8192x4096 complex64 speckle, smooth-field secondary +15 dB, 120x60 grid "
+             "(6844 valid pairs), chip 64, window 128, refine x2 margin 8, zoom 8",
+    "large": "8192x8192 float32 lognormal(0.5) texture * 5x5 Gaussian, smooth-field secondary "
+             "(<= 12 px) x (1 + 0.1 N), 64x32 grid (1984 valid pairs), chip 128, window 256, "
+             "refine x2 margin 8, zoom 16",
+}
 
 
 def parse():
@@ -45,6 +52,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target duration of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--workload", choices=("scene", "large"), default="scene",
+                    help="scene: 8192x4096 complex64, chip 64 / window 128, zoom 8 (default); "
+                         "large: 8192x8192 float32 amplitude, chip 128 / window 256, zoom 16")
     ap.add_argument("--serve-jobs", type=int, default=256,
                     help="jobs in the serving stream (0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=100.0,
@@ -92,12 +102,18 @@ def main():
         raise RuntimeError("no HIP device visible")
 
     t0 = time.perf_counter()
-    scene = synth.field_scene()
+    if a.workload == "large":
+        scene = synth.amplitude_scene()
+        if a.precision == "bf16":
+            a.precision = "fp16"
+    else:
+        scene = synth.field_scene()
     print(f"[bench] scene ready in {time.perf_counter() - t0:.1f} s: {scene.pairs} pairs",
           file=sys.stderr, flush=True)
     N, C, W = scene.pairs, scene.chip, scene.window
-    ref = lib.raster(scene.reference, *scene.reference.shape)
-    tgt = lib.raster(scene.target, *scene.target.shape)
+    dtype = "complex64" if scene.reference.dtype == np.complex64 else "float32"
+    ref = lib.raster(scene.reference, *scene.reference.shape, dtype=dtype)
+    tgt = lib.raster(scene.target, *scene.target.shape, dtype=dtype)
 
     def timed(precision):
         worker = lib.sequential(N, (C, C), (W, W), scene.refine_margin, scene.refine_factor,
@@ -141,7 +157,7 @@ def main():
     # FETCH_SIZE / WRITE_SIZE passes (copy-calibrated) on this same workload
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic_scene8192x4096.json")
-    if os.path.exists(tfile):
+    if os.path.exists(tfile) and a.workload == "scene":
         with open(tfile) as fh:
             traffic = json.load(fh).get("bytes_per_launch", {}).get(a.precision, {}).get(dom)
     peak = FP32_PEAK_TFLOPS if a.precision == "fp32" else MIXED_PEAK_TFLOPS
@@ -160,7 +176,7 @@ def main():
         print(f"[bench] accuracy vs FP64 oracle: {accuracy}", file=sys.stderr, flush=True)
 
     serving = None
-    if a.serve_jobs > 0:
+    if a.serve_jobs > 0 and a.workload == "scene":
         from ampcor_amd import serving as serve
         log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)
         t = time.perf_counter()
@@ -177,9 +193,7 @@ def main():
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms_per_step, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32" if a.precision == "fp32" else a.precision, "data": "synthetic",
-        "config": {"workload": "scene 8192x4096 complex64 speckle, smooth-field secondary +15 dB, "
-                               "120x60 grid (6844 valid pairs), chip 64, window 128, refine x2 "
-                               "margin 8, zoom 8",
+        "config": {"workload": WORKLOADS[a.workload],
                    "pairs": N, "chip": C, "window": W, "refine_factor": scene.refine_factor,
                    "refine_margin": scene.refine_margin, "zoom_factor": scene.zoom_factor,
                    "precision": a.precision, "fp32_pairs_per_sec": round(fp32_value, 1),

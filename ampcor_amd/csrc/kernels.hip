@@ -220,13 +220,19 @@ __global__ __launch_bounds__(kWave) void k_window_sums(const float *__restrict__
 // ---------------------------------------------------------------------------
 // the contraction: num(u, v) = sum_j sum_i r'(i, j) t(u + i, v + j)
 //
-// One workgroup per (pair, placement tile, chip-column range).  The target
-// rows/columns the tile needs are staged in LDS; the zero-mean chip is read
-// with wave-uniform addresses (scalar loads).  Each lane owns BU consecutive
-// placement rows at one placement column and walks the chip column by column;
-// down a column it slides a BU-deep register window over the target rows, so
-// every LDS read feeds BU fused multiply-adds.
+// One workgroup per (pair, placement tile, chip-column range), one task per
+// thread: a lane owns BU consecutive placement rows at one placement column
+// and walks the chip column by column (j outer, i inner); down a column it
+// slides a BU-deep register window over the target rows, so every LDS read
+// feeds BU fused multiply-adds.  The zero-mean chip is read with wave-uniform
+// addresses (scalar loads).  The target streams through LDS in blocks of
+// kJB chip columns: a block needs every target row of the tile but only
+// vcount + kJB - 1 target columns, so the LDS footprint does not grow with
+// the chip width.  The accumulation order (j ascending, i ascending) is the
+// same whatever the tiling.
 // ---------------------------------------------------------------------------
+constexpr int kJB = 16;
+
 template <int BU>
 __global__ __launch_bounds__(1024) void k_correlate(const float *__restrict__ rT,
                                                     const float *__restrict__ amp, int C, int W,
@@ -250,27 +256,32 @@ __global__ __launch_bounds__(1024) void k_correlate(const float *__restrict__ rT
     const int jlen = C / nsplit;
     const int j0 = part * jlen;
     const int rows = nub * BU + C - 1;
-    const int cols = vcount + jlen - 1;
+    const int cols = vcount + kJB - 1;
 
-    // stage target rows [u0, u0 + rows) x columns [v0 + j0, v0 + j0 + cols)
+    const int task = threadIdx.x;
+    const bool active = task < vcount * nub;
+    const int v = active ? task % vcount : 0;
+    const int ubk = active ? task / vcount : 0;
     const float *src = amp + (size_t)pair * W * W;
-    for (int k = threadIdx.x; k < rows * cols; k += blockDim.x) {
-        const int r = k / cols, c = k - r * cols;
-        const int gr = u0 + r;
-        tile[k] = (gr < W) ? src[(size_t)gr * W + v0 + j0 + c] : 0.0f;
-    }
-    __syncthreads();
-
     const float *rp = rT + (size_t)pair * C * C;
-    const int tasks = vcount * nub;
-    for (int task = threadIdx.x; task < tasks; task += blockDim.x) {
-        const int v = task % vcount;
-        const int ubk = task / vcount;
-        float acc[BU];
+    float acc[BU];
 #pragma unroll
-        for (int a = 0; a < BU; ++a) acc[a] = 0.0f;
-        for (int jj = 0; jj < jlen; ++jj) {
-            const float *rc = rp + (size_t)(j0 + jj) * C;
+    for (int a = 0; a < BU; ++a) acc[a] = 0.0f;
+
+    for (int jb = 0; jb < jlen; jb += kJB) {
+        const int jn = min(kJB, jlen - jb);
+        // stage target rows [u0, u0 + rows) x columns [v0 + j0 + jb, + cols)
+        __syncthreads();
+        const int cbase = v0 + j0 + jb;
+        for (int k = threadIdx.x; k < rows * cols; k += blockDim.x) {
+            const int r = k / cols, c = k - r * cols;
+            const int gr = u0 + r, gc = cbase + c;
+            tile[k] = (gr < W && gc < W) ? src[(size_t)gr * W + gc] : 0.0f;
+        }
+        __syncthreads();
+        if (!active) continue;
+        for (int jj = 0; jj < jn; ++jj) {
+            const float *rc = rp + (size_t)(j0 + jb + jj) * C;
             const float *tc = tile + (size_t)(ubk * BU) * cols + v + jj;
             float w[BU];
 #pragma unroll
@@ -285,12 +296,13 @@ __global__ __launch_bounds__(1024) void k_correlate(const float *__restrict__ rT
                 }
             }
         }
-        float *dst = partial + ((size_t)pair * nsplit + part) * P * P;
+    }
+    if (!active) return;
+    float *dst = partial + ((size_t)pair * nsplit + part) * P * P;
 #pragma unroll
-        for (int a = 0; a < BU; ++a) {
-            const int uu = ubk * BU + a;
-            if (uu < ucount) dst[(size_t)(u0 + uu) * P + v0 + v] = acc[a];
-        }
+    for (int a = 0; a < BU; ++a) {
+        const int uu = ubk * BU + a;
+        if (uu < ucount) dst[(size_t)(u0 + uu) * P + v0 + v] = acc[a];
     }
 }
 
@@ -913,10 +925,10 @@ __global__ __launch_bounds__(kZoomThreads) void k_zoom(
 // launchers
 // ---------------------------------------------------------------------------
 
-static size_t correlate_lds(int C, int bu, int ucount, int vcount, int nsplit)
+static size_t correlate_lds(int C, int bu, int ucount, int vcount)
 {
     const int nub = (ucount + bu - 1) / bu;
-    return (size_t)(nub * bu + C - 1) * (vcount + C / nsplit - 1) * sizeof(float);
+    return (size_t)(nub * bu + C - 1) * (vcount + kJB - 1) * sizeof(float);
 }
 
 CorrelateShape choose_correlate_shape(int C, int P, bool exact, int bu, int nsplit)
@@ -928,7 +940,7 @@ CorrelateShape choose_correlate_shape(int C, int P, bool exact, int bu, int nspl
         s.bu = 4;  // measured best for the split refined contraction
     else
         s.bu = (C % 8 == 0) ? 8 : (C % 4 == 0) ? 4 : (C % 2 == 0) ? 2 : 1;
-    const size_t budget = 96 * 1024;
+    const size_t budget = 64 * 1024;
     // chip-column split: the exact form keeps the whole contraction in a lane
     s.nsplit = 1;
     if (!exact) {
@@ -942,19 +954,20 @@ CorrelateShape choose_correlate_shape(int C, int P, bool exact, int bu, int nspl
                 }
         }
     }
-    int extent = P;
-    const int candidates[] = {P, 64, 48, 32, 24, 16, 8};
+    // the largest square tile whose tasks (one per thread) and LDS fit
+    int extent = 1;
+    const int candidates[] = {P, 96, 80, 64, 48, 32, 24, 16, 8, 4, 2, 1};
     for (int e : candidates) {
         if (e > P) continue;
         extent = e;
-        if (correlate_lds(C, s.bu, e, e, s.nsplit) <= budget) break;
+        const int tasks = e * ((e + s.bu - 1) / s.bu);
+        if (tasks <= 1024 && correlate_lds(C, s.bu, e, e) <= budget) break;
     }
     s.ub = s.vb = extent;
     s.tiles_u = s.tiles_v = (P + extent - 1) / extent;
-    s.lds_bytes = correlate_lds(C, s.bu, extent, extent, s.nsplit);
+    s.lds_bytes = correlate_lds(C, s.bu, extent, extent);
     const int tasks = extent * ((extent + s.bu - 1) / s.bu);
-    int threads = ((tasks + kWave - 1) / kWave) * kWave;
-    s.threads = threads > 1024 ? 1024 : (threads < kWave ? kWave : threads);
+    s.threads = ((tasks + kWave - 1) / kWave) * kWave;
     return s;
 }
 

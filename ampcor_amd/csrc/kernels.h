@@ -38,6 +38,7 @@ struct CorrelateShape {
     int ub, vb;  // placement tile extent
     int tiles_u, tiles_v;
     int nsplit;  // chip-column ranges (partial numerators summed by the normalizer)
+    int jblock;  // chip columns staged per LDS pass
     int threads;
     size_t lds_bytes;
 };

@@ -226,18 +226,19 @@ __global__ __launch_bounds__(kWave) void k_window_sums(const float *__restrict__
 // slides a BU-deep register window over the target rows, so every LDS read
 // feeds BU fused multiply-adds.  The zero-mean chip is read with wave-uniform
 // addresses (scalar loads).  The target streams through LDS in blocks of
-// kJB chip columns: a block needs every target row of the tile but only
-// vcount + kJB - 1 target columns, so the LDS footprint does not grow with
-// the chip width.  The accumulation order (j ascending, i ascending) is the
-// same whatever the tiling.
+// `jblock` chip columns: a block needs every target row of the tile but only
+// vcount + jblock - 1 target columns, so the LDS footprint need not grow with
+// the chip width (one block covers the whole range when it fits).  The
+// accumulation order (j ascending, i ascending) is the same whatever the
+// tiling.
 // ---------------------------------------------------------------------------
-constexpr int kJB = 16;
+constexpr int kJB = 16;  // column block when the whole chip range does not fit
 
 template <int BU>
 __global__ __launch_bounds__(1024) void k_correlate(const float *__restrict__ rT,
                                                     const float *__restrict__ amp, int C, int W,
                                                     int P, int ub_extent, int vb_extent,
-                                                    int tiles_u, int tiles_v, int nsplit,
+                                                    int tiles_u, int tiles_v, int nsplit, int jblock,
                                                     float *__restrict__ partial)
 {
     extern __shared__ float tile[];
@@ -256,7 +257,7 @@ __global__ __launch_bounds__(1024) void k_correlate(const float *__restrict__ rT
     const int jlen = C / nsplit;
     const int j0 = part * jlen;
     const int rows = nub * BU + C - 1;
-    const int cols = vcount + kJB - 1;
+    const int cols = vcount + jblock - 1;
 
     const int task = threadIdx.x;
     const bool active = task < vcount * nub;
@@ -268,8 +269,8 @@ __global__ __launch_bounds__(1024) void k_correlate(const float *__restrict__ rT
 #pragma unroll
     for (int a = 0; a < BU; ++a) acc[a] = 0.0f;
 
-    for (int jb = 0; jb < jlen; jb += kJB) {
-        const int jn = min(kJB, jlen - jb);
+    for (int jb = 0; jb < jlen; jb += jblock) {
+        const int jn = min(jblock, jlen - jb);
         // stage target rows [u0, u0 + rows) x columns [v0 + j0 + jb, + cols)
         __syncthreads();
         const int cbase = v0 + j0 + jb;
@@ -925,10 +926,10 @@ __global__ __launch_bounds__(kZoomThreads) void k_zoom(
 // launchers
 // ---------------------------------------------------------------------------
 
-static size_t correlate_lds(int C, int bu, int ucount, int vcount)
+static size_t correlate_lds(int C, int bu, int ucount, int vcount, int jblock)
 {
     const int nub = (ucount + bu - 1) / bu;
-    return (size_t)(nub * bu + C - 1) * (vcount + kJB - 1) * sizeof(float);
+    return (size_t)(nub * bu + C - 1) * (vcount + jblock - 1) * sizeof(float);
 }
 
 CorrelateShape choose_correlate_shape(int C, int P, bool exact, int bu, int nsplit)
@@ -961,11 +962,14 @@ CorrelateShape choose_correlate_shape(int C, int P, bool exact, int bu, int nspl
         if (e > P) continue;
         extent = e;
         const int tasks = e * ((e + s.bu - 1) / s.bu);
-        if (tasks <= 1024 && correlate_lds(C, s.bu, e, e) <= budget) break;
+        if (tasks <= 1024 && correlate_lds(C, s.bu, e, e, kJB) <= budget) break;
     }
     s.ub = s.vb = extent;
     s.tiles_u = s.tiles_v = (P + extent - 1) / extent;
-    s.lds_bytes = correlate_lds(C, s.bu, extent, extent);
+    // stage the whole chip-column range at once when it fits
+    const int jlen = C / s.nsplit;
+    s.jblock = correlate_lds(C, s.bu, extent, extent, jlen) <= 80 * 1024 ? jlen : kJB;
+    s.lds_bytes = correlate_lds(C, s.bu, extent, extent, s.jblock);
     const int tasks = extent * ((extent + s.bu - 1) / s.bu);
     s.threads = ((tasks + kWave - 1) / kWave) * kWave;
     return s;
@@ -1018,7 +1022,7 @@ hipError_t launch_correlate(hipStream_t s, const CorrelateShape &sh, const float
     const dim3 grid((unsigned)pairs * sh.tiles_u * sh.tiles_v * sh.nsplit), block(sh.threads);
 #define AMPCOR_CORRELATE(BUV)                                                                    \
     hipLaunchKernelGGL(k_correlate<BUV>, grid, block, sh.lds_bytes, s, rT, amp, C, W, P, sh.ub,  \
-                       sh.vb, sh.tiles_u, sh.tiles_v, sh.nsplit, partial)
+                       sh.vb, sh.tiles_u, sh.tiles_v, sh.nsplit, sh.jblock, partial)
     switch (sh.bu) {
     case 16: AMPCOR_CORRELATE(16); break;
     case 8: AMPCOR_CORRELATE(8); break;

@@ -657,11 +657,14 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2(
     }
 
     const int tiles = Np / 16;
-    // stage 1: T_1 = H x
-    for (int tile = wave; tile < tiles * tiles; tile += kUp2Waves) {
-        const int mt = tile / tiles, nt = tile - mt * tiles;
+    // stage 1: T_1 = H x; a wave takes two output tiles of one tile row at a
+    // time, four independent accumulation chains
+    for (int pr = wave; pr < tiles * ((tiles + 1) / 2); pr += kUp2Waves) {
+        const int mt = pr / ((tiles + 1) / 2);
+        const int nt0 = 2 * (pr - mt * ((tiles + 1) / 2));
+        const bool two = nt0 + 1 < tiles;
         const int m = 16 * mt + lm;
-        floatx4 accR = {0.0f, 0.0f, 0.0f, 0.0f}, accI = {0.0f, 0.0f, 0.0f, 0.0f};
+        floatx4 aR0 = {0.0f, 0.0f, 0.0f, 0.0f}, aI0 = aR0, aR1 = aR0, aI1 = aR0;
         for (int k16 = 0; k16 < Np; k16 += 16)
 #pragma unroll
         for (int k0 = k16; k0 < k16 + 16; k0 += 4) {
@@ -669,14 +672,24 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2(
             int dd = m - k;
             dd += (dd < 0) ? N : 0;
             const float a = (k < N && m < N) ? gs[dd] : 0.0f;
-            accR = __builtin_amdgcn_mfma_f32_16x16x4f32(a, xr[k * ls + 16 * nt + lm], accR, 0, 0, 0);
-            if (DTYPE == 0)
-                accI = __builtin_amdgcn_mfma_f32_16x16x4f32(a, xi[k * ls + 16 * nt + lm], accI, 0, 0, 0);
+            const float *xrow = xr + k * ls + 16 * nt0 + lm;
+            const float *irow = xi + k * ls + 16 * nt0 + lm;
+            aR0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, xrow[0], aR0, 0, 0, 0);
+            if (DTYPE == 0) aI0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, irow[0], aI0, 0, 0, 0);
+            if (two) {
+                aR1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, xrow[16], aR1, 0, 0, 0);
+                if (DTYPE == 0) aI1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, irow[16], aI1, 0, 0, 0);
+            }
         }
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            tr[(16 * mt + 4 * lk + i) * ls + 16 * nt + lm] = accR[i];
-            ti[(16 * mt + 4 * lk + i) * ls + 16 * nt + lm] = (DTYPE == 0) ? accI[i] : 0.0f;
+            const int row = (16 * mt + 4 * lk + i) * ls + 16 * nt0 + lm;
+            tr[row] = aR0[i];
+            ti[row] = (DTYPE == 0) ? aI0[i] : 0.0f;
+            if (two) {
+                tr[row + 16] = aR1[i];
+                ti[row + 16] = (DTYPE == 0) ? aI1[i] : 0.0f;
+            }
         }
     }
     __syncthreads();
@@ -684,15 +697,12 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2(
         const int a = k / N, b = k - a * N;
         emit(2 * a + 1, 2 * b, amplitude(tr[a * ls + b], ti[a * ls + b]));
     }
-    // stage 2: odd columns, y[2a + s][2b + 1] = (T_s H^T)[a][b]
-    for (int tile = wave; tile < 2 * tiles * tiles; tile += kUp2Waves) {
-        const int sphase = tile / (tiles * tiles);
-        const int rem = tile - sphase * tiles * tiles;
-        const int mt = rem / tiles, nt = rem - mt * tiles;
-        const float *Ar = sphase ? tr : xr;
-        const float *Ai = sphase ? ti : xi;
+    // stage 2: odd columns, y[2a + s][2b + 1] = (T_s H^T)[a][b]; a wave takes
+    // the same output tile of both phases s at a time (four chains, shared B)
+    for (int tile = wave; tile < tiles * tiles; tile += kUp2Waves) {
+        const int mt = tile / tiles, nt = tile - mt * tiles;
         const int n = 16 * nt + lm;
-        floatx4 accR = {0.0f, 0.0f, 0.0f, 0.0f}, accI = {0.0f, 0.0f, 0.0f, 0.0f};
+        floatx4 xR = {0.0f, 0.0f, 0.0f, 0.0f}, xI = xR, tR = xR, tI = xR;
         for (int k16 = 0; k16 < Np; k16 += 16)
 #pragma unroll
         for (int k0 = k16; k0 < k16 + 16; k0 += 4) {
@@ -700,13 +710,19 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2(
             int dd = n - k;
             dd += (dd < 0) ? N : 0;
             const float b = (k < N && n < N) ? gs[dd] : 0.0f;
-            accR = __builtin_amdgcn_mfma_f32_16x16x4f32(Ar[(16 * mt + lm) * ls + k], b, accR, 0, 0, 0);
-            accI = __builtin_amdgcn_mfma_f32_16x16x4f32(Ai[(16 * mt + lm) * ls + k], b, accI, 0, 0, 0);
+            const int ai = (16 * mt + lm) * ls + k;
+            xR = __builtin_amdgcn_mfma_f32_16x16x4f32(xr[ai], b, xR, 0, 0, 0);
+            xI = __builtin_amdgcn_mfma_f32_16x16x4f32(xi[ai], b, xI, 0, 0, 0);
+            tR = __builtin_amdgcn_mfma_f32_16x16x4f32(tr[ai], b, tR, 0, 0, 0);
+            tI = __builtin_amdgcn_mfma_f32_16x16x4f32(ti[ai], b, tI, 0, 0, 0);
         }
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int a = 16 * mt + 4 * lk + i;
-            if (a < N && n < N) emit(2 * a + sphase, 2 * n + 1, amplitude(accR[i], accI[i]));
+            if (a < N && n < N) {
+                emit(2 * a, 2 * n + 1, amplitude(xR[i], xI[i]));
+                emit(2 * a + 1, 2 * n + 1, amplitude(tR[i], tI[i]));
+            }
         }
     }
     if (QMODE != kUpFP32) {

@@ -218,9 +218,11 @@ __global__ __launch_bounds__(512) void k_correlate_mfma(
     if (wave >= wg_ut || 16 * ut >= P) return;
     const int vcount = min(nvt, (P - 16 * vt0 + 15) / 16);
 
-    floatx4 acc[kMaxVT];
+    // two accumulators per tile (even and odd K-steps): twice the independent
+    // MFMA chains; summed at the end
+    floatx4 acc[kMaxVT], acc2[kMaxVT];
 #pragma unroll
-    for (int v = 0; v < kMaxVT; ++v) acc[v] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int v = 0; v < kMaxVT; ++v) acc[v] = acc2[v] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
 
     // Toeplitz fragment geometry: element index j = 4 D - beta + e,
     // D = 4 kappa + lk + 4 - alpha, lm = 4 alpha + beta
@@ -249,7 +251,10 @@ __global__ __launch_bounds__(512) void k_correlate_mfma(
         for (int kap = 0; kap < NK; ++kap)
 #pragma unroll
             for (int v = 0; v < kMaxVT; ++v)
-                if (v < vcount) acc[v] = mfma16<FP16>(A[v + kap], B[kap], acc[v]);
+                if (v < vcount) {
+                    if (kap & 1) acc2[v] = mfma16<FP16>(A[v + kap], B[kap], acc2[v]);
+                    else acc[v] = mfma16<FP16>(A[v + kap], B[kap], acc[v]);
+                }
     }
     float *dst = partial + ((size_t)pair * nsplit + part) * P * P;
 #pragma unroll
@@ -259,7 +264,7 @@ __global__ __launch_bounds__(512) void k_correlate_mfma(
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const int uu = 16 * ut + 4 * lk + e;
-            if (uu < P && vv < P) dst[(size_t)uu * P + vv] = acc[v][e];
+            if (uu < P && vv < P) dst[(size_t)uu * P + vv] = acc[v][e] + acc2[v][e];
         }
     }
 }
@@ -314,8 +319,11 @@ MixedShape choose_mixed_shape(int C, int P, int nsplit)
         else if (s.wg_vt > 1) s.wg_vt = (s.wg_vt + 1) / 2;
         else break;
     }
-    // more independent workgroups for small surfaces
-    if (tiles <= 3 && C % 2 == 0 && C / 2 >= 32 && s.nsplit == 1) s.nsplit = 2;
+    // more independent, smaller workgroups for small surfaces: at most ~40 KiB
+    // of LDS each, so that several share a CU
+    while (tiles <= 3 && C % (2 * s.nsplit) == 0 && C / (2 * s.nsplit) >= 32 &&
+           lds(s.wg_ut, s.wg_vt, s.nsplit) > 40 * 1024)
+        s.nsplit *= 2;
     // a requested split (tuning) that keeps at least 16 chip rows per range
     if (nsplit > s.nsplit && (nsplit & (nsplit - 1)) == 0 && C % nsplit == 0 && C / nsplit >= 16)
         s.nsplit = nsplit;

@@ -101,6 +101,8 @@ int parse_precision(const char *name, int *precision)
     if (!strcmp(name, "fp32")) *precision = AMPCOR_HIP_FP32;
     else if (!strcmp(name, "bf16")) *precision = AMPCOR_HIP_BF16;
     else if (!strcmp(name, "fp16")) *precision = AMPCOR_HIP_FP16;
+    else if (!strcmp(name, "bf16x3")) *precision = AMPCOR_HIP_BF16X3;
+    else if (!strcmp(name, "fp16x3")) *precision = AMPCOR_HIP_FP16X3;
     else {
         PyErr_Format(PyExc_ValueError, "unknown precision '%s'", name);
         return -1;

@@ -72,10 +72,12 @@ hipError_t launch_coarse_peak(hipStream_t s, const float *gamma, int pairs, int 
 // columns on the left), role 1 the window q[pair][M][M]; and its share of
 // the pair's MixedStats.  amp may be null for role 0 then.
 bool upsample_fuses_quantization(int N, int rf);
+// qlo (optional): second quantized plane holding the rounding remainder of q
+// (the split hi/lo operands), same layout as q
 hipError_t launch_upsample(hipStream_t s, const PairDesc *desc, int pairs, int dtype, int role,
                            const int2 *window, int N, int rf, const float *U, const float *g,
                            float *amp, int qmode = kUpFP32, uint16_t *q = nullptr, int cs = 0,
-                           MixedStats *stats = nullptr);
+                           MixedStats *stats = nullptr, uint16_t *qlo = nullptr);
 // mixed coarse stage in one pass when the tiles fit in LDS: gather, detect,
 // write the FP32 window (for the window sums) and both quantized operands
 bool gather_prep_fits(int C, int W);
@@ -88,17 +90,22 @@ hipError_t launch_gather_prep_mixed(hipStream_t s, const PairDesc *desc, int pai
 // target columns per tile, workgroups of wg_ut waves x wg_vt v-tiles
 struct MixedShape {
     int nk, cs;
+    bool split;
     int wg_ut, wg_vt, tiles_u, tiles_v, nsplit;
     size_t lds_bytes;
 };
 // nsplit > 0 asks for at least that many chip-row ranges (tuning)
-MixedShape choose_mixed_shape(int C, int P, int nsplit = 0);
+// split: hi/lo operands (three products per pair of elements), twice the LDS
+MixedShape choose_mixed_shape(int C, int P, int nsplit = 0, bool split = false);
+// chip_lo / tgt_lo (optional): the rounding remainders, for the split form
 hipError_t launch_prep_mixed(hipStream_t s, const float *ref_amp, const float *tgt_amp, int pairs,
                              int C, int W, int CS, bool fp16, uint16_t *chip_q, uint16_t *tgt_q,
-                             MixedStats *stats);
+                             MixedStats *stats, uint16_t *chip_lo = nullptr,
+                             uint16_t *tgt_lo = nullptr);
 hipError_t launch_correlate_mixed(hipStream_t s, const MixedShape &shape, const uint16_t *chip_q,
                                   const uint16_t *tgt_q, int pairs, int C, int W, bool fp16,
-                                  float *partial);
+                                  float *partial, const uint16_t *chip_lo = nullptr,
+                                  const uint16_t *tgt_lo = nullptr);
 hipError_t launch_normalize_mixed(hipStream_t s, const float *partial, int nsplit,
                                   const double2 *sums, const MixedStats *stats, int pairs, int C,
                                   int W, float *gamma);

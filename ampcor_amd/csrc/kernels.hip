@@ -558,8 +558,8 @@ __device__ inline double quant_scale(int qmode, float maxabs)
 template <int DTYPE, int QMODE>
 __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2(
     const PairDesc *__restrict__ desc, int role, const int2 *__restrict__ window, int N, int Np,
-    const float *__restrict__ g, float *__restrict__ amp, uint16_t *__restrict__ qout, int cs,
-    MixedStats *__restrict__ stats)
+    const float *__restrict__ g, float *__restrict__ amp, uint16_t *__restrict__ qout,
+    uint16_t *__restrict__ qlo, int cs, MixedStats *__restrict__ stats)
 {
     extern __shared__ float lds[];
     __shared__ double red[kUp2Waves * kWave];
@@ -630,6 +630,7 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2(
                 const int p = k / (cs - M), c = k - p * (cs - M);
                 const int col = c < 16 ? c : M + c;
                 qout[((size_t)pair * M + p) * cs + col] = 0;
+                if (qlo) qlo[((size_t)pair * M + p) * cs + col] = 0;
             }
     }
     __syncthreads();
@@ -640,14 +641,22 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2(
     auto emit = [&](int p, int q, float v) {
         if (QMODE == kUpFP32 || role == 1) out[(size_t)p * M + q] = v;
         if (QMODE != kUpFP32) {
-            const uint16_t h = quantize(QMODE, (v - qc) * qs);
+            const float x = (v - qc) * qs;
+            const uint16_t h = quantize(QMODE, x);
+            // split form: the rounding remainder as a second operand
+            const uint16_t l = qlo ? quantize(QMODE, x - dequantize(QMODE, h)) : 0;
             if (role == 0) {
-                qout[((size_t)pair * M + p) * cs + 16 + q] = h;
-                const double back = (double)dequantize(QMODE, h) / qs;
+                const size_t at = ((size_t)pair * M + p) * cs + 16 + q;
+                qout[at] = h;
+                if (qlo) qlo[at] = l;
+                const double back =
+                    ((double)dequantize(QMODE, h) + (qlo ? (double)dequantize(QMODE, l) : 0.0)) / qs;
                 s1 += back;
                 s2 += back * back;
             } else {
-                qout[((size_t)pair * M + p) * M + q] = h;
+                const size_t at = ((size_t)pair * M + p) * M + q;
+                qout[at] = h;
+                if (qlo) qlo[at] = l;
             }
         }
     };
@@ -1075,7 +1084,8 @@ bool upsample_fuses_quantization(int N, int rf) { return rf == 2 && (N + 15) / 1
 
 hipError_t launch_upsample(hipStream_t s, const PairDesc *desc, int pairs, int dtype, int role,
                            const int2 *window, int N, int rf, const float *U, const float *g,
-                           float *amp, int qmode, uint16_t *q, int cs, MixedStats *stats)
+                           float *amp, int qmode, uint16_t *q, int cs, MixedStats *stats,
+                           uint16_t *qlo)
 {
     const int Np = (N + 15) / 16 * 16;
     if (rf == 2 && Np <= kUp2MaxN && g) {
@@ -1083,7 +1093,7 @@ hipError_t launch_upsample(hipStream_t s, const PairDesc *desc, int pairs, int d
         const dim3 grid(pairs), block(kUp2Waves * kWave);
 #define AMPCOR_UP2(DT, QM)                                                                        \
     hipLaunchKernelGGL((k_upsample2<DT, QM>), grid, block, lds, s, desc, role, window, N, Np, g, \
-                       amp, q, cs, stats)
+                       amp, q, qlo, cs, stats)
         if (dtype == 0) {
             if (qmode == kUpBF16) AMPCOR_UP2(0, kUpBF16);
             else if (qmode == kUpFP16) AMPCOR_UP2(0, kUpFP16);

@@ -92,7 +92,8 @@ __device__ inline double unit_scale(float maxabs)
 template <bool FP16>
 __global__ __launch_bounds__(kPrepThreads) void k_prep_mixed(
     const float *__restrict__ ref_amp, const float *__restrict__ tgt_amp, int C, int W, int CS,
-    uint16_t *__restrict__ chip_q, uint16_t *__restrict__ tgt_q, MixedStats *__restrict__ stats)
+    uint16_t *__restrict__ chip_q, uint16_t *__restrict__ tgt_q, MixedStats *__restrict__ stats,
+    uint16_t *__restrict__ chip_lo, uint16_t *__restrict__ tgt_lo)
 {
     __shared__ double pd[kPrepThreads];
     __shared__ float pf[kPrepThreads];
@@ -112,15 +113,22 @@ __global__ __launch_bounds__(kPrepThreads) void k_prep_mixed(
     double s1 = 0.0, s2 = 0.0;
     for (int k = t; k < C * CS; k += kPrepThreads) {
         const int i = k / CS, j = k - i * CS - 16;
-        uint16_t q = 0;
+        uint16_t q = 0, ql = 0;
         if (j >= 0 && j < C) {
             const float x = (float)((double)(r[(size_t)i * C + j] - mean_r) * a);
             q = FP16 ? to_f16(x) : to_bf16(x);
-            const double v = (double)(FP16 ? from_f16(q) : from_bf16(q)) / a;
+            const float hq = FP16 ? from_f16(q) : from_bf16(q);
+            double v = (double)hq;
+            if (chip_lo) {
+                ql = FP16 ? to_f16(x - hq) : to_bf16(x - hq);
+                v += (double)(FP16 ? from_f16(ql) : from_bf16(ql));
+            }
+            v /= a;
             s1 += v;
             s2 += v * v;
         }
         cq[k] = q;
+        if (chip_lo) chip_lo[(size_t)pair * C * CS + k] = ql;
     }
     const double sumr = block_sum(pd, s1);
     const double sumr2 = block_sum(pd, s2);
@@ -133,7 +141,12 @@ __global__ __launch_bounds__(kPrepThreads) void k_prep_mixed(
     const double b = unit_scale(block_max(pf, mx));
     for (int k = t; k < W * W; k += kPrepThreads) {
         const float x = (float)((double)(g[k] - mean_t) * b);
-        tq[k] = FP16 ? to_f16(x) : to_bf16(x);
+        const uint16_t q = FP16 ? to_f16(x) : to_bf16(x);
+        tq[k] = q;
+        if (tgt_lo) {
+            const float hq = FP16 ? from_f16(q) : from_bf16(q);
+            tgt_lo[(size_t)pair * W * W + k] = FP16 ? to_f16(x - hq) : to_bf16(x - hq);
+        }
     }
     if (t == 0) {
         MixedStats st;
@@ -165,11 +178,12 @@ __device__ inline floatx4 mfma16(uint2 a, uint2 b, floatx4 c)
 
 constexpr int kMaxVT = 5;
 
-template <int NK, bool FP16>
+template <int NK, bool FP16, bool SPLIT>
 __global__ __launch_bounds__(512) void k_correlate_mfma(
     const uint16_t *__restrict__ chip_q, const uint16_t *__restrict__ tgt_q, int C, int W, int P,
     int CS, int wg_ut, int wg_vt, int tiles_u, int tiles_v, int nsplit,
-    float *__restrict__ partial)
+    float *__restrict__ partial, const uint16_t *__restrict__ chip_lo,
+    const uint16_t *__restrict__ tgt_lo)
 {
     extern __shared__ uint2 lds_u2[];
     uint16_t *lds = reinterpret_cast<uint16_t *>(lds_u2);
@@ -189,28 +203,34 @@ __global__ __launch_bounds__(512) void k_correlate_mfma(
     const int nrows = 16 * wg_ut + ilen - 1;
     const int ncols = 16 * (wg_vt + NK - 1);
     const int TS = ncols + 8;  // row stride (elements), keeps 8-byte alignment
-    uint16_t *tile = lds;
-    uint16_t *chip = lds + (size_t)nrows * TS;
+    constexpr int planes = SPLIT ? 2 : 1;
+    uint16_t *tile = lds;                                      // [plane][nrows][TS]
+    uint16_t *chip = lds + (size_t)planes * nrows * TS;        // [plane][ilen][CS]
 
     // stage the target block (zero outside the window) and the chip rows
-    const uint16_t *src = tgt_q + (size_t)pair * W * W;
     const int r0 = 16 * ut0 + i0, c0 = 16 * vt0;
-    for (int k = threadIdx.x; k < nrows * (ncols / 4); k += blockDim.x) {
-        const int rr = k / (ncols / 4), cc = 4 * (k - rr * (ncols / 4));
-        const int gr = r0 + rr, gc = c0 + cc;
-        uint2 v = make_uint2(0u, 0u);
-        if (gr < W) {
-            uint16_t e[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) e[q] = (gc + q < W) ? src[(size_t)gr * W + gc + q] : 0;
-            v.x = e[0] | ((uint32_t)e[1] << 16);
-            v.y = e[2] | ((uint32_t)e[3] << 16);
+    for (int pl = 0; pl < planes; ++pl) {
+        const uint16_t *src = (pl ? tgt_lo : tgt_q) + (size_t)pair * W * W;
+        uint16_t *dstt = tile + (size_t)pl * nrows * TS;
+        for (int k = threadIdx.x; k < nrows * (ncols / 4); k += blockDim.x) {
+            const int rr = k / (ncols / 4), cc = 4 * (k - rr * (ncols / 4));
+            const int gr = r0 + rr, gc = c0 + cc;
+            uint2 v = make_uint2(0u, 0u);
+            if (gr < W) {
+                uint16_t e[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) e[q] = (gc + q < W) ? src[(size_t)gr * W + gc + q] : 0;
+                v.x = e[0] | ((uint32_t)e[1] << 16);
+                v.y = e[2] | ((uint32_t)e[3] << 16);
+            }
+            *reinterpret_cast<uint2 *>(dstt + (size_t)rr * TS + cc) = v;
         }
-        *reinterpret_cast<uint2 *>(tile + (size_t)rr * TS + cc) = v;
+        const uint16_t *csrc = (pl ? chip_lo : chip_q) + ((size_t)pair * C + i0) * CS;
+        uint16_t *dstc = chip + (size_t)pl * ilen * CS;
+        for (int k = threadIdx.x; k < ilen * CS / 4; k += blockDim.x)
+            reinterpret_cast<uint2 *>(dstc)[k] = reinterpret_cast<const uint2 *>(csrc)[k];
     }
-    const uint16_t *csrc = chip_q + ((size_t)pair * C + i0) * CS;
-    for (int k = threadIdx.x; k < ilen * CS / 4; k += blockDim.x)
-        reinterpret_cast<uint2 *>(chip)[k] = reinterpret_cast<const uint2 *>(csrc)[k];
     __syncthreads();
 
     const int ut = ut0 + wave;
@@ -228,9 +248,7 @@ __global__ __launch_bounds__(512) void k_correlate_mfma(
     // D = 4 kappa + lk + 4 - alpha, lm = 4 alpha + beta
     const int alpha = lm >> 2, beta = lm & 3;
     const bool hi_pair = beta >= 2, odd = beta & 1;
-    for (int ii = 0; ii < ilen; ++ii) {
-        const uint16_t *crow = chip + (size_t)ii * CS;
-        uint2 B[NK];
+    auto build = [&](const uint16_t *crow, uint2 *B) {
 #pragma unroll
         for (int kap = 0; kap < NK; ++kap) {
             const int D = 4 * kap + lk + 4 - alpha;
@@ -242,18 +260,31 @@ __global__ __launch_bounds__(512) void k_correlate_mfma(
             B[kap].x = odd ? __builtin_amdgcn_alignbyte(s1, s0, 2) : s1;
             B[kap].y = odd ? __builtin_amdgcn_alignbyte(s2, s1, 2) : s2;
         }
-        const uint16_t *arow = tile + (size_t)(16 * wave + lm + ii) * TS + 4 * lk;
-        uint2 A[kMaxVT + NK - 1];
+    };
+    for (int ii = 0; ii < ilen; ++ii) {
+        uint2 B[NK], BL[SPLIT ? NK : 1];
+        build(chip + (size_t)ii * CS, B);
+        if constexpr (SPLIT) build(chip + (size_t)(ilen + ii) * CS, BL);
+        const size_t arow_off = (size_t)(16 * wave + lm + ii) * TS + 4 * lk;
+        uint2 A[kMaxVT + NK - 1], AL[SPLIT ? kMaxVT + NK - 1 : 1];
 #pragma unroll
         for (int xi = 0; xi < kMaxVT + NK - 1; ++xi)
-            if (xi < vcount + NK - 1) A[xi] = *reinterpret_cast<const uint2 *>(arow + 16 * xi);
+            if (xi < vcount + NK - 1) {
+                A[xi] = *reinterpret_cast<const uint2 *>(tile + arow_off + 16 * xi);
+                if constexpr (SPLIT)
+                    AL[xi] = *reinterpret_cast<const uint2 *>(tile + (size_t)nrows * TS + arow_off + 16 * xi);
+            }
 #pragma unroll
         for (int kap = 0; kap < NK; ++kap)
 #pragma unroll
             for (int v = 0; v < kMaxVT; ++v)
                 if (v < vcount) {
-                    if (kap & 1) acc2[v] = mfma16<FP16>(A[v + kap], B[kap], acc2[v]);
-                    else acc[v] = mfma16<FP16>(A[v + kap], B[kap], acc[v]);
+                    floatx4 &c = (kap & 1) ? acc2[v] : acc[v];
+                    c = mfma16<FP16>(A[v + kap], B[kap], c);
+                    if constexpr (SPLIT) {
+                        c = mfma16<FP16>(A[v + kap], BL[kap], c);
+                        c = mfma16<FP16>(AL[v + kap], B[kap], c);
+                    }
                 }
     }
     float *dst = partial + ((size_t)pair * nsplit + part) * P * P;
@@ -298,11 +329,12 @@ __global__ __launch_bounds__(256) void k_normalize_mixed(const float *__restrict
 
 }  // namespace
 
-MixedShape choose_mixed_shape(int C, int P, int nsplit)
+MixedShape choose_mixed_shape(int C, int P, int nsplit, bool split)
 {
     MixedShape s{};
     s.nk = (C + 15) / 16 + 1;
     s.cs = 16 * (s.nk + 1);
+    s.split = split;
     const int tiles = (P + 15) / 16;
     s.wg_vt = tiles < kMaxVT ? tiles : kMaxVT;
     s.wg_ut = tiles < 8 ? tiles : 8;
@@ -311,7 +343,7 @@ MixedShape choose_mixed_shape(int C, int P, int nsplit)
     auto lds = [&](int ut, int vt, int ns) {
         const int ilen = C / ns;
         const size_t rows = 16 * ut + ilen - 1, cols = 16 * (vt + s.nk - 1) + 8;
-        return (rows * cols + (size_t)ilen * s.cs) * sizeof(uint16_t);
+        return (rows * cols + (size_t)ilen * s.cs) * sizeof(uint16_t) * (split ? 2 : 1);
     };
     while (lds(s.wg_ut, s.wg_vt, s.nsplit) > budget) {
         if (C % (2 * s.nsplit) == 0 && C / (2 * s.nsplit) >= 16) s.nsplit *= 2;
@@ -335,33 +367,37 @@ MixedShape choose_mixed_shape(int C, int P, int nsplit)
 
 hipError_t launch_prep_mixed(hipStream_t s, const float *ref_amp, const float *tgt_amp, int pairs,
                              int C, int W, int CS, bool fp16, uint16_t *chip_q, uint16_t *tgt_q,
-                             MixedStats *stats)
+                             MixedStats *stats, uint16_t *chip_lo, uint16_t *tgt_lo)
 {
     if (fp16)
         hipLaunchKernelGGL(k_prep_mixed<true>, dim3(pairs), dim3(kPrepThreads), 0, s, ref_amp,
-                           tgt_amp, C, W, CS, chip_q, tgt_q, stats);
+                           tgt_amp, C, W, CS, chip_q, tgt_q, stats, chip_lo, tgt_lo);
     else
         hipLaunchKernelGGL(k_prep_mixed<false>, dim3(pairs), dim3(kPrepThreads), 0, s, ref_amp,
-                           tgt_amp, C, W, CS, chip_q, tgt_q, stats);
+                           tgt_amp, C, W, CS, chip_q, tgt_q, stats, chip_lo, tgt_lo);
     return hipGetLastError();
 }
 
 hipError_t launch_correlate_mixed(hipStream_t s, const MixedShape &sh, const uint16_t *chip_q,
                                   const uint16_t *tgt_q, int pairs, int C, int W, bool fp16,
-                                  float *partial)
+                                  float *partial, const uint16_t *chip_lo, const uint16_t *tgt_lo)
 {
     const int P = W - C + 1;
+    if (sh.split && (!chip_lo || !tgt_lo)) return hipErrorInvalidValue;
     const dim3 grid((unsigned)pairs * sh.tiles_u * sh.tiles_v * sh.nsplit), block(64 * sh.wg_ut);
+#define AMPCOR_MIXED_ONE(NKV, F16, SPL)                                                             \
+    hipLaunchKernelGGL((k_correlate_mfma<NKV, F16, SPL>), grid, block, sh.lds_bytes, s, chip_q,     \
+                       tgt_q, C, W, P, sh.cs, sh.wg_ut, sh.wg_vt, sh.tiles_u, sh.tiles_v,            \
+                       sh.nsplit, partial, chip_lo, tgt_lo)
 #define AMPCOR_MIXED(NKV)                                                                           \
     do {                                                                                            \
-        if (fp16)                                                                                   \
-            hipLaunchKernelGGL((k_correlate_mfma<NKV, true>), grid, block, sh.lds_bytes, s, chip_q, \
-                               tgt_q, C, W, P, sh.cs, sh.wg_ut, sh.wg_vt, sh.tiles_u, sh.tiles_v,   \
-                               sh.nsplit, partial);                                                 \
-        else                                                                                        \
-            hipLaunchKernelGGL((k_correlate_mfma<NKV, false>), grid, block, sh.lds_bytes, s,        \
-                               chip_q, tgt_q, C, W, P, sh.cs, sh.wg_ut, sh.wg_vt, sh.tiles_u,       \
-                               sh.tiles_v, sh.nsplit, partial);                                     \
+        if (sh.split) {                                                                             \
+            if (fp16) AMPCOR_MIXED_ONE(NKV, true, true);                                            \
+            else AMPCOR_MIXED_ONE(NKV, false, true);                                                \
+        } else {                                                                                    \
+            if (fp16) AMPCOR_MIXED_ONE(NKV, true, false);                                           \
+            else AMPCOR_MIXED_ONE(NKV, false, false);                                               \
+        }                                                                                           \
     } while (0)
     switch (sh.nk) {
     case 2: AMPCOR_MIXED(2); break;
@@ -376,6 +412,7 @@ hipError_t launch_correlate_mixed(hipStream_t s, const MixedShape &sh, const uin
     default: return hipErrorInvalidValue;
     }
 #undef AMPCOR_MIXED
+#undef AMPCOR_MIXED_ONE
     return hipGetLastError();
 }
 

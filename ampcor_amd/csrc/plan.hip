@@ -249,8 +249,8 @@ int ampcor_hip_plan_create(int device, size_t pairs, size_t ref_lines, size_t re
     const size_t R = Wr - Cr + 1;
     if (Wr > 512) return fail(AMPCOR_HIP_EINVAL, "refined window edge must be <= 512");
     if (R > 129) return fail(AMPCOR_HIP_EINVAL, "refined surface too large (2 rf rm + 1 > 129)");
-    if (precision != AMPCOR_HIP_FP32 && precision != AMPCOR_HIP_BF16 && precision != AMPCOR_HIP_FP16)
-        return fail(AMPCOR_HIP_EINVAL, "unknown precision (FP32, BF16 or FP16)");
+    if (precision < AMPCOR_HIP_FP32 || precision > AMPCOR_HIP_FP16X3)
+        return fail(AMPCOR_HIP_EINVAL, "unknown precision (FP32, BF16, FP16, BF16X3 or FP16X3)");
     if (precision != AMPCOR_HIP_FP32 && (C % 16 || C * refine_factor > 256))
         return fail(AMPCOR_HIP_EINVAL, "the mixed forms need a chip edge that is a multiple of 16 "
                                        "and a refined chip edge <= 256");
@@ -285,9 +285,11 @@ int ampcor_hip_plan_create(int device, size_t pairs, size_t ref_lines, size_t re
         p->refined_shape = choose_correlate_shape(p->Cr, p->R, false, rbu, rns);
     }
     p->mixed = precision != AMPCOR_HIP_FP32;
+    p->split = precision == AMPCOR_HIP_BF16X3 || precision == AMPCOR_HIP_FP16X3;
+    p->fp16 = precision == AMPCOR_HIP_FP16 || precision == AMPCOR_HIP_FP16X3;
     if (p->mixed) {
         p->mixed_coarse = choose_mixed_shape(p->C, p->P);
-        p->mixed_refined = choose_mixed_shape(p->Cr, p->R);
+        p->mixed_refined = choose_mixed_shape(p->Cr, p->R, 0, p->split);
     }
 
     DeviceGuard guard(device);
@@ -334,6 +336,10 @@ int ampcor_hip_plan_create(int device, size_t pairs, size_t ref_lines, size_t re
         A(p->tgt_q.alloc(N * W * W));
         A(p->chip_q_r.alloc(N * Cr * p->mixed_refined.cs));
         A(p->tgt_q_r.alloc(N * Wr * Wr));
+        if (p->split) {
+            A(p->chip_lo_r.alloc(N * Cr * p->mixed_refined.cs));
+            A(p->tgt_lo_r.alloc(N * Wr * Wr));
+        }
         A(p->mstats.alloc(N));
         A(p->mstats_r.alloc(N));
     }
@@ -463,7 +469,7 @@ int plan_enqueue(ampcor_hip_plan *plan, int N)
     } while (0)
 
     auto *p = plan;
-    const bool fp16 = p->precision == AMPCOR_HIP_FP16;
+    const bool fp16 = p->fp16;
     const int qmode = fp16 ? kUpFP16 : kUpBF16;
     if (p->mixed && gather_prep_fits(p->C, p->W)) {
         STEP(launch_gather_prep_mixed(s, p->d_desc.ptr, N, p->dtype, p->C, p->W, p->mixed_coarse.cs,
@@ -507,11 +513,11 @@ int plan_enqueue(ampcor_hip_plan *plan, int N)
     if (fused) {
         STEP(launch_upsample(s, p->d_desc.ptr, N, p->dtype, 0, p->window.ptr, p->C, p->rf,
                              p->U_ref.ptr, p->g_ref.ptr, nullptr, qmode, p->chip_q_r.ptr,
-                             p->mixed_refined.cs, p->mstats_r.ptr),
+                             p->mixed_refined.cs, p->mstats_r.ptr, p->chip_lo_r.ptr),
              "upsample_ref");
         STEP(launch_upsample(s, p->d_desc.ptr, N, p->dtype, 1, p->window.ptr, p->E, p->rf,
                              p->U_tgt.ptr, p->g_tgt.ptr, p->amp_tgt_r.ptr, qmode, p->tgt_q_r.ptr, 0,
-                             p->mstats_r.ptr),
+                             p->mstats_r.ptr, p->tgt_lo_r.ptr),
              "upsample_tgt");
     } else {
         STEP(launch_upsample(s, p->d_desc.ptr, N, p->dtype, 0, p->window.ptr, p->C, p->rf,
@@ -523,7 +529,7 @@ int plan_enqueue(ampcor_hip_plan *plan, int N)
         if (p->mixed)
             STEP(launch_prep_mixed(s, p->amp_ref_r.ptr, p->amp_tgt_r.ptr, N, p->Cr, p->Wr,
                                    p->mixed_refined.cs, fp16, p->chip_q_r.ptr, p->tgt_q_r.ptr,
-                                   p->mstats_r.ptr),
+                                   p->mstats_r.ptr, p->chip_lo_r.ptr, p->tgt_lo_r.ptr),
                  "prep_refined");
         else
             STEP(launch_chip_stats(s, p->amp_ref_r.ptr, N, p->Cr, p->rT_r.ptr, p->stats_r.ptr),
@@ -533,7 +539,8 @@ int plan_enqueue(ampcor_hip_plan *plan, int N)
          "window_sums_refined");
     if (p->mixed) {
         STEP(launch_correlate_mixed(s, p->mixed_refined, p->chip_q_r.ptr, p->tgt_q_r.ptr, N, p->Cr,
-                                    p->Wr, fp16, p->partial_r.ptr),
+                                    p->Wr, fp16, p->partial_r.ptr, p->chip_lo_r.ptr,
+                                    p->tgt_lo_r.ptr),
              "correlate_refined");
         STEP(launch_normalize_mixed(s, p->partial_r.ptr, p->mixed_refined.nsplit, p->sums_r.ptr,
                                     p->mstats_r.ptr, N, p->Cr, p->Wr, p->gamma_r.ptr),
@@ -658,7 +665,7 @@ int ampcor_hip_plan_autotune(ampcor_hip_plan *plan, int reps)
     DeviceGuard guard(plan->device);
     hipStream_t s = plan->stream;
     const int N = plan->last_n > 0 ? plan->last_n : (int)plan->pairs;
-    const bool fp16 = plan->precision == AMPCOR_HIP_FP16;
+    const bool fp16 = plan->fp16;
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0), "creating tuning events");
     HIP_TRY(hipEventCreate(&e1), "creating tuning events");
@@ -706,21 +713,24 @@ int ampcor_hip_plan_autotune(ampcor_hip_plan *plan, int reps)
         struct Stage {
             MixedShape *shape;
             int C, W, P, maxsplit;
-            uint16_t *cq, *tq;
+            uint16_t *cq, *tq, *cl, *tl;
             float *partial;
+            bool split;
         } stages[] = {{&plan->mixed_coarse, plan->C, plan->W, plan->P, kTuneMaxSplitCoarse,
-                       plan->chip_q.ptr, plan->tgt_q.ptr, plan->partial.ptr},
+                       plan->chip_q.ptr, plan->tgt_q.ptr, nullptr, nullptr, plan->partial.ptr, false},
                       {&plan->mixed_refined, plan->Cr, plan->Wr, plan->R, kTuneMaxSplitRefined,
-                       plan->chip_q_r.ptr, plan->tgt_q_r.ptr, plan->partial_r.ptr}};
+                       plan->chip_q_r.ptr, plan->tgt_q_r.ptr, plan->chip_lo_r.ptr, plan->tgt_lo_r.ptr,
+                       plan->partial_r.ptr, plan->split}};
         for (auto &st : stages) {
             float best = 1e30f;
             MixedShape pick = *st.shape;
             for (int ns = 1; ns <= st.maxsplit && !rc; ns *= 2) {
-                MixedShape sh = choose_mixed_shape(st.C, st.P, ns);
+                MixedShape sh = choose_mixed_shape(st.C, st.P, ns, st.split);
                 if (sh.nsplit > st.maxsplit || sh.lds_bytes > 150 * 1024) continue;
                 float ms;
                 if ((rc = timeit([&] { return launch_correlate_mixed(s, sh, st.cq, st.tq, N, st.C,
-                                                                     st.W, fp16, st.partial); },
+                                                                     st.W, fp16, st.partial, st.cl,
+                                                                     st.tl); },
                                  &ms)))
                     break;
                 if (ms < best) best = ms, pick = sh;
@@ -748,6 +758,8 @@ size_t plan_bytes_per_pair(int C, int W, int rf, int rm, int precision)
     if (precision != AMPCOR_HIP_FP32)
         b += 2 * (size_t)C * (C + 48) + 2 * (size_t)W * W + 2 * Cr * (Cr + 48) + 2 * Wr * Wr +
              2 * sizeof(MixedStats);
+    if (precision == AMPCOR_HIP_BF16X3 || precision == AMPCOR_HIP_FP16X3)
+        b += 2 * Cr * (Cr + 48) + 2 * Wr * Wr;
     return b;
 }
 

@@ -78,6 +78,8 @@ struct ampcor_hip_plan {
     ampcor_hip::CorrelateShape coarse_shape{}, refined_shape{};
     ampcor_hip::MixedShape mixed_coarse{}, mixed_refined{};
     bool mixed = false;
+    bool split = false;  // hi/lo operands in the refined contraction
+    bool fp16 = false;
 
     ampcor_hip::DeviceBuffer<PairDesc> d_desc;
     ampcor_hip::DeviceBuffer<float> amp_ref, amp_tgt, rT, partial, gamma;
@@ -88,7 +90,7 @@ struct ampcor_hip_plan {
     ampcor_hip::DeviceBuffer<float> amp_ref_r, amp_tgt_r, rT_r, partial_r, gamma_r;
     ampcor_hip::DeviceBuffer<float> U_ref, U_tgt, g_ref, g_tgt;
     ampcor_hip::DeviceBuffer<double> D;
-    ampcor_hip::DeviceBuffer<uint16_t> chip_q, tgt_q, chip_q_r, tgt_q_r;
+    ampcor_hip::DeviceBuffer<uint16_t> chip_q, tgt_q, chip_q_r, tgt_q_r, chip_lo_r, tgt_lo_r;
     ampcor_hip::DeviceBuffer<ampcor_hip::MixedStats> mstats, mstats_r;
 
     bool timing = false;
@@ -110,6 +112,8 @@ struct ampcor_hip_plan {
         tgt_q.release();
         chip_q_r.release();
         tgt_q_r.release();
+        chip_lo_r.release();
+        tgt_lo_r.release();
         mstats.release();
         mstats_r.release();
         D.release();

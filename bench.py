@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--precision", default="bf16",
                     help="form whose throughput is the headline value (fp32, bf16, fp16); "
                          "the FP32 drop-in form is always timed beside it")
+    ap.add_argument("--extra-forms", default="bf16x3",
+                    help="further forms timed beside FP32 and the headline (comma list, or '')")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target duration of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -141,6 +143,9 @@ def main():
         return elapsed / a.steps, stages, off
 
     forms = ["fp32"] if a.precision == "fp32" else ["fp32", a.precision]
+    for f in (x.strip() for x in a.extra_forms.split(",")):
+        if f and f not in forms:
+            forms.append(f)
     runs = {f: timed(f) for f in forms}
     step_s, stages, off = runs[a.precision]
     ms_per_step = 1e3 * step_s
@@ -187,7 +192,7 @@ def main():
         pool = serve.raster_pool()
         log(f"serving raster pool ready in {time.perf_counter() - t:.1f} s")
         serving = {}
-        for f in forms:
+        for f in forms[:2]:
             serving[f] = serve.serve_benchmark(precision=f, count=a.serve_jobs, rate=a.serve_rate,
                                                pool=pool, log=log)
             log(f"serving {f}: {serving[f]}")
@@ -201,6 +206,7 @@ def main():
                    "pairs": N, "chip": C, "window": W, "refine_factor": scene.refine_factor,
                    "refine_margin": scene.refine_margin, "zoom_factor": scene.zoom_factor,
                    "precision": a.precision, "fp32_pairs_per_sec": round(fp32_value, 1),
+                   "forms_pairs_per_sec": {f: round(N / runs[f][0], 1) for f in forms},
                    "speedup_vs_fp32": round(value / fp32_value, 3), "accuracy": accuracy},
         "roofline": roofline,
         "cpu_baseline": cpu,

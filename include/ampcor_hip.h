@@ -51,6 +51,9 @@ extern "C" {
 #define AMPCOR_HIP_FP32 0 /* plain FP32: the drop-in form */
 #define AMPCOR_HIP_BF16 1 /* mixed: BF16 MFMA contraction, FP32 accumulate */
 #define AMPCOR_HIP_FP16 2 /* mixed: FP16 MFMA contraction, FP32 accumulate */
+/* mixed, with split hi/lo operands (three products) in the refined re-correlation */
+#define AMPCOR_HIP_BF16X3 3
+#define AMPCOR_HIP_FP16X3 4
 
 /* raster pixel types */
 #define AMPCOR_HIP_COMPLEX64 0 /* SLC: interleaved float re, im */

@@ -28,7 +28,7 @@ def run(lib, s, precision):
                    s.refine_factor, s.refine_margin, s.zoom_factor, precision=precision)
 
 
-@pytest.mark.parametrize("precision", ["bf16", "fp16"])
+@pytest.mark.parametrize("precision", ["bf16", "fp16", "bf16x3", "fp16x3"])
 def test_mixed_shifted_scene(lib, precision):
     s = synth.shifted_scene()
     hip, exact = run(lib, s, precision), fp64(s)
@@ -40,7 +40,7 @@ def test_mixed_shifted_scene(lib, precision):
     np.testing.assert_allclose(hip["refined_peak"], exact["refined_peak"], rtol=PEAK_REL_TOL)
 
 
-@pytest.mark.parametrize("precision", ["bf16", "fp16"])
+@pytest.mark.parametrize("precision", ["bf16", "fp16", "bf16x3", "fp16x3"])
 def test_mixed_field_scene(lib, precision):
     s = synth.field_scene(seed=6, lines=1024, samples=1024, grid=(12, 12), range_ramp=3.0,
                           azimuth_quadratic=1.5)
@@ -73,3 +73,14 @@ def test_mixed_float32_rasters(lib):
     s.plan()
     hip, exact = run(lib, s, "fp16"), fp64(s)
     assert np.abs(hip["offsets"] - exact["offsets"]).max() <= MIXED_OFFSET_TOL
+
+
+def test_split_refinement_is_closer_to_fp64(lib):
+    """The hi/lo products of the split form win back accuracy in the refined stage."""
+    s = synth.field_scene(seed=7, lines=1024, samples=1024, grid=(12, 12), range_ramp=3.0,
+                          azimuth_quadratic=1.5)
+    exact = fp64(s)
+    plain = np.abs(run(lib, s, "bf16")["offsets"] - exact["offsets"]).max()
+    split = np.abs(run(lib, s, "bf16x3")["offsets"] - exact["offsets"]).max()
+    print(f"bf16 {plain:.2e} px, bf16x3 {split:.2e} px", flush=True)
+    assert split <= plain

@@ -13,6 +13,8 @@ import shutil
 import subprocess
 import sys
 import sysconfig
+import tempfile
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -20,9 +22,10 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 ARCH = os.environ.get("AMPCOR_HIP_ARCH", "gfx950")
 
-HIP_SOURCES = ["kernels.hip", "mixed.hip", "plan.hip", "server.hip"]
+HIP_SOURCES = ["kernels.hip", "mixed.hip", "mixed_nk_a.hip", "mixed_nk_b.hip", "mixed_nk_c.hip",
+               "mixed_nk_d.hip", "plan.hip", "server.hip"]
 HIPCC_FLAGS = [
-    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
@@ -52,11 +55,20 @@ def module_path():
 
 
 def build(verbose=False):
+    """Compiles the HIP sources in parallel (one object each), links the library, then
+    builds the CPython module against it."""
     lib = library_path()
-    cmd = [_hipcc()] + HIPCC_FLAGS + [f"-I{INCLUDE}"] + [os.path.join(CSRC, s) for s in HIP_SOURCES]
-    cmd += ["-o", lib + ".tmp"]
-    out = _run(cmd)
+    hipcc = _hipcc()
+    with tempfile.TemporaryDirectory(prefix="ampcor_hip_build_") as tmp:
+        objs = [os.path.join(tmp, os.path.splitext(s)[0] + ".o") for s in HIP_SOURCES]
+        cmds = [[hipcc] + HIPCC_FLAGS + [f"-I{INCLUDE}", "-c", os.path.join(CSRC, s), "-o", o]
+                for s, o in zip(HIP_SOURCES, objs)]
+        with ThreadPoolExecutor(max_workers=min(len(cmds), os.cpu_count() or 1)) as pool:
+            outs = list(pool.map(_run, cmds))
+        outs.append(_run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC"] + objs
+                         + ["-o", lib + ".tmp"]))
     os.replace(lib + ".tmp", lib)
+    out = "\n".join(o for o in outs if o)
     if verbose and out:
         print(out, file=sys.stderr)
     mod = module_path()

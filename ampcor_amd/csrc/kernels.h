@@ -86,7 +86,7 @@ hipError_t launch_gather_prep_mixed(hipStream_t s, const PairDesc *desc, int pai
                                     uint16_t *chip_q, uint16_t *tgt_q, MixedStats *stats);
 // ---- mixed precision (mixed.hip) ----
 
-// tiling of the MFMA contraction: 16 x 16 placement tiles, nk K-steps of 16
+// tiling of the MFMA contraction: 16 x 16 placement tiles, nk K-steps of 32
 // target columns per tile, workgroups of wg_ut waves x wg_vt v-tiles
 struct MixedShape {
     int nk, cs;

@@ -522,15 +522,21 @@ __global__ __launch_bounds__(kUpWaves * kWave) void k_upsample(
 // a circulant H[a][n] = g((a - n) mod N), so
 //   y[2a][2b] = x,  y[2a+1][2b] = (H x),  y[2a+s][2b+1] = (T_s H^T),
 // with T_0 = x, T_1 = H x: three N^3 products instead of rf^3 N^3 / ... of
-// the dense form.  8 waves share the output tiles.
+// the dense form.  8 waves share the output tiles.  The circulant is read
+// from g laid out periodically in LDS (H[r][c] = gx[r - c + Np]: fixed offsets
+// in the MFMA loops), the tile is gathered with kUp2Batch loads in flight
+// per thread, and all four phases of an output 2 x 2 block are written
+// together at the end (paired stores).
 // ---------------------------------------------------------------------------
 constexpr int kUp2Waves = 8;
 constexpr int kUp2MaxN = 96;
+constexpr int kUp2Batch = 8;
 
+// |z| in FP32 (v_sqrt_f32, within 1 ulp): the refined stage is compared with
+// the FP64 oracle by tolerance
 __device__ inline float amplitude(float re, float im)
 {
-    const double r = re, i = im;
-    return (float)sqrt(r * r + i * i);
+    return __builtin_amdgcn_sqrtf(re * re + im * im);
 }
 
 __device__ inline uint16_t quantize(int qmode, float x)
@@ -555,6 +561,28 @@ __device__ inline double quant_scale(int qmode, float maxabs)
     return ldexp(1.0, -e);
 }
 
+// sum (FP64) and max over the workgroup; every thread gets the results
+__device__ inline void block_sum_max(double *red, float *redm, double &sum, float &mx)
+{
+    for (int d = kWave / 2; d > 0; d >>= 1) {
+        sum += __shfl_xor(sum, d, kWave);
+        mx = fmaxf(mx, __shfl_xor(mx, d, kWave));
+    }
+    const int wave = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        red[wave] = sum;
+        redm[wave] = mx;
+    }
+    __syncthreads();
+    sum = 0.0;
+    mx = 0.0f;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+        sum += red[w];
+        mx = fmaxf(mx, redm[w]);
+    }
+}
+
 template <int DTYPE, int QMODE>
 __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2(
     const PairDesc *__restrict__ desc, int role, const int2 *__restrict__ window, int N, int Np,
@@ -562,12 +590,11 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2(
     uint16_t *__restrict__ qlo, int cs, MixedStats *__restrict__ stats)
 {
     extern __shared__ float lds[];
-    __shared__ double red[kUp2Waves * kWave];
-    __shared__ double red2[kUp2Waves * kWave];
-    __shared__ float qconst[2];
+    __shared__ double red[kUp2Waves], red2[kUp2Waves];
+    __shared__ float redm[kUp2Waves];
     const int ls = Np + 4;
     float *xr = lds, *xi = xr + Np * ls, *tr = xi + Np * ls, *ti = tr + Np * ls;
-    float *gs = ti + Np * ls;
+    float *gx = ti + Np * ls;   // gx[j] = g((j - Np) mod N), j < 2 Np
     const int pair = blockIdx.x;
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x >> 6;
@@ -584,46 +611,53 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2(
     }
     double asum = 0.0;
     float amax = 0.0f;
-    for (int k = threadIdx.x; k < Np * Np; k += blockDim.x) {
-        const int r = k / Np, c = k - r * Np;
-        float re = 0.0f, im = 0.0f;
-        if (r < N && c < N) {
-            const size_t off = (size_t)(line + r) * stride + sample + c;
-            if (DTYPE == 0) {
-                const float2 z = static_cast<const float2 *>(base)[off];
-                re = z.x;
-                im = z.y;
-            } else {
-                re = static_cast<const float *>(base)[off];
-            }
-            if (QMODE != kUpFP32) {
-                const float a = amplitude(re, im);
-                asum += a;
-                amax = fmaxf(amax, a);
+    const int total = Np * Np;
+    for (int k0 = threadIdx.x; k0 < total; k0 += kUp2Batch * blockDim.x) {
+        float re[kUp2Batch], im[kUp2Batch];
+#pragma unroll
+        for (int j = 0; j < kUp2Batch; ++j) {
+            const int k = k0 + j * blockDim.x;
+            const int r = k / Np, c = k - r * Np;
+            re[j] = 0.0f;
+            im[j] = 0.0f;
+            if (k < total && r < N && c < N) {
+                const size_t off = (size_t)(line + r) * stride + sample + c;
+                if (DTYPE == 0) {
+                    const float2 z = static_cast<const float2 *>(base)[off];
+                    re[j] = z.x;
+                    im[j] = z.y;
+                } else {
+                    re[j] = static_cast<const float *>(base)[off];
+                }
             }
         }
-        xr[r * ls + c] = re;
-        xi[r * ls + c] = im;
+#pragma unroll
+        for (int j = 0; j < kUp2Batch; ++j) {
+            const int k = k0 + j * blockDim.x;
+            if (k < total) {
+                const int r = k / Np, c = k - r * Np;
+                xr[r * ls + c] = re[j];
+                xi[r * ls + c] = im[j];
+                if (QMODE != kUpFP32) {
+                    const float a = amplitude(re[j], im[j]);
+                    asum += a;
+                    amax = fmaxf(amax, a);
+                }
+            }
+        }
     }
-    for (int k = threadIdx.x; k < N; k += blockDim.x) gs[k] = g[k];
+    for (int k = threadIdx.x; k < 2 * Np; k += blockDim.x) {
+        const int t = (k - Np) % N;
+        gx[k] = g[t < 0 ? t + N : t];
+    }
     const int M = 2 * N;
+    float qc = 0.0f, qs = 1.0f;
     if (QMODE != kUpFP32) {
         // the quantization constants come from the coarse tile: its mean
         // amplitude is removed, and twice its peak bounds the upsampled values
-        red[threadIdx.x] = asum;
-        red2[threadIdx.x] = amax;
-        __syncthreads();
-        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-            if (threadIdx.x < s) {
-                red[threadIdx.x] += red[threadIdx.x + s];
-                red2[threadIdx.x] = fmax(red2[threadIdx.x], red2[threadIdx.x + s]);
-            }
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) {
-            qconst[0] = (float)(red[0] / ((double)N * N));
-            qconst[1] = (float)quant_scale(QMODE, 2.0f * (float)red2[0]);
-        }
+        block_sum_max(red, redm, asum, amax);
+        qc = (float)(asum / ((double)N * N));
+        qs = (float)quant_scale(QMODE, 2.0f * amax);
         // zero padding of the chip rows
         if (role == 0)
             for (int k = threadIdx.x; k < M * (cs - M); k += blockDim.x) {
@@ -634,36 +668,6 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2(
             }
     }
     __syncthreads();
-    const float qc = (QMODE != kUpFP32) ? qconst[0] : 0.0f;
-    const float qs = (QMODE != kUpFP32) ? qconst[1] : 1.0f;
-    float *out = amp ? amp + (size_t)pair * M * M : nullptr;
-    double s1 = 0.0, s2 = 0.0;
-    auto emit = [&](int p, int q, float v) {
-        if (QMODE == kUpFP32 || role == 1) out[(size_t)p * M + q] = v;
-        if (QMODE != kUpFP32) {
-            const float x = (v - qc) * qs;
-            const uint16_t h = quantize(QMODE, x);
-            // split form: the rounding remainder as a second operand
-            const uint16_t l = qlo ? quantize(QMODE, x - dequantize(QMODE, h)) : 0;
-            if (role == 0) {
-                const size_t at = ((size_t)pair * M + p) * cs + 16 + q;
-                qout[at] = h;
-                if (qlo) qlo[at] = l;
-                const double back =
-                    ((double)dequantize(QMODE, h) + (qlo ? (double)dequantize(QMODE, l) : 0.0)) / qs;
-                s1 += back;
-                s2 += back * back;
-            } else {
-                const size_t at = ((size_t)pair * M + p) * M + q;
-                qout[at] = h;
-                if (qlo) qlo[at] = l;
-            }
-        }
-    };
-    for (int k = threadIdx.x; k < N * N; k += blockDim.x) {
-        const int a = k / N, b = k - a * N;
-        emit(2 * a, 2 * b, amplitude(xr[a * ls + b], xi[a * ls + b]));
-    }
 
     const int tiles = Np / 16;
     // stage 1: T_1 = H x; a wave takes two output tiles of one tile row at a
@@ -672,24 +676,26 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2(
         const int mt = pr / ((tiles + 1) / 2);
         const int nt0 = 2 * (pr - mt * ((tiles + 1) / 2));
         const bool two = nt0 + 1 < tiles;
-        const int m = 16 * mt + lm;
+        // H[m][k] for rows m >= N or columns k >= N multiplies zeros or
+        // lands in outputs that are not written
+        const float *hrow = gx + (16 * mt + lm - lk + Np);
+        const float *xrow = xr + lk * ls + 16 * nt0 + lm;
+        const float *irow = xi + lk * ls + 16 * nt0 + lm;
         floatx4 aR0 = {0.0f, 0.0f, 0.0f, 0.0f}, aI0 = aR0, aR1 = aR0, aI1 = aR0;
         for (int k16 = 0; k16 < Np; k16 += 16)
 #pragma unroll
-        for (int k0 = k16; k0 < k16 + 16; k0 += 4) {
-            const int k = k0 + lk;
-            int dd = m - k;
-            dd += (dd < 0) ? N : 0;
-            const float a = (k < N && m < N) ? gs[dd] : 0.0f;
-            const float *xrow = xr + k * ls + 16 * nt0 + lm;
-            const float *irow = xi + k * ls + 16 * nt0 + lm;
-            aR0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, xrow[0], aR0, 0, 0, 0);
-            if (DTYPE == 0) aI0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, irow[0], aI0, 0, 0, 0);
-            if (two) {
-                aR1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, xrow[16], aR1, 0, 0, 0);
-                if (DTYPE == 0) aI1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, irow[16], aI1, 0, 0, 0);
+            for (int k0 = k16; k0 < k16 + 16; k0 += 4) {
+                const float a = hrow[-k0];
+                aR0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, xrow[k0 * ls], aR0, 0, 0, 0);
+                if (DTYPE == 0)
+                    aI0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, irow[k0 * ls], aI0, 0, 0, 0);
+                if (two) {
+                    aR1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, xrow[k0 * ls + 16], aR1, 0, 0, 0);
+                    if (DTYPE == 0)
+                        aI1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, irow[k0 * ls + 16], aI1, 0,
+                                                                   0, 0);
+                }
             }
-        }
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int row = (16 * mt + 4 * lk + i) * ls + 16 * nt0 + lm;
@@ -702,54 +708,73 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2(
         }
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < N * N; k += blockDim.x) {
-        const int a = k / N, b = k - a * N;
-        emit(2 * a + 1, 2 * b, amplitude(tr[a * ls + b], ti[a * ls + b]));
-    }
+
+    float *out = amp ? amp + (size_t)pair * M * M : nullptr;
+    double s1 = 0.0, s2 = 0.0;
+    // y[p][q], y[p][q + 1] (q even)
+    auto emit2 = [&](int p, int q, float v0, float v1) {
+        if (QMODE == kUpFP32 || role == 1)
+            *reinterpret_cast<float2 *>(out + (size_t)p * M + q) = make_float2(v0, v1);
+        if (QMODE != kUpFP32) {
+            const float x0 = (v0 - qc) * qs, x1 = (v1 - qc) * qs;
+            const uint16_t h0 = quantize(QMODE, x0), h1 = quantize(QMODE, x1);
+            // split form: the rounding remainders as a second operand
+            uint16_t l0 = 0, l1 = 0;
+            if (qlo) {
+                l0 = quantize(QMODE, x0 - dequantize(QMODE, h0));
+                l1 = quantize(QMODE, x1 - dequantize(QMODE, h1));
+            }
+            const size_t at = role == 0 ? ((size_t)pair * M + p) * cs + 16 + q
+                                        : ((size_t)pair * M + p) * M + q;
+            *reinterpret_cast<uint32_t *>(qout + at) = (uint32_t)h0 | ((uint32_t)h1 << 16);
+            if (qlo) *reinterpret_cast<uint32_t *>(qlo + at) = (uint32_t)l0 | ((uint32_t)l1 << 16);
+            if (role == 0) {
+                const double b0 = ((double)dequantize(QMODE, h0) +
+                                   (qlo ? (double)dequantize(QMODE, l0) : 0.0)) / qs;
+                const double b1 = ((double)dequantize(QMODE, h1) +
+                                   (qlo ? (double)dequantize(QMODE, l1) : 0.0)) / qs;
+                s1 += b0 + b1;
+                s2 += b0 * b0 + b1 * b1;
+            }
+        }
+    };
     // stage 2: odd columns, y[2a + s][2b + 1] = (T_s H^T)[a][b]; a wave takes
-    // the same output tile of both phases s at a time (four chains, shared B)
+    // the same output tile of both phases s at a time (four chains, shared B),
+    // and writes the 2 x 2 output blocks of its tile
     for (int tile = wave; tile < tiles * tiles; tile += kUp2Waves) {
         const int mt = tile / tiles, nt = tile - mt * tiles;
         const int n = 16 * nt + lm;
+        const float *hrow = gx + (n - lk + Np);
+        const int arow = (16 * mt + lm) * ls + lk;
         floatx4 xR = {0.0f, 0.0f, 0.0f, 0.0f}, xI = xR, tR = xR, tI = xR;
         for (int k16 = 0; k16 < Np; k16 += 16)
 #pragma unroll
-        for (int k0 = k16; k0 < k16 + 16; k0 += 4) {
-            const int k = k0 + lk;
-            int dd = n - k;
-            dd += (dd < 0) ? N : 0;
-            const float b = (k < N && n < N) ? gs[dd] : 0.0f;
-            const int ai = (16 * mt + lm) * ls + k;
-            xR = __builtin_amdgcn_mfma_f32_16x16x4f32(xr[ai], b, xR, 0, 0, 0);
-            xI = __builtin_amdgcn_mfma_f32_16x16x4f32(xi[ai], b, xI, 0, 0, 0);
-            tR = __builtin_amdgcn_mfma_f32_16x16x4f32(tr[ai], b, tR, 0, 0, 0);
-            tI = __builtin_amdgcn_mfma_f32_16x16x4f32(ti[ai], b, tI, 0, 0, 0);
-        }
+            for (int k0 = k16; k0 < k16 + 16; k0 += 4) {
+                const float b = hrow[-k0];
+                const int ai = arow + k0;
+                xR = __builtin_amdgcn_mfma_f32_16x16x4f32(xr[ai], b, xR, 0, 0, 0);
+                xI = __builtin_amdgcn_mfma_f32_16x16x4f32(xi[ai], b, xI, 0, 0, 0);
+                tR = __builtin_amdgcn_mfma_f32_16x16x4f32(tr[ai], b, tR, 0, 0, 0);
+                tI = __builtin_amdgcn_mfma_f32_16x16x4f32(ti[ai], b, tI, 0, 0, 0);
+            }
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int a = 16 * mt + 4 * lk + i;
             if (a < N && n < N) {
-                emit(2 * a, 2 * n + 1, amplitude(xR[i], xI[i]));
-                emit(2 * a + 1, 2 * n + 1, amplitude(tR[i], tI[i]));
+                const int at = a * ls + n;
+                emit2(2 * a, 2 * n, amplitude(xr[at], xi[at]), amplitude(xR[i], xI[i]));
+                emit2(2 * a + 1, 2 * n, amplitude(tr[at], ti[at]), amplitude(tR[i], tI[i]));
             }
         }
     }
     if (QMODE != kUpFP32) {
         if (role == 0) {
-            __syncthreads();
-            red[threadIdx.x] = s1;
-            red2[threadIdx.x] = s2;
-            __syncthreads();
-            for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-                if (threadIdx.x < s) {
-                    red[threadIdx.x] += red[threadIdx.x + s];
-                    red2[threadIdx.x] += red2[threadIdx.x + s];
-                }
-                __syncthreads();
-            }
+            float unused = 0.0f;
+            block_sum_max(red, redm, s1, unused);
+            block_sum_max(red2, redm, s2, unused);
             if (threadIdx.x == 0) {
-                stats[pair].sumr = red[0];
-                stats[pair].sumr2 = red2[0];
+                stats[pair].sumr = s1;
+                stats[pair].sumr2 = s2;
                 stats[pair].scale_r = qs;
             }
         } else if (threadIdx.x == 0) {
@@ -1089,7 +1114,7 @@ hipError_t launch_upsample(hipStream_t s, const PairDesc *desc, int pairs, int d
 {
     const int Np = (N + 15) / 16 * 16;
     if (rf == 2 && Np <= kUp2MaxN && g) {
-        const size_t lds = (4 * (size_t)Np * (Np + 4) + Np) * sizeof(float);
+        const size_t lds = (4 * (size_t)Np * (Np + 4) + 2 * Np) * sizeof(float);
         const dim3 grid(pairs), block(kUp2Waves * kWave);
 #define AMPCOR_UP2(DT, QM)                                                                        \
     hipLaunchKernelGGL((k_upsample2<DT, QM>), grid, block, lds, s, desc, role, window, N, Np, g, \

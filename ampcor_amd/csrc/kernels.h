@@ -59,6 +59,12 @@ hipError_t launch_correlate(hipStream_t s, const CorrelateShape &shape, const fl
                             const float *amp, int pairs, int C, int W, float *partial);
 hipError_t launch_normalize(hipStream_t s, const float *partial, int nsplit, const double2 *sums,
                             const double2 *stats, int pairs, int C, int W, float *gamma);
+// window sums and normalization in one pass (W <= 512, P <= 144; tolerance-checked
+// stages): the FP32 form with `stats` (sum r', |r'|), the mixed form with `mstats`
+bool normalize_fuses(int W, int C);
+hipError_t launch_normalize_fused(hipStream_t s, const float *amp, int pairs, int W, int C,
+                                  const float *partial, int nsplit, const double2 *stats,
+                                  const MixedStats *mstats, float *gamma);
 hipError_t launch_coarse_peak(hipStream_t s, const float *gamma, int pairs, int P, int W, int E,
                               int refine_margin, int exclude, int2 *loc, int2 *window,
                               float *peak, float *snr);

@@ -341,6 +341,161 @@ __global__ __launch_bounds__(256) void k_normalize(const float *__restrict__ par
 }
 
 // ---------------------------------------------------------------------------
+// window sums and normalization in one pass, for the stages that are checked
+// against the FP64 oracle by tolerance (every mixed-precision stage and the
+// FP32 refined stage; the FP32 coarse stage keeps the separate kernels whose
+// order the CPU twin reproduces).  One workgroup per pair walks the
+// placement rows in blocks of kFuseRows: threads own target columns and
+// slide the vertical box sums V(u, x) = sum_{i<C} t(u+i, x) down in FP64;
+// the block's rows of V go to LDS and become row prefix sums (16 threads a
+// row, a 16-lane scan of the chunk totals), and each window sum is the
+// difference of two prefix entries.  The numerator partials are summed as in
+// the separate normalizers (pairwise tree for the FP32 form, in order for the
+// mixed form), and the sums never leave the chip.
+// ---------------------------------------------------------------------------
+constexpr int kFuseRows = 16;
+constexpr int kFuseThreads = 256;
+constexpr int kFuseMaxCols = 2;  // target columns per thread: W <= 512
+constexpr int kFuseOut = 9;      // outputs per thread and block: P <= 144
+
+template <bool MIXED>
+__global__ __launch_bounds__(kFuseThreads) void k_normalize_fused(
+    const float *__restrict__ amp, int W, int C, const float *__restrict__ partial, int nsplit,
+    const double2 *__restrict__ stats, const MixedStats *__restrict__ mstats,
+    float *__restrict__ gamma)
+{
+    extern __shared__ double fl[];   // [2][kFuseRows][W + 1]
+    const int pair = blockIdx.x, t = threadIdx.x;
+    const int P = W - C + 1, ws = W + 1;
+    double *p1 = fl, *p2 = fl + kFuseRows * ws;
+    const float *tp = amp + (size_t)pair * W * W;
+    const double n = (double)C * C;
+    // per-pair constants of the normalizer: the chip sum and the squared chip
+    // norm (centred), and for the mixed form the operand scales and the
+    // window constant
+    double sumr, chip2, inv_ab = 1.0, mean_t = 0.0;
+    if constexpr (MIXED) {
+        const MixedStats st = mstats[pair];
+        sumr = st.sumr;
+        chip2 = st.sumr2 - st.sumr * st.sumr / n;
+        inv_ab = 1.0 / (st.scale_r * st.scale_t);
+        mean_t = st.mean_t;
+    } else {
+        const double2 st = stats[pair];
+        sumr = st.x;
+        chip2 = st.y * st.y;
+    }
+
+    // vertical box sums at u = 0
+    double v1[kFuseMaxCols], v2[kFuseMaxCols];
+#pragma unroll
+    for (int q = 0; q < kFuseMaxCols; ++q) {
+        const int x = t + kFuseThreads * q;
+        v1[q] = v2[q] = 0.0;
+        if (x < W)
+#pragma unroll 16
+            for (int i = 0; i < C; ++i) {
+                const double d = (double)tp[(size_t)i * W + x];
+                v1[q] = v1[q] + d;
+                v2[q] = v2[q] + d * d;
+            }
+    }
+    const int r = t >> 4, sl = t & 15;      // row of the block, slot in the row
+    const int cw = (W + 15) / 16;            // prefix chunk per slot
+    const int x0 = sl * cw, x1 = min(W, x0 + cw);
+
+    for (int u0 = 0; u0 < P; u0 += kFuseRows) {
+        const int rows = min(kFuseRows, P - u0);
+        // the numerators of this thread's outputs (row r, columns sl + 16 j),
+        // loaded first so that their latency overlaps the box sums below
+        float num[kFuseOut];
+#pragma unroll
+        for (int j = 0; j < kFuseOut; ++j) num[j] = 0.0f;
+        if (r < rows) {
+            const float *pp = partial + (size_t)pair * nsplit * P * P + (size_t)(u0 + r) * P + sl;
+            for (int k = 0; k < nsplit; ++k)
+#pragma unroll
+                for (int j = 0; j < kFuseOut; ++j)
+                    if (sl + 16 * j < P) num[j] += pp[(size_t)k * P * P + 16 * j];
+        }
+        // slide V down to the block's rows and park them in LDS
+#pragma unroll
+        for (int q = 0; q < kFuseMaxCols; ++q) {
+            const int x = t + kFuseThreads * q;
+            if (x >= W) continue;
+            float a[kFuseRows], b[kFuseRows];
+#pragma unroll
+            for (int k = 0; k < kFuseRows; ++k) {
+                const int u = u0 + k;
+                a[k] = b[k] = 0.0f;
+                if (k < rows && u > 0) {
+                    a[k] = tp[(size_t)(u + C - 1) * W + x];
+                    b[k] = tp[(size_t)(u - 1) * W + x];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kFuseRows; ++k) {
+                if (k >= rows) break;
+                if (u0 + k > 0) {
+                    const double da = a[k], db = b[k];
+                    v1[q] = (v1[q] + da) - db;
+                    v2[q] = (v2[q] + da * da) - db * db;
+                }
+                p1[k * ws + x] = v1[q];
+                p2[k * ws + x] = v2[q];
+            }
+        }
+        __syncthreads();
+        // row prefix sums: chunk prefix, then the 16-lane scan of chunk totals
+        double run1 = 0.0, run2 = 0.0;
+        if (r < rows)
+            for (int x = x0; x < x1; ++x) {
+                run1 += p1[r * ws + x];
+                run2 += p2[r * ws + x];
+                p1[r * ws + x] = run1;
+                p2[r * ws + x] = run2;
+            }
+        double o1 = run1, o2 = run2;
+#pragma unroll
+        for (int d = 1; d < 16; d <<= 1) {
+            const double y1 = __shfl_up(o1, d, 16), y2 = __shfl_up(o2, d, 16);
+            if (sl >= d) {
+                o1 += y1;
+                o2 += y2;
+            }
+        }
+        o1 -= run1;   // exclusive: the totals of the chunks to the left
+        o2 -= run2;
+        if (r < rows && sl > 0)
+            for (int x = x0; x < x1; ++x) {
+                p1[r * ws + x] += o1;
+                p2[r * ws + x] += o2;
+            }
+        __syncthreads();
+        if (r < rows) {
+            const int u = u0 + r;
+#pragma unroll
+            for (int j = 0; j < kFuseOut; ++j) {
+                const int v = sl + 16 * j;
+                if (v >= P) break;
+                const double S1 = p1[r * ws + v + C - 1] - (v > 0 ? p1[r * ws + v - 1] : 0.0);
+                const double S2 = p2[r * ws + v + C - 1] - (v > 0 ? p2[r * ws + v - 1] : 0.0);
+                // FP64 throughout: the centred sums cancel, and a tiny variance
+                // must not flush to zero
+                const double mu = S1 / n;
+                const double var = S2 - mu * S1;
+                const double num_c = MIXED ? (double)num[j] * inv_ab + (mean_t - mu) * sumr
+                                           : (double)num[j] - mu * sumr;
+                float g = __builtin_nanf("");
+                if (var > 0.0 && chip2 > 0.0) g = (float)(num_c / sqrt(chip2 * var));
+                gamma[(size_t)pair * P * P + (size_t)u * P + v] = g;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // coarse peak (first maximum, NaN never wins), SNR and nudge; one wave a pair
 // ---------------------------------------------------------------------------
 __device__ inline void better(float &best, int &bidx, float x, int k)
@@ -1093,6 +1248,26 @@ hipError_t launch_normalize(hipStream_t s, const float *partial, int nsplit, con
     const unsigned blocks = (unsigned)((total + 255) / 256);
     hipLaunchKernelGGL(k_normalize, dim3(blocks), dim3(256), 0, s, partial, nsplit, sums, stats, C,
                        P, pairs, gamma);
+    return hipGetLastError();
+}
+
+bool normalize_fuses(int W, int C)
+{
+    return W <= kFuseThreads * kFuseMaxCols && W - C + 1 <= 16 * kFuseOut;
+}
+
+hipError_t launch_normalize_fused(hipStream_t s, const float *amp, int pairs, int W, int C,
+                                  const float *partial, int nsplit, const double2 *stats,
+                                  const MixedStats *mstats, float *gamma)
+{
+    if (!normalize_fuses(W, C) || (!stats && !mstats)) return hipErrorInvalidValue;
+    const size_t lds = 2 * (size_t)kFuseRows * (W + 1) * sizeof(double);
+    if (mstats)
+        hipLaunchKernelGGL(k_normalize_fused<true>, dim3(pairs), dim3(kFuseThreads), lds, s, amp, W,
+                           C, partial, nsplit, stats, mstats, gamma);
+    else
+        hipLaunchKernelGGL(k_normalize_fused<false>, dim3(pairs), dim3(kFuseThreads), lds, s, amp,
+                           W, C, partial, nsplit, stats, mstats, gamma);
     return hipGetLastError();
 }
 

@@ -489,14 +489,25 @@ int plan_enqueue(ampcor_hip_plan *plan, int N)
             STEP(launch_chip_stats(s, p->amp_ref.ptr, N, p->C, p->rT.ptr, p->stats.ptr),
                  "chip_stats");
     }
-    STEP(launch_window_sums(s, p->amp_tgt.ptr, N, p->W, p->C, p->sums.ptr), "window_sums");
+    // the FP32 coarse stage keeps the separate window sums, whose FP64 order the
+    // CPU twin reproduces; the other stages fuse them into the normalizer
+    const bool fuse_c = p->mixed && normalize_fuses(p->W, p->C);
+    const bool fuse_r = normalize_fuses(p->Wr, p->Cr);
+    if (!fuse_c)
+        STEP(launch_window_sums(s, p->amp_tgt.ptr, N, p->W, p->C, p->sums.ptr), "window_sums");
     if (p->mixed) {
         STEP(launch_correlate_mixed(s, p->mixed_coarse, p->chip_q.ptr, p->tgt_q.ptr, N, p->C, p->W,
                                     fp16, p->partial.ptr),
              "correlate");
-        STEP(launch_normalize_mixed(s, p->partial.ptr, p->mixed_coarse.nsplit, p->sums.ptr,
-                                    p->mstats.ptr, N, p->C, p->W, p->gamma.ptr),
-             "normalize");
+        if (fuse_c)
+            STEP(launch_normalize_fused(s, p->amp_tgt.ptr, N, p->W, p->C, p->partial.ptr,
+                                        p->mixed_coarse.nsplit, nullptr, p->mstats.ptr,
+                                        p->gamma.ptr),
+                 "normalize");
+        else
+            STEP(launch_normalize_mixed(s, p->partial.ptr, p->mixed_coarse.nsplit, p->sums.ptr,
+                                        p->mstats.ptr, N, p->C, p->W, p->gamma.ptr),
+                 "normalize");
     } else {
         STEP(launch_correlate(s, p->coarse_shape, p->rT.ptr, p->amp_tgt.ptr, N, p->C, p->W,
                               p->partial.ptr),
@@ -535,23 +546,36 @@ int plan_enqueue(ampcor_hip_plan *plan, int N)
             STEP(launch_chip_stats(s, p->amp_ref_r.ptr, N, p->Cr, p->rT_r.ptr, p->stats_r.ptr),
                  "chip_stats_refined");
     }
-    STEP(launch_window_sums(s, p->amp_tgt_r.ptr, N, p->Wr, p->Cr, p->sums_r.ptr),
-         "window_sums_refined");
+    if (!fuse_r)
+        STEP(launch_window_sums(s, p->amp_tgt_r.ptr, N, p->Wr, p->Cr, p->sums_r.ptr),
+             "window_sums_refined");
     if (p->mixed) {
         STEP(launch_correlate_mixed(s, p->mixed_refined, p->chip_q_r.ptr, p->tgt_q_r.ptr, N, p->Cr,
                                     p->Wr, fp16, p->partial_r.ptr, p->chip_lo_r.ptr,
                                     p->tgt_lo_r.ptr),
              "correlate_refined");
-        STEP(launch_normalize_mixed(s, p->partial_r.ptr, p->mixed_refined.nsplit, p->sums_r.ptr,
-                                    p->mstats_r.ptr, N, p->Cr, p->Wr, p->gamma_r.ptr),
-             "normalize_refined");
+        if (fuse_r)
+            STEP(launch_normalize_fused(s, p->amp_tgt_r.ptr, N, p->Wr, p->Cr, p->partial_r.ptr,
+                                        p->mixed_refined.nsplit, nullptr, p->mstats_r.ptr,
+                                        p->gamma_r.ptr),
+                 "normalize_refined");
+        else
+            STEP(launch_normalize_mixed(s, p->partial_r.ptr, p->mixed_refined.nsplit, p->sums_r.ptr,
+                                        p->mstats_r.ptr, N, p->Cr, p->Wr, p->gamma_r.ptr),
+                 "normalize_refined");
     } else {
         STEP(launch_correlate(s, p->refined_shape, p->rT_r.ptr, p->amp_tgt_r.ptr, N, p->Cr, p->Wr,
                               p->partial_r.ptr),
              "correlate_refined");
-        STEP(launch_normalize(s, p->partial_r.ptr, p->refined_shape.nsplit, p->sums_r.ptr,
-                              p->stats_r.ptr, N, p->Cr, p->Wr, p->gamma_r.ptr),
-             "normalize_refined");
+        if (fuse_r)
+            STEP(launch_normalize_fused(s, p->amp_tgt_r.ptr, N, p->Wr, p->Cr, p->partial_r.ptr,
+                                        p->refined_shape.nsplit, p->stats_r.ptr, nullptr,
+                                        p->gamma_r.ptr),
+                 "normalize_refined");
+        else
+            STEP(launch_normalize(s, p->partial_r.ptr, p->refined_shape.nsplit, p->sums_r.ptr,
+                                  p->stats_r.ptr, N, p->Cr, p->Wr, p->gamma_r.ptr),
+                 "normalize_refined");
     }
     STEP(launch_zoom(s, p->gamma_r.ptr, N, p->R, p->zf, p->D.ptr, p->window.ptr, p->peak.ptr, p->m,
                      p->rf, p->offsets.ptr, p->peak_r.ptr, p->zoom_loc.ptr),

@@ -79,11 +79,16 @@ hipError_t launch_coarse_peak(hipStream_t s, const float *gamma, int pairs, int 
 // the pair's MixedStats.  amp may be null for role 0 then.
 bool upsample_fuses_quantization(int N, int rf);
 // qlo (optional): second quantized plane holding the rounding remainder of q
-// (the split hi/lo operands), same layout as q
+// (the split hi/lo operands), same layout as q.
+// hfrag (optional, quantized modes without qlo): the half-sample circulant as
+// FP16 MFMA fragments (upsample_fragments); the products then run on FP16
+// matrix cores instead of FP32 ones
+int upsample_fragment_count(int N);
 hipError_t launch_upsample(hipStream_t s, const PairDesc *desc, int pairs, int dtype, int role,
                            const int2 *window, int N, int rf, const float *U, const float *g,
                            float *amp, int qmode = kUpFP32, uint16_t *q = nullptr, int cs = 0,
-                           MixedStats *stats = nullptr, uint16_t *qlo = nullptr);
+                           MixedStats *stats = nullptr, uint16_t *qlo = nullptr,
+                           const uint16_t *hfrag = nullptr);
 // mixed coarse stage in one pass when the tiles fit in LDS: gather, detect,
 // write the FP32 window (for the window sums) and both quantized operands
 bool gather_prep_fits(int C, int W);

@@ -940,6 +940,243 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2(
 }
 
 // ---------------------------------------------------------------------------
+// x2 upsampling for the BF16/FP16 forms: the same three circulant products as
+// k_upsample2, on v_mfma_f32_16x16x32_f16 with FP32 accumulation.  The tile
+// is scaled by a power of two into FP16 range (a first pass over it takes
+// its peak and mean amplitude, and keeps the FP32 amplitudes of the coarse
+// samples y[2a][2b]); LDS holds it twice, row- and column-major, because the
+// first product contracts over its rows and the second over its columns.
+// The circulant comes as precomputed FP16 fragments (hfrag, one 16 x 32
+// block per 64 lanes x 8 values: H[16 mt + l % 16][32 kc + 8 (l / 16) + e]),
+// which serve as the A operand of T = H x and the B operand of
+// Y_s = T_s H^T.  T = H x is kept in FP16 in the space of the column-major
+// copy.  Rounding T and x to FP16 (11 bits) stays well below the BF16/FP16
+// rounding of the operands written for the contraction.
+// ---------------------------------------------------------------------------
+constexpr int kUp2hMaxN = 96;
+
+typedef _Float16 halfx8_t __attribute__((ext_vector_type(8)));
+
+__device__ inline floatx4 mfma_f16(uint4 a, uint4 b, floatx4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(halfx8_t, a),
+                                                  __builtin_bit_cast(halfx8_t, b), c, 0, 0, 0);
+}
+
+// LDS layout of k_upsample2h (bytes): four FP16 planes [Np][Np + 8], the FP32
+// amplitudes [Np][Np + 1], the fragments
+__host__ __device__ inline size_t up2h_plane_bytes(int Np) { return (size_t)Np * (Np + 8) * 2; }
+__host__ __device__ inline size_t up2h_amp_offset(int Np) { return 4 * up2h_plane_bytes(Np); }
+__host__ __device__ inline size_t up2h_frag_offset(int Np)
+{
+    return (up2h_amp_offset(Np) + (size_t)Np * (Np + 1) * 4 + 15) / 16 * 16;
+}
+__host__ __device__ inline size_t up2h_lds_bytes(int Np)
+{
+    return up2h_frag_offset(Np) + (size_t)(Np / 16) * (Np / 32) * 64 * 16;
+}
+
+template <int DTYPE, int QMODE>
+__global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2h(
+    const PairDesc *__restrict__ desc, int role, const int2 *__restrict__ window, int N, int Np,
+    const uint4 *__restrict__ hfrag, float *__restrict__ amp, uint16_t *__restrict__ qout, int cs,
+    MixedStats *__restrict__ stats)
+{
+    extern __shared__ uint4 lds4[];
+    __shared__ double red[kUp2Waves], red2[kUp2Waves];
+    __shared__ float redm[kUp2Waves];
+    unsigned char *lb = reinterpret_cast<unsigned char *>(lds4);
+    const int Ls = Np + 8, As = Np + 1;
+    const size_t pb = up2h_plane_bytes(Np);
+    _Float16 *xr = reinterpret_cast<_Float16 *>(lb);            // x, row-major
+    _Float16 *xi = reinterpret_cast<_Float16 *>(lb + pb);
+    _Float16 *cr = reinterpret_cast<_Float16 *>(lb + 2 * pb);   // x column-major, then T row-major
+    _Float16 *ci = reinterpret_cast<_Float16 *>(lb + 3 * pb);
+    float *ax = reinterpret_cast<float *>(lb + up2h_amp_offset(Np));   // |x| (FP32)
+    uint4 *hf = reinterpret_cast<uint4 *>(lb + up2h_frag_offset(Np));
+    const int MT = Np / 16, KC = Np / 32;
+    const int pair = blockIdx.x;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x >> 6;
+    const int lm = lane & 15, lk = lane >> 4;
+    const PairDesc d = desc[pair];
+    const void *base = role == 0 ? d.ref : d.tgt;
+    const size_t stride = role == 0 ? d.ref_stride : d.tgt_stride;
+    int line = role == 0 ? d.ref_line : d.tgt_line;
+    int sample = role == 0 ? d.ref_sample : d.tgt_sample;
+    if (role == 1) {
+        const int2 tl = window[pair];
+        line += tl.x;
+        sample += tl.y;
+    }
+    auto load = [&](int r, int c, float &re, float &im) {
+        const size_t off = (size_t)(line + r) * stride + sample + c;
+        if (DTYPE == 0) {
+            const float2 z = static_cast<const float2 *>(base)[off];
+            re = z.x;
+            im = z.y;
+        } else {
+            re = static_cast<const float *>(base)[off];
+            im = 0.0f;
+        }
+    };
+
+    // pass 1: FP32 amplitudes of the coarse samples, their sum and peak
+    double asum = 0.0;
+    float amax = 0.0f;
+    const int nn = N * N;
+    for (int k0 = threadIdx.x; k0 < nn; k0 += kUp2Batch * blockDim.x) {
+        float re[kUp2Batch], im[kUp2Batch];
+#pragma unroll
+        for (int j = 0; j < kUp2Batch; ++j) {
+            const int k = k0 + j * blockDim.x;
+            re[j] = im[j] = 0.0f;
+            if (k < nn) load(k / N, k % N, re[j], im[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < kUp2Batch; ++j) {
+            const int k = k0 + j * blockDim.x;
+            if (k < nn) {
+                const float a = amplitude(re[j], im[j]);
+                ax[(k / N) * As + k % N] = a;
+                asum += a;
+                amax = fmaxf(amax, a);
+            }
+        }
+    }
+    for (int k = threadIdx.x; k < MT * KC * kWave; k += blockDim.x) hf[k] = hfrag[k];
+    block_sum_max(red, redm, asum, amax);
+    const int M = 2 * N;
+    const float qc = (float)(asum / ((double)N * N));
+    const float qs = (float)quant_scale(QMODE, 2.0f * amax);
+    // FP16 range: |re|, |im| <= amax, scaled below 2^13 (T = H x stays below 2^15)
+    int ex = 0;
+    if (amax > 0.0f) frexpf(amax, &ex);
+    const float sx = amax > 0.0f ? ldexpf(1.0f, 13 - ex) : 1.0f, inv_sx = 1.0f / sx;
+    if (role == 0)
+        for (int k = threadIdx.x; k < M * (cs - M); k += blockDim.x) {
+            const int p = k / (cs - M), c = k - p * (cs - M);
+            const int col = c < 16 ? c : M + c;
+            qout[((size_t)pair * M + p) * cs + col] = 0;
+        }
+    // pass 2: the scaled FP16 tile, both layouts, zero-padded to Np x Np
+    const int npp = Np * Np;
+    for (int k0 = threadIdx.x; k0 < npp; k0 += kUp2Batch * blockDim.x) {
+        float re[kUp2Batch], im[kUp2Batch];
+#pragma unroll
+        for (int j = 0; j < kUp2Batch; ++j) {
+            const int k = k0 + j * blockDim.x;
+            const int r = k / Np, c = k - r * Np;
+            re[j] = im[j] = 0.0f;
+            if (k < npp && r < N && c < N) load(r, c, re[j], im[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < kUp2Batch; ++j) {
+            const int k = k0 + j * blockDim.x;
+            if (k < npp) {
+                const int r = k / Np, c = k - r * Np;
+                const _Float16 hr = (_Float16)(re[j] * sx), hi = (_Float16)(im[j] * sx);
+                xr[r * Ls + c] = hr;
+                xi[r * Ls + c] = hi;
+                cr[c * Ls + r] = hr;
+                ci[c * Ls + r] = hi;
+            }
+        }
+    }
+    __syncthreads();
+
+    // stage 1: T = H x, tiles (mt, ct); a wave keeps its tiles until every
+    // wave is done with the column-major copy
+    constexpr int kMaxT = (kUp2hMaxN / 16) * (kUp2hMaxN / 16) / kUp2Waves + 1;
+    floatx4 tR[kMaxT], tI[kMaxT];
+    const int ntile = MT * MT;
+#pragma unroll
+    for (int q = 0; q < kMaxT; ++q) {
+        tR[q] = tI[q] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+        const int tile = wave + kUp2Waves * q;
+        if (tile >= ntile) continue;
+        const int mt = tile / MT, ct = tile - mt * MT;
+        const _Float16 *br = cr + (16 * ct + lm) * Ls + 8 * lk;
+        const _Float16 *bi = ci + (16 * ct + lm) * Ls + 8 * lk;
+        for (int kc = 0; kc < KC; ++kc) {
+            const uint4 a = hf[(mt * KC + kc) * kWave + lane];
+            tR[q] = mfma_f16(a, *reinterpret_cast<const uint4 *>(br + 32 * kc), tR[q]);
+            tI[q] = mfma_f16(a, *reinterpret_cast<const uint4 *>(bi + 32 * kc), tI[q]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kMaxT; ++q) {
+        const int tile = wave + kUp2Waves * q;
+        if (tile >= ntile) continue;
+        const int mt = tile / MT, ct = tile - mt * MT;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int at = (16 * mt + 4 * lk + i) * Ls + 16 * ct + lm;
+            cr[at] = (_Float16)tR[q][i];
+            ci[at] = (_Float16)tI[q][i];
+        }
+    }
+    __syncthreads();
+
+    float *out = amp ? amp + (size_t)pair * M * M : nullptr;
+    double s1 = 0.0, s2 = 0.0;
+    auto emit2 = [&](int p, int q, float v0, float v1) {
+        if (role == 1)
+            *reinterpret_cast<float2 *>(out + (size_t)p * M + q) = make_float2(v0, v1);
+        const uint16_t h0 = quantize(QMODE, (v0 - qc) * qs), h1 = quantize(QMODE, (v1 - qc) * qs);
+        const size_t at = role == 0 ? ((size_t)pair * M + p) * cs + 16 + q
+                                    : ((size_t)pair * M + p) * M + q;
+        *reinterpret_cast<uint32_t *>(qout + at) = (uint32_t)h0 | ((uint32_t)h1 << 16);
+        if (role == 0) {
+            const double b0 = (double)dequantize(QMODE, h0) / qs;
+            const double b1 = (double)dequantize(QMODE, h1) / qs;
+            s1 += b0 + b1;
+            s2 += b0 * b0 + b1 * b1;
+        }
+    };
+    // stage 2: odd columns, Y_s = T_s H^T for T_0 = x and T_1 = T, then the
+    // four phases of each output 2 x 2 block
+    for (int tile = wave; tile < ntile; tile += kUp2Waves) {
+        const int at = tile / MT, bt = tile - at * MT;
+        const int arow = (16 * at + lm) * Ls + 8 * lk;
+        floatx4 xR = {0.0f, 0.0f, 0.0f, 0.0f}, xI = xR, yR = xR, yI = xR;
+        for (int kc = 0; kc < KC; ++kc) {
+            const uint4 b = hf[(bt * KC + kc) * kWave + lane];
+            const int o = arow + 32 * kc;
+            xR = mfma_f16(*reinterpret_cast<const uint4 *>(xr + o), b, xR);
+            xI = mfma_f16(*reinterpret_cast<const uint4 *>(xi + o), b, xI);
+            yR = mfma_f16(*reinterpret_cast<const uint4 *>(cr + o), b, yR);
+            yI = mfma_f16(*reinterpret_cast<const uint4 *>(ci + o), b, yI);
+        }
+        const int bcol = 16 * bt + lm;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int a = 16 * at + 4 * lk + i;
+            if (a < N && bcol < N) {
+                const int t = a * Ls + bcol;
+                const float t1 = amplitude((float)cr[t], (float)ci[t]) * inv_sx;
+                emit2(2 * a, 2 * bcol, ax[a * As + bcol], amplitude(xR[i], xI[i]) * inv_sx);
+                emit2(2 * a + 1, 2 * bcol, t1, amplitude(yR[i], yI[i]) * inv_sx);
+            }
+        }
+    }
+    if (role == 0) {
+        float unused = 0.0f;
+        block_sum_max(red, redm, s1, unused);
+        block_sum_max(red2, redm, s2, unused);
+        if (threadIdx.x == 0) {
+            stats[pair].sumr = s1;
+            stats[pair].sumr2 = s2;
+            stats[pair].scale_r = qs;
+        }
+    } else if (threadIdx.x == 0) {
+        stats[pair].scale_t = qs;
+        stats[pair].mean_t = qc;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // mixed coarse stage in one pass: one workgroup per pair gathers and detects
 // the chip into LDS (its exact mean is removed before rounding), then streams
 // the window: FP32 amplitudes for the exact window sums, and the quantized
@@ -1282,12 +1519,32 @@ hipError_t launch_coarse_peak(hipStream_t s, const float *gamma, int pairs, int 
 
 bool upsample_fuses_quantization(int N, int rf) { return rf == 2 && (N + 15) / 16 * 16 <= kUp2MaxN; }
 
+int upsample_fragment_count(int N) { return ((N + 31) / 32 * 32 / 16) * ((N + 31) / 32) * 64 * 8; }
+
 hipError_t launch_upsample(hipStream_t s, const PairDesc *desc, int pairs, int dtype, int role,
                            const int2 *window, int N, int rf, const float *U, const float *g,
                            float *amp, int qmode, uint16_t *q, int cs, MixedStats *stats,
-                           uint16_t *qlo)
+                           uint16_t *qlo, const uint16_t *hfrag)
 {
     const int Np = (N + 15) / 16 * 16;
+    const int Np32 = (N + 31) / 32 * 32;
+    if (rf == 2 && hfrag && qmode != kUpFP32 && !qlo && Np32 <= kUp2hMaxN) {
+        const size_t lds = up2h_lds_bytes(Np32);
+        const dim3 grid(pairs), block(kUp2Waves * kWave);
+        const uint4 *hf = reinterpret_cast<const uint4 *>(hfrag);
+#define AMPCOR_UP2H(DT, QM)                                                                        \
+    hipLaunchKernelGGL((k_upsample2h<DT, QM>), grid, block, lds, s, desc, role, window, N, Np32, hf, \
+                       amp, q, cs, stats)
+        if (dtype == 0) {
+            if (qmode == kUpFP16) AMPCOR_UP2H(0, kUpFP16);
+            else AMPCOR_UP2H(0, kUpBF16);
+        } else {
+            if (qmode == kUpFP16) AMPCOR_UP2H(1, kUpFP16);
+            else AMPCOR_UP2H(1, kUpBF16);
+        }
+#undef AMPCOR_UP2H
+        return hipGetLastError();
+    }
     if (rf == 2 && Np <= kUp2MaxN && g) {
         const size_t lds = (4 * (size_t)Np * (Np + 4) + 2 * Np) * sizeof(float);
         const dim3 grid(pairs), block(kUp2Waves * kWave);

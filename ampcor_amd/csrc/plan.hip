@@ -102,6 +102,32 @@ int upload_upsampler(DeviceBuffer<float> &U, DeviceBuffer<float> &g, int N, int 
     return upload_matrix(g, row);
 }
 
+// the half-sample circulant H[a][n] = g((a - n) mod N) as FP16 fragments of the
+// 16 x 16 x 32 MFMA: block (mt, kc), lane l, value e holds
+// H[16 mt + l % 16][32 kc + 8 (l / 16) + e], zero outside N x N
+int upload_fragments(DeviceBuffer<uint16_t> &hf, int N)
+{
+    const int Np = (N + 31) / 32 * 32, MT = Np / 16, KC = Np / 32;
+    std::vector<double> g(N);
+    for (int d = 0; d < N; ++d) g[d] = upsample_kernel(N, 2, 2 * d + 1, 0);
+    std::vector<uint16_t> h((size_t)upsample_fragment_count(N), 0);
+    size_t at = 0;
+    for (int mt = 0; mt < MT; ++mt)
+        for (int kc = 0; kc < KC; ++kc)
+            for (int l = 0; l < 64; ++l)
+                for (int e = 0; e < 8; ++e, ++at) {
+                    const int row = 16 * mt + l % 16, col = 32 * kc + 8 * (l / 16) + e;
+                    if (row < N && col < N) {
+                        const _Float16 v = (_Float16)g[((row - col) % N + N) % N];
+                        h[at] = __builtin_bit_cast(uint16_t, v);
+                    }
+                }
+    hipError_t e = hf.alloc(h.size());
+    if (e == hipSuccess)
+        e = hipMemcpy(hf.ptr, h.data(), h.size() * sizeof(uint16_t), hipMemcpyHostToDevice);
+    return e == hipSuccess ? AMPCOR_HIP_OK : hip_fail(e, "uploading the upsampling fragments");
+}
+
 int register_tile(ampcor_hip_plan *plan, const ampcor_hip_raster *raster, size_t idx,
                   size_t bl, size_t bs, size_t el, size_t es, bool reference)
 {
@@ -354,6 +380,11 @@ int ampcor_hip_plan_create(int device, size_t pairs, size_t ref_lines, size_t re
     }
     if (int rc = upload_upsampler(p->U_ref, p->g_ref, p->C, p->rf)) return cleanup(rc);
     if (int rc = upload_upsampler(p->U_tgt, p->g_tgt, p->E, p->rf)) return cleanup(rc);
+    // the BF16/FP16 forms upsample on FP16 matrix cores (the split forms keep FP32)
+    if (p->mixed && !p->split && p->rf == 2) {
+        if (int rc = upload_fragments(p->hfrag_ref, p->C)) return cleanup(rc);
+        if (int rc = upload_fragments(p->hfrag_tgt, p->E)) return cleanup(rc);
+    }
     {
         std::vector<double> d(2 * p->Z + 1);
         for (int k = -p->Z; k <= p->Z; ++k) d[k + p->Z] = dirichlet(p->R, p->zf, k);
@@ -524,11 +555,12 @@ int plan_enqueue(ampcor_hip_plan *plan, int N)
     if (fused) {
         STEP(launch_upsample(s, p->d_desc.ptr, N, p->dtype, 0, p->window.ptr, p->C, p->rf,
                              p->U_ref.ptr, p->g_ref.ptr, nullptr, qmode, p->chip_q_r.ptr,
-                             p->mixed_refined.cs, p->mstats_r.ptr, p->chip_lo_r.ptr),
+                             p->mixed_refined.cs, p->mstats_r.ptr, p->chip_lo_r.ptr,
+                             p->hfrag_ref.ptr),
              "upsample_ref");
         STEP(launch_upsample(s, p->d_desc.ptr, N, p->dtype, 1, p->window.ptr, p->E, p->rf,
                              p->U_tgt.ptr, p->g_tgt.ptr, p->amp_tgt_r.ptr, qmode, p->tgt_q_r.ptr, 0,
-                             p->mstats_r.ptr, p->tgt_lo_r.ptr),
+                             p->mstats_r.ptr, p->tgt_lo_r.ptr, p->hfrag_tgt.ptr),
              "upsample_tgt");
     } else {
         STEP(launch_upsample(s, p->d_desc.ptr, N, p->dtype, 0, p->window.ptr, p->C, p->rf,

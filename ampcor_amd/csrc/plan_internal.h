@@ -91,6 +91,7 @@ struct ampcor_hip_plan {
     ampcor_hip::DeviceBuffer<float> U_ref, U_tgt, g_ref, g_tgt;
     ampcor_hip::DeviceBuffer<double> D;
     ampcor_hip::DeviceBuffer<uint16_t> chip_q, tgt_q, chip_q_r, tgt_q_r, chip_lo_r, tgt_lo_r;
+    ampcor_hip::DeviceBuffer<uint16_t> hfrag_ref, hfrag_tgt;  // FP16 circulant fragments
     ampcor_hip::DeviceBuffer<ampcor_hip::MixedStats> mstats, mstats_r;
 
     bool timing = false;
@@ -114,6 +115,8 @@ struct ampcor_hip_plan {
         tgt_q_r.release();
         chip_lo_r.release();
         tgt_lo_r.release();
+        hfrag_ref.release();
+        hfrag_tgt.release();
         mstats.release();
         mstats_r.release();
         D.release();

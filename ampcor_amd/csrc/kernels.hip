@@ -943,15 +943,17 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2(
 // x2 upsampling for the BF16/FP16 forms: the same three circulant products as
 // k_upsample2, on v_mfma_f32_16x16x32_f16 with FP32 accumulation.  The tile
 // is scaled by a power of two into FP16 range (a first pass over it takes
-// its peak and mean amplitude, and keeps the FP32 amplitudes of the coarse
-// samples y[2a][2b]); LDS holds it twice, row- and column-major, because the
-// first product contracts over its rows and the second over its columns.
-// The circulant comes as precomputed FP16 fragments (hfrag, one 16 x 32
-// block per 64 lanes x 8 values: H[16 mt + l % 16][32 kc + 8 (l / 16) + e]),
-// which serve as the A operand of T = H x and the B operand of
-// Y_s = T_s H^T.  T = H x is kept in FP16 in the space of the column-major
-// copy.  Rounding T and x to FP16 (11 bits) stays well below the BF16/FP16
-// rounding of the operands written for the contraction.
+// its peak and mean amplitude); LDS holds it twice, row- and column-major,
+// because the first product contracts over its rows and the second over its
+// columns (78 KiB for N = 96: two workgroups share a CU, so that one's
+// global loads overlap the other's products).  The circulant comes as
+// precomputed FP16 fragments (hfrag, one 16 x 32 block per 64 lanes x 8
+// values: H[16 mt + l % 16][32 kc + 8 (l / 16) + e]), read from global
+// memory (L2-resident, shared by every pair), which serve as the A operand
+// of T = H x and the B operand of Y_s = T_s H^T.  T = H x is kept in FP16 in
+// the space of the column-major copy.  Rounding x and T to FP16 (11 bits)
+// stays well below the BF16/FP16 rounding of the operands written for the
+// contraction.
 // ---------------------------------------------------------------------------
 constexpr int kUp2hMaxN = 96;
 
@@ -963,18 +965,9 @@ __device__ inline floatx4 mfma_f16(uint4 a, uint4 b, floatx4 c)
                                                   __builtin_bit_cast(halfx8_t, b), c, 0, 0, 0);
 }
 
-// LDS layout of k_upsample2h (bytes): four FP16 planes [Np][Np + 8], the FP32
-// amplitudes [Np][Np + 1], the fragments
+// LDS of k_upsample2h (bytes): four FP16 planes [Np][Np + 8]
 __host__ __device__ inline size_t up2h_plane_bytes(int Np) { return (size_t)Np * (Np + 8) * 2; }
-__host__ __device__ inline size_t up2h_amp_offset(int Np) { return 4 * up2h_plane_bytes(Np); }
-__host__ __device__ inline size_t up2h_frag_offset(int Np)
-{
-    return (up2h_amp_offset(Np) + (size_t)Np * (Np + 1) * 4 + 15) / 16 * 16;
-}
-__host__ __device__ inline size_t up2h_lds_bytes(int Np)
-{
-    return up2h_frag_offset(Np) + (size_t)(Np / 16) * (Np / 32) * 64 * 16;
-}
+__host__ __device__ inline size_t up2h_lds_bytes(int Np) { return 4 * up2h_plane_bytes(Np); }
 
 template <int DTYPE, int QMODE>
 __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2h(
@@ -986,14 +979,12 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2h(
     __shared__ double red[kUp2Waves], red2[kUp2Waves];
     __shared__ float redm[kUp2Waves];
     unsigned char *lb = reinterpret_cast<unsigned char *>(lds4);
-    const int Ls = Np + 8, As = Np + 1;
+    const int Ls = Np + 8;
     const size_t pb = up2h_plane_bytes(Np);
     _Float16 *xr = reinterpret_cast<_Float16 *>(lb);            // x, row-major
     _Float16 *xi = reinterpret_cast<_Float16 *>(lb + pb);
     _Float16 *cr = reinterpret_cast<_Float16 *>(lb + 2 * pb);   // x column-major, then T row-major
     _Float16 *ci = reinterpret_cast<_Float16 *>(lb + 3 * pb);
-    float *ax = reinterpret_cast<float *>(lb + up2h_amp_offset(Np));   // |x| (FP32)
-    uint4 *hf = reinterpret_cast<uint4 *>(lb + up2h_frag_offset(Np));
     const int MT = Np / 16, KC = Np / 32;
     const int pair = blockIdx.x;
     const int lane = threadIdx.x & (kWave - 1);
@@ -1021,7 +1012,7 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2h(
         }
     };
 
-    // pass 1: FP32 amplitudes of the coarse samples, their sum and peak
+    // pass 1: the sum and peak of the coarse amplitudes
     double asum = 0.0;
     float amax = 0.0f;
     const int nn = N * N;
@@ -1038,13 +1029,11 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2h(
             const int k = k0 + j * blockDim.x;
             if (k < nn) {
                 const float a = amplitude(re[j], im[j]);
-                ax[(k / N) * As + k % N] = a;
                 asum += a;
                 amax = fmaxf(amax, a);
             }
         }
     }
-    for (int k = threadIdx.x; k < MT * KC * kWave; k += blockDim.x) hf[k] = hfrag[k];
     block_sum_max(red, redm, asum, amax);
     const int M = 2 * N;
     const float qc = (float)(asum / ((double)N * N));
@@ -1098,8 +1087,14 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2h(
         const int mt = tile / MT, ct = tile - mt * MT;
         const _Float16 *br = cr + (16 * ct + lm) * Ls + 8 * lk;
         const _Float16 *bi = ci + (16 * ct + lm) * Ls + 8 * lk;
-        for (int kc = 0; kc < KC; ++kc) {
-            const uint4 a = hf[(mt * KC + kc) * kWave + lane];
+        uint4 fa[kUp2hMaxN / 32];
+#pragma unroll
+        for (int kc = 0; kc < kUp2hMaxN / 32; ++kc)
+            if (kc < KC) fa[kc] = hfrag[(mt * KC + kc) * kWave + lane];
+#pragma unroll
+        for (int kc = 0; kc < kUp2hMaxN / 32; ++kc) {
+            if (kc >= KC) break;
+            const uint4 a = fa[kc];
             tR[q] = mfma_f16(a, *reinterpret_cast<const uint4 *>(br + 32 * kc), tR[q]);
             tI[q] = mfma_f16(a, *reinterpret_cast<const uint4 *>(bi + 32 * kc), tI[q]);
         }
@@ -1141,8 +1136,14 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2h(
         const int at = tile / MT, bt = tile - at * MT;
         const int arow = (16 * at + lm) * Ls + 8 * lk;
         floatx4 xR = {0.0f, 0.0f, 0.0f, 0.0f}, xI = xR, yR = xR, yI = xR;
-        for (int kc = 0; kc < KC; ++kc) {
-            const uint4 b = hf[(bt * KC + kc) * kWave + lane];
+        uint4 fb[kUp2hMaxN / 32];
+#pragma unroll
+        for (int kc = 0; kc < kUp2hMaxN / 32; ++kc)
+            if (kc < KC) fb[kc] = hfrag[(bt * KC + kc) * kWave + lane];
+#pragma unroll
+        for (int kc = 0; kc < kUp2hMaxN / 32; ++kc) {
+            if (kc >= KC) break;
+            const uint4 b = fb[kc];
             const int o = arow + 32 * kc;
             xR = mfma_f16(*reinterpret_cast<const uint4 *>(xr + o), b, xR);
             xI = mfma_f16(*reinterpret_cast<const uint4 *>(xi + o), b, xI);
@@ -1155,8 +1156,9 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2h(
             const int a = 16 * at + 4 * lk + i;
             if (a < N && bcol < N) {
                 const int t = a * Ls + bcol;
+                const float x0 = amplitude((float)xr[t], (float)xi[t]) * inv_sx;
                 const float t1 = amplitude((float)cr[t], (float)ci[t]) * inv_sx;
-                emit2(2 * a, 2 * bcol, ax[a * As + bcol], amplitude(xR[i], xI[i]) * inv_sx);
+                emit2(2 * a, 2 * bcol, x0, amplitude(xR[i], xI[i]) * inv_sx);
                 emit2(2 * a + 1, 2 * bcol, t1, amplitude(yR[i], yI[i]) * inv_sx);
             }
         }

@@ -60,10 +60,13 @@ hipError_t launch_correlate(hipStream_t s, const CorrelateShape &shape, const fl
 hipError_t launch_normalize(hipStream_t s, const float *partial, int nsplit, const double2 *sums,
                             const double2 *stats, int pairs, int C, int W, float *gamma);
 // window sums and normalization in one pass (W <= 512, P <= 144; tolerance-checked
-// stages): the FP32 form with `stats` (sum r', |r'|), the mixed form with `mstats`
+// stages).  form kFuseFP32: FP32 amplitudes and `stats` (sum r', |r'|);
+// kFuseMixed: FP32 amplitudes and `mstats`; kFuseBF16 / kFuseFP16: the sums
+// come from the quantized target (the contraction's operand) and `mstats`
+enum { kFuseFP32 = 0, kFuseMixed = 1, kFuseBF16 = 2, kFuseFP16 = 3 };
 bool normalize_fuses(int W, int C);
-hipError_t launch_normalize_fused(hipStream_t s, const float *amp, int pairs, int W, int C,
-                                  const float *partial, int nsplit, const double2 *stats,
+hipError_t launch_normalize_fused(hipStream_t s, int form, const void *target, int pairs, int W,
+                                  int C, const float *partial, int nsplit, const double2 *stats,
                                   const MixedStats *mstats, float *gamma);
 hipError_t launch_coarse_peak(hipStream_t s, const float *gamma, int pairs, int P, int W, int E,
                               int refine_margin, int exclude, int2 *loc, int2 *window,
@@ -73,10 +76,11 @@ hipError_t launch_coarse_peak(hipStream_t s, const float *gamma, int pairs, int 
 // windows (N = E, origin = window corner + nudged (top, left)); U is the
 // (rf N) x N upsampling matrix stored padded to multiples of 16, zero-filled;
 // g (rf = 2 only) is its half-sample row, U[2a + 1][n] = g((a - n) mod N).
-// With a quantized output mode (rf = 2, N <= 96) the kernel also writes the
+// With a quantized output mode (rf = 2, N <= 512) the kernel also writes the
 // mixed-contraction operand: role 0 the chip rows q[pair][M][cs] (16 zero
 // columns on the left), role 1 the window q[pair][M][M]; and its share of
-// the pair's MixedStats.  amp may be null for role 0 then.
+// the pair's MixedStats.  amp may be null then (role 1 too, when the FP16
+// fragments are given and the window sums come from the quantized target).
 bool upsample_fuses_quantization(int N, int rf);
 // qlo (optional): second quantized plane holding the rounding remainder of q
 // (the split hi/lo operands), same layout as q.
@@ -84,11 +88,19 @@ bool upsample_fuses_quantization(int N, int rf);
 // FP16 MFMA fragments (upsample_fragments); the products then run on FP16
 // matrix cores instead of FP32 ones
 int upsample_fragment_count(int N);
+// tiles with 96 < N <= 512 (rf = 2) upsample in two banded passes through
+// global scratch: T [pairs][N][N], band and row statistics [pairs][ceil(N / 16)]
+struct UpsampleScratch {
+    float2 *T;
+    double2 *band, *rows;
+};
+bool upsample_needs_scratch(int N, int rf);
 hipError_t launch_upsample(hipStream_t s, const PairDesc *desc, int pairs, int dtype, int role,
                            const int2 *window, int N, int rf, const float *U, const float *g,
                            float *amp, int qmode = kUpFP32, uint16_t *q = nullptr, int cs = 0,
                            MixedStats *stats = nullptr, uint16_t *qlo = nullptr,
-                           const uint16_t *hfrag = nullptr);
+                           const uint16_t *hfrag = nullptr,
+                           const UpsampleScratch *scratch = nullptr);
 // mixed coarse stage in one pass when the tiles fit in LDS: gather, detect,
 // write the FP32 window (for the window sums) and both quantized operands
 bool gather_prep_fits(int C, int W);

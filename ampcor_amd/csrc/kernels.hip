@@ -349,44 +349,73 @@ __global__ __launch_bounds__(256) void k_normalize(const float *__restrict__ par
 // slide the vertical box sums V(u, x) = sum_{i<C} t(u+i, x) down in FP64;
 // the block's rows of V go to LDS and become row prefix sums (16 threads a
 // row, a 16-lane scan of the chunk totals), and each window sum is the
-// difference of two prefix entries.  The numerator partials are summed as in
-// the separate normalizers (pairwise tree for the FP32 form, in order for the
-// mixed form), and the sums never leave the chip.
+// difference of two prefix entries.  The numerator partials (chip-row or
+// chip-column ranges) are summed in order, and the sums never leave the chip.
+//
+// Forms: kFuseFP32 (FP32 amplitudes; chip stats (sum r', |r'|)), kFuseMixed
+// (FP32 amplitudes; the mixed normalizer with the operand scales), and
+// kFuseBF16 / kFuseFP16, which take the window sums from the quantized
+// target itself.  With every sum in quantized units the scales and the
+// removed constants cancel, and gamma is the normalized correlation of
+// exactly the operands the contraction multiplied:
+//   gamma = (num - sum R sum T / n) /
+//           sqrt((sum R^2 - (sum R)^2 / n) (sum T^2 - (sum T)^2 / n)).
 // ---------------------------------------------------------------------------
 constexpr int kFuseRows = 16;
 constexpr int kFuseThreads = 256;
 constexpr int kFuseMaxCols = 2;  // target columns per thread: W <= 512
 constexpr int kFuseOut = 9;      // outputs per thread and block: P <= 144
 
-template <bool MIXED>
+template <int FORM>
+__device__ inline float fuse_load(const void *t, size_t k)
+{
+    if constexpr (FORM == kFuseBF16)
+        return __uint_as_float((uint32_t) static_cast<const uint16_t *>(t)[k] << 16);
+    else if constexpr (FORM == kFuseFP16)
+        return (float)__builtin_bit_cast(_Float16, static_cast<const uint16_t *>(t)[k]);
+    else
+        return static_cast<const float *>(t)[k];
+}
+
+template <int FORM>
 __global__ __launch_bounds__(kFuseThreads) void k_normalize_fused(
-    const float *__restrict__ amp, int W, int C, const float *__restrict__ partial, int nsplit,
+    const void *__restrict__ target, int W, int C, const float *__restrict__ partial, int nsplit,
     const double2 *__restrict__ stats, const MixedStats *__restrict__ mstats,
-    float *__restrict__ gamma)
+    float *__restrict__ gamma, int span)
 {
     extern __shared__ double fl[];   // [2][kFuseRows][W + 1]
     const int pair = blockIdx.x, t = threadIdx.x;
+    // this workgroup's placement rows: [ub, ue), `span` blocks of kFuseRows
+    const int ub = blockIdx.y * span * kFuseRows;
     const int P = W - C + 1, ws = W + 1;
     double *p1 = fl, *p2 = fl + kFuseRows * ws;
-    const float *tp = amp + (size_t)pair * W * W;
+    const size_t tbase = (size_t)pair * W * W;
+    auto tval = [&](int row, int x) { return fuse_load<FORM>(target, tbase + (size_t)row * W + x); };
     const double n = (double)C * C;
-    // per-pair constants of the normalizer: the chip sum and the squared chip
-    // norm (centred), and for the mixed form the operand scales and the
-    // window constant
+    // per-pair constants: the chip sum and the squared centred chip norm in the
+    // units of the target sums, and for the FP32-amplitude mixed form the
+    // operand scales and the window constant
     double sumr, chip2, inv_ab = 1.0, mean_t = 0.0;
-    if constexpr (MIXED) {
-        const MixedStats st = mstats[pair];
-        sumr = st.sumr;
-        chip2 = st.sumr2 - st.sumr * st.sumr / n;
-        inv_ab = 1.0 / (st.scale_r * st.scale_t);
-        mean_t = st.mean_t;
-    } else {
+    if constexpr (FORM == kFuseFP32) {
         const double2 st = stats[pair];
         sumr = st.x;
         chip2 = st.y * st.y;
+    } else {
+        const MixedStats st = mstats[pair];
+        if constexpr (FORM == kFuseMixed) {
+            sumr = st.sumr;
+            chip2 = st.sumr2 - st.sumr * st.sumr / n;
+            inv_ab = 1.0 / (st.scale_r * st.scale_t);
+            mean_t = st.mean_t;
+        } else {
+            sumr = st.sumr * st.scale_r;
+            chip2 = st.sumr2 * st.scale_r * st.scale_r - sumr * sumr / n;
+        }
     }
 
-    // vertical box sums at u = 0
+    const int P_ = W - C + 1;
+    const int ue = min(P_, ub + span * kFuseRows);
+    // vertical box sums at u = ub, summed directly
     double v1[kFuseMaxCols], v2[kFuseMaxCols];
 #pragma unroll
     for (int q = 0; q < kFuseMaxCols; ++q) {
@@ -395,17 +424,18 @@ __global__ __launch_bounds__(kFuseThreads) void k_normalize_fused(
         if (x < W)
 #pragma unroll 16
             for (int i = 0; i < C; ++i) {
-                const double d = (double)tp[(size_t)i * W + x];
+                const double d = (double)tval(ub + i, x);
                 v1[q] = v1[q] + d;
                 v2[q] = v2[q] + d * d;
             }
     }
     const int r = t >> 4, sl = t & 15;      // row of the block, slot in the row
-    const int cw = (W + 15) / 16;            // prefix chunk per slot
-    const int x0 = sl * cw, x1 = min(W, x0 + cw);
+    // prefix chunk per slot, odd: the 16 slots of a row start in distinct LDS banks
+    const int cw = ((W + 15) / 16) | 1;
+    const int x0 = min(W, sl * cw), x1 = min(W, x0 + cw);
 
-    for (int u0 = 0; u0 < P; u0 += kFuseRows) {
-        const int rows = min(kFuseRows, P - u0);
+    for (int u0 = ub; u0 < ue; u0 += kFuseRows) {
+        const int rows = min(kFuseRows, ue - u0);
         // the numerators of this thread's outputs (row r, columns sl + 16 j),
         // loaded first so that their latency overlaps the box sums below
         float num[kFuseOut];
@@ -428,15 +458,15 @@ __global__ __launch_bounds__(kFuseThreads) void k_normalize_fused(
             for (int k = 0; k < kFuseRows; ++k) {
                 const int u = u0 + k;
                 a[k] = b[k] = 0.0f;
-                if (k < rows && u > 0) {
-                    a[k] = tp[(size_t)(u + C - 1) * W + x];
-                    b[k] = tp[(size_t)(u - 1) * W + x];
+                if (k < rows && u > ub) {
+                    a[k] = tval(u + C - 1, x);
+                    b[k] = tval(u - 1, x);
                 }
             }
 #pragma unroll
             for (int k = 0; k < kFuseRows; ++k) {
                 if (k >= rows) break;
-                if (u0 + k > 0) {
+                if (u0 + k > ub) {
                     const double da = a[k], db = b[k];
                     v1[q] = (v1[q] + da) - db;
                     v2[q] = (v2[q] + da * da) - db * db;
@@ -447,13 +477,25 @@ __global__ __launch_bounds__(kFuseThreads) void k_normalize_fused(
         }
         __syncthreads();
         // row prefix sums: chunk prefix, then the 16-lane scan of chunk totals
+        // (eight entries in flight per step)
         double run1 = 0.0, run2 = 0.0;
         if (r < rows)
-            for (int x = x0; x < x1; ++x) {
-                run1 += p1[r * ws + x];
-                run2 += p2[r * ws + x];
-                p1[r * ws + x] = run1;
-                p2[r * ws + x] = run2;
+            for (int xb = x0; xb < x1; xb += 8) {
+                double e1[8], e2[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    e1[i] = xb + i < x1 ? p1[r * ws + xb + i] : 0.0;
+                    e2[i] = xb + i < x1 ? p2[r * ws + xb + i] : 0.0;
+                }
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    run1 += e1[i];
+                    run2 += e2[i];
+                    if (xb + i < x1) {
+                        p1[r * ws + xb + i] = run1;
+                        p2[r * ws + xb + i] = run2;
+                    }
+                }
             }
         double o1 = run1, o2 = run2;
 #pragma unroll
@@ -467,9 +509,19 @@ __global__ __launch_bounds__(kFuseThreads) void k_normalize_fused(
         o1 -= run1;   // exclusive: the totals of the chunks to the left
         o2 -= run2;
         if (r < rows && sl > 0)
-            for (int x = x0; x < x1; ++x) {
-                p1[r * ws + x] += o1;
-                p2[r * ws + x] += o2;
+            for (int xb = x0; xb < x1; xb += 8) {
+                double e1[8], e2[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    e1[i] = xb + i < x1 ? p1[r * ws + xb + i] : 0.0;
+                    e2[i] = xb + i < x1 ? p2[r * ws + xb + i] : 0.0;
+                }
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+                    if (xb + i < x1) {
+                        p1[r * ws + xb + i] = e1[i] + o1;
+                        p2[r * ws + xb + i] = e2[i] + o2;
+                    }
             }
         __syncthreads();
         if (r < rows) {
@@ -484,8 +536,9 @@ __global__ __launch_bounds__(kFuseThreads) void k_normalize_fused(
                 // must not flush to zero
                 const double mu = S1 / n;
                 const double var = S2 - mu * S1;
-                const double num_c = MIXED ? (double)num[j] * inv_ab + (mean_t - mu) * sumr
-                                           : (double)num[j] - mu * sumr;
+                const double num_c = FORM == kFuseMixed
+                                         ? (double)num[j] * inv_ab + (mean_t - mu) * sumr
+                                         : (double)num[j] - mu * sumr;
                 float g = __builtin_nanf("");
                 if (var > 0.0 && chip2 > 0.0) g = (float)(num_c / sqrt(chip2 * var));
                 gamma[(size_t)pair * P * P + (size_t)u * P + v] = g;
@@ -1114,11 +1167,10 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2h(
     }
     __syncthreads();
 
-    float *out = amp ? amp + (size_t)pair * M * M : nullptr;
+    float *out = (role == 1 && amp) ? amp + (size_t)pair * M * M : nullptr;
     double s1 = 0.0, s2 = 0.0;
     auto emit2 = [&](int p, int q, float v0, float v1) {
-        if (role == 1)
-            *reinterpret_cast<float2 *>(out + (size_t)p * M + q) = make_float2(v0, v1);
+        if (out) *reinterpret_cast<float2 *>(out + (size_t)p * M + q) = make_float2(v0, v1);
         const uint16_t h0 = quantize(QMODE, (v0 - qc) * qs), h1 = quantize(QMODE, (v1 - qc) * qs);
         const size_t at = role == 0 ? ((size_t)pair * M + p) * cs + 16 + q
                                     : ((size_t)pair * M + p) * M + q;
@@ -1179,12 +1231,290 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2h(
 }
 
 // ---------------------------------------------------------------------------
+// x2 upsampling of larger tiles (96 < N <= kUpBandMaxN) in two passes over
+// bands of 16, on FP32 matrix cores, with the circulant H[a][n] = g((a - n)
+// mod N) read from a periodic copy of g in LDS (gx[Np + a - n]):
+//   pass 1 (column bands): T = H x for 16 columns, to global memory, and the
+//          band's amplitude sum and peak;
+//   pass 2 (row bands): Y_s = T_s H^T for 16 rows of x and of T, and the four
+//          phases of output rows 2a, 2a + 1, written whole -- FP32 amplitudes
+//          and, in the quantized modes, the contraction's operand with the
+//          constants of all column bands (summed in band order) and the
+//          band's share of the chip statistics;
+//   k_up2_finish sums the chip statistics of the row bands in order.
+// Small LDS footprints (two 16-wide strips) keep many workgroups per CU.
+// ---------------------------------------------------------------------------
+constexpr int kUpBandThreads = 256;
+constexpr int kUpBandMaxN = 512;
+
+template <int DTYPE>
+__device__ inline void load_sample(const void *base, size_t off, float &re, float &im)
+{
+    if (DTYPE == 0) {
+        const float2 z = static_cast<const float2 *>(base)[off];
+        re = z.x;
+        im = z.y;
+    } else {
+        re = static_cast<const float *>(base)[off];
+        im = 0.0f;
+    }
+}
+
+__device__ inline void tile_origin(const PairDesc &d, int role, const int2 *window, int pair,
+                                   const void *&base, size_t &stride, int &line, int &sample)
+{
+    base = role == 0 ? d.ref : d.tgt;
+    stride = role == 0 ? d.ref_stride : d.tgt_stride;
+    line = role == 0 ? d.ref_line : d.tgt_line;
+    sample = role == 0 ? d.ref_sample : d.tgt_sample;
+    if (role == 1) {
+        const int2 tl = window[pair];
+        line += tl.x;
+        sample += tl.y;
+    }
+}
+
+template <int DTYPE>
+__global__ __launch_bounds__(kUpBandThreads) void k_up2_cols(
+    const PairDesc *__restrict__ desc, int role, const int2 *__restrict__ window, int N, int Np,
+    const float *__restrict__ g, float2 *__restrict__ T, double2 *__restrict__ band)
+{
+    extern __shared__ float ub[];
+    float *xr = ub, *xi = xr + Np * 16, *gx = xi + Np * 16;   // the band [Np][16], g
+    __shared__ double red[kUpBandThreads / kWave];
+    __shared__ float redm[kUpBandThreads / kWave];
+    const int pair = blockIdx.x, cb = blockIdx.y, nb = gridDim.y, c0 = 16 * cb;
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6;
+    const int lm = lane & 15, lk = lane >> 4;
+    const PairDesc d = desc[pair];
+    const void *base;
+    size_t stride;
+    int line, sample;
+    tile_origin(d, role, window, pair, base, stride, line, sample);
+    double asum = 0.0;
+    float amax = 0.0f;
+    const int total = Np * 16;
+    for (int k0 = threadIdx.x; k0 < total; k0 += kUp2Batch * blockDim.x) {
+        float re[kUp2Batch], im[kUp2Batch];
+#pragma unroll
+        for (int j = 0; j < kUp2Batch; ++j) {
+            const int k = k0 + j * blockDim.x;
+            const int r = k >> 4, c = c0 + (k & 15);
+            re[j] = im[j] = 0.0f;
+            if (k < total && r < N && c < N)
+                load_sample<DTYPE>(base, (size_t)(line + r) * stride + sample + c, re[j], im[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < kUp2Batch; ++j) {
+            const int k = k0 + j * blockDim.x;
+            if (k < total) {
+                xr[k] = re[j];
+                xi[k] = im[j];
+                const float a = amplitude(re[j], im[j]);   // zero outside the tile
+                asum += a;
+                amax = fmaxf(amax, a);
+            }
+        }
+    }
+    for (int k = threadIdx.x; k < 2 * Np; k += blockDim.x) {
+        const int t = (k - Np) % N;
+        gx[k] = g[t < 0 ? t + N : t];
+    }
+    block_sum_max(red, redm, asum, amax);
+    if (threadIdx.x == 0) band[(size_t)pair * nb + cb] = make_double2(asum, (double)amax);
+    __syncthreads();
+    // T[a][c0 + n] for the row tiles of this wave
+    const int tiles = Np / 16;
+    for (int mt = wave; mt < tiles; mt += kUpBandThreads / kWave) {
+        floatx4 aR = {0.0f, 0.0f, 0.0f, 0.0f}, aI = aR;
+        const float *hrow = gx + (16 * mt + lm - lk + Np);
+        for (int k16 = 0; k16 < Np; k16 += 16)
+#pragma unroll
+            for (int k0 = k16; k0 < k16 + 16; k0 += 4) {
+                const float a = hrow[-k0];
+                const int bi = (k0 + lk) * 16 + lm;
+                aR = __builtin_amdgcn_mfma_f32_16x16x4f32(a, xr[bi], aR, 0, 0, 0);
+                if (DTYPE == 0) aI = __builtin_amdgcn_mfma_f32_16x16x4f32(a, xi[bi], aI, 0, 0, 0);
+            }
+        const int c = c0 + lm;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int a = 16 * mt + 4 * lk + i;
+            if (a < N && c < N) T[((size_t)pair * N + a) * N + c] = make_float2(aR[i], aI[i]);
+        }
+    }
+}
+
+template <int DTYPE, int QMODE>
+__global__ __launch_bounds__(kUpBandThreads) void k_up2_rows(
+    const PairDesc *__restrict__ desc, int role, const int2 *__restrict__ window, int N, int Np,
+    const float *__restrict__ g, const float2 *__restrict__ T, const double2 *__restrict__ band,
+    int nb, float *__restrict__ amp, uint16_t *__restrict__ qout, uint16_t *__restrict__ qlo,
+    int cs, double2 *__restrict__ rowstats, MixedStats *__restrict__ stats)
+{
+    extern __shared__ float ub[];
+    const int S = Np + 1;   // odd row stride: the 16 rows of an A fragment hit distinct banks
+    float *xr = ub, *xi = xr + 16 * S, *tr = xi + 16 * S, *ti = tr + 16 * S, *gx = ti + 16 * S;
+    __shared__ double red[kUpBandThreads / kWave], red2[kUpBandThreads / kWave];
+    __shared__ float redm[kUpBandThreads / kWave];
+    const int pair = blockIdx.x, rb = blockIdx.y, a0 = 16 * rb;
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6;
+    const int lm = lane & 15, lk = lane >> 4;
+    const PairDesc d = desc[pair];
+    const void *base;
+    size_t stride;
+    int line, sample;
+    tile_origin(d, role, window, pair, base, stride, line, sample);
+    const int total = 16 * Np;
+    for (int k0 = threadIdx.x; k0 < total; k0 += kUp2Batch * blockDim.x) {
+        float re[kUp2Batch], im[kUp2Batch];
+        float2 tv[kUp2Batch];
+#pragma unroll
+        for (int j = 0; j < kUp2Batch; ++j) {
+            const int k = k0 + j * blockDim.x;
+            const int r = k / Np, c = k - r * Np, a = a0 + r;
+            re[j] = im[j] = 0.0f;
+            tv[j] = make_float2(0.0f, 0.0f);
+            if (k < total && a < N && c < N) {
+                load_sample<DTYPE>(base, (size_t)(line + a) * stride + sample + c, re[j], im[j]);
+                tv[j] = T[((size_t)pair * N + a) * N + c];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kUp2Batch; ++j) {
+            const int k = k0 + j * blockDim.x;
+            if (k < total) {
+                const int r = k / Np, c = k - r * Np;
+                xr[r * S + c] = re[j];
+                xi[r * S + c] = im[j];
+                tr[r * S + c] = tv[j].x;
+                ti[r * S + c] = tv[j].y;
+            }
+        }
+    }
+    for (int k = threadIdx.x; k < 2 * Np; k += blockDim.x) {
+        const int t = (k - Np) % N;
+        gx[k] = g[t < 0 ? t + N : t];
+    }
+    const int M = 2 * N;
+    float qc = 0.0f, qs = 1.0f;
+    if (QMODE != kUpFP32) {
+        // the quantization constants of the whole coarse tile, from the column bands
+        double asum = 0.0;
+        float amax = 0.0f;
+        for (int b = 0; b < nb; ++b) {
+            const double2 v = band[(size_t)pair * nb + b];
+            asum += v.x;
+            amax = fmaxf(amax, (float)v.y);
+        }
+        qc = (float)(asum / ((double)N * N));
+        qs = (float)quant_scale(QMODE, 2.0f * amax);
+        if (role == 0) {
+            // zero padding of this band's chip rows
+            const int rows = 2 * min(16, N - a0), pad = cs - M;
+            for (int k = threadIdx.x; k < rows * pad; k += blockDim.x) {
+                const int p = 2 * a0 + k / pad, c = k % pad;
+                const int col = c < 16 ? c : M + c;
+                qout[((size_t)pair * M + p) * cs + col] = 0;
+                if (qlo) qlo[((size_t)pair * M + p) * cs + col] = 0;
+            }
+        }
+    }
+    __syncthreads();
+    float *out = amp ? amp + (size_t)pair * M * M : nullptr;
+    double s1 = 0.0, s2 = 0.0;
+    auto emit2 = [&](int p, int q, float v0, float v1) {
+        if (out) *reinterpret_cast<float2 *>(out + (size_t)p * M + q) = make_float2(v0, v1);
+        if (QMODE != kUpFP32) {
+            const float x0 = (v0 - qc) * qs, x1 = (v1 - qc) * qs;
+            const uint16_t h0 = quantize(QMODE, x0), h1 = quantize(QMODE, x1);
+            uint16_t l0 = 0, l1 = 0;
+            if (qlo) {
+                l0 = quantize(QMODE, x0 - dequantize(QMODE, h0));
+                l1 = quantize(QMODE, x1 - dequantize(QMODE, h1));
+            }
+            const size_t at = role == 0 ? ((size_t)pair * M + p) * cs + 16 + q
+                                        : ((size_t)pair * M + p) * M + q;
+            *reinterpret_cast<uint32_t *>(qout + at) = (uint32_t)h0 | ((uint32_t)h1 << 16);
+            if (qlo) *reinterpret_cast<uint32_t *>(qlo + at) = (uint32_t)l0 | ((uint32_t)l1 << 16);
+            if (role == 0) {
+                const double b0 = ((double)dequantize(QMODE, h0) +
+                                   (qlo ? (double)dequantize(QMODE, l0) : 0.0)) / qs;
+                const double b1 = ((double)dequantize(QMODE, h1) +
+                                   (qlo ? (double)dequantize(QMODE, l1) : 0.0)) / qs;
+                s1 += b0 + b1;
+                s2 += b0 * b0 + b1 * b1;
+            }
+        }
+    };
+    const int tiles = Np / 16;
+    for (int bt = wave; bt < tiles; bt += kUpBandThreads / kWave) {
+        const int n = 16 * bt + lm;
+        const float *hrow = gx + (n - lk + Np);
+        const int arow = lm * S + lk;
+        floatx4 xR = {0.0f, 0.0f, 0.0f, 0.0f}, xI = xR, tR = xR, tI = xR;
+        for (int k16 = 0; k16 < Np; k16 += 16)
+#pragma unroll
+            for (int k0 = k16; k0 < k16 + 16; k0 += 4) {
+                const float b = hrow[-k0];
+                const int ai = arow + k0;
+                xR = __builtin_amdgcn_mfma_f32_16x16x4f32(xr[ai], b, xR, 0, 0, 0);
+                if (DTYPE == 0) xI = __builtin_amdgcn_mfma_f32_16x16x4f32(xi[ai], b, xI, 0, 0, 0);
+                tR = __builtin_amdgcn_mfma_f32_16x16x4f32(tr[ai], b, tR, 0, 0, 0);
+                if (DTYPE == 0) tI = __builtin_amdgcn_mfma_f32_16x16x4f32(ti[ai], b, tI, 0, 0, 0);
+            }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int r = 4 * lk + i, a = a0 + r;
+            if (a < N && n < N) {
+                const int at = r * S + n;
+                emit2(2 * a, 2 * n, amplitude(xr[at], xi[at]), amplitude(xR[i], xI[i]));
+                emit2(2 * a + 1, 2 * n, amplitude(tr[at], ti[at]), amplitude(tR[i], tI[i]));
+            }
+        }
+    }
+    if (QMODE != kUpFP32) {
+        if (role == 0) {
+            float unused = 0.0f;
+            block_sum_max(red, redm, s1, unused);
+            block_sum_max(red2, redm, s2, unused);
+            if (threadIdx.x == 0) rowstats[(size_t)pair * gridDim.y + rb] = make_double2(s1, s2);
+        } else if (rb == 0 && threadIdx.x == 0) {
+            stats[pair].scale_t = qs;
+            stats[pair].mean_t = qc;
+        }
+    }
+}
+
+// the chip statistics of a pair from its row bands, in band order, and the
+// chip's quantization scale (recomputed from the column bands)
+template <int QMODE>
+__global__ void k_up2_finish(int pairs, int N, const double2 *__restrict__ band, int nb,
+                             const double2 *__restrict__ rowstats, int nr,
+                             MixedStats *__restrict__ stats)
+{
+    const int pair = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pair >= pairs) return;
+    float amax = 0.0f;
+    for (int b = 0; b < nb; ++b) amax = fmaxf(amax, (float)band[(size_t)pair * nb + b].y);
+    double s1 = 0.0, s2 = 0.0;
+    for (int b = 0; b < nr; ++b) {
+        const double2 v = rowstats[(size_t)pair * nr + b];
+        s1 += v.x;
+        s2 += v.y;
+    }
+    stats[pair].sumr = s1;
+    stats[pair].sumr2 = s2;
+    stats[pair].scale_r = (float)quant_scale(QMODE, 2.0f * amax);
+}
+
+// ---------------------------------------------------------------------------
 // mixed coarse stage in one pass: one workgroup per pair gathers and detects
 // the chip into LDS (its exact mean is removed before rounding), then streams
-// the window: FP32 amplitudes for the exact window sums, and the quantized
-// operand with the chip's mean amplitude removed -- any constant works there,
-// the normalizer takes it back exactly, and the chip's mean is close to the
-// window's for the same scene
+// the window: the quantized operand with the chip's mean amplitude removed --
+// any constant works there, the normalizer takes it back exactly, and the
+// chip's mean is close to the window's for the same scene -- and, when
+// tgt_amp is given, the FP32 amplitudes
 // ---------------------------------------------------------------------------
 constexpr int kGatherPrepThreads = 256;
 constexpr size_t kGatherPrepLds = 64 * 1024;
@@ -1217,12 +1547,26 @@ __global__ __launch_bounds__(kGatherPrepThreads) void k_gather_prep_mixed(
         __syncthreads();
     };
     double rs = 0.0, rm = 0.0;
-    for (int k = t; k < C * C; k += kGatherPrepThreads) {
-        const int i = k / C, j = k - i * C;
-        const float v = detect(DTYPE, d.ref, (size_t)(d.ref_line + i) * d.ref_stride + d.ref_sample + j);
-        ra[k] = v;
-        rs += v;
-        rm = fmax(rm, (double)v);
+    constexpr int kB = 8;   // global loads in flight per thread
+    for (int k0 = t; k0 < C * C; k0 += kB * kGatherPrepThreads) {
+        float v[kB];
+#pragma unroll
+        for (int q = 0; q < kB; ++q) {
+            const int k = k0 + q * kGatherPrepThreads;
+            const int i = k / C, j = k - i * C;
+            v[q] = k < C * C ? detect(DTYPE, d.ref, (size_t)(d.ref_line + i) * d.ref_stride +
+                                                        d.ref_sample + j)
+                             : 0.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < kB; ++q) {
+            const int k = k0 + q * kGatherPrepThreads;
+            if (k < C * C) {
+                ra[k] = v[q];
+                rs += v[q];
+                rm = fmax(rm, (double)v[q]);
+            }
+        }
     }
     reduce(rs, rm, true);
     const float cr = (float)(rs / ((double)C * C));
@@ -1242,13 +1586,26 @@ __global__ __launch_bounds__(kGatherPrepThreads) void k_gather_prep_mixed(
         }
         cq[k] = h;
     }
-    float *ga = tgt_amp + (size_t)pair * W * W;
+    float *ga = tgt_amp ? tgt_amp + (size_t)pair * W * W : nullptr;
     uint16_t *tq = tgt_q + (size_t)pair * W * W;
-    for (int k = t; k < W * W; k += kGatherPrepThreads) {
-        const int i = k / W, j = k - i * W;
-        const float v = detect(DTYPE, d.tgt, (size_t)(d.tgt_line + i) * d.tgt_stride + d.tgt_sample + j);
-        ga[k] = v;
-        tq[k] = quantize(QMODE, (v - cr) * at);
+    for (int k0 = t; k0 < W * W; k0 += kB * kGatherPrepThreads) {
+        float v[kB];
+#pragma unroll
+        for (int q = 0; q < kB; ++q) {
+            const int k = k0 + q * kGatherPrepThreads;
+            const int i = k / W, j = k - i * W;
+            v[q] = k < W * W ? detect(DTYPE, d.tgt, (size_t)(d.tgt_line + i) * d.tgt_stride +
+                                                        d.tgt_sample + j)
+                             : 0.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < kB; ++q) {
+            const int k = k0 + q * kGatherPrepThreads;
+            if (k < W * W) {
+                if (ga) ga[k] = v[q];
+                tq[k] = quantize(QMODE, (v[q] - cr) * at);
+            }
+        }
     }
     reduce(s1, s2, false);
     if (t == 0) {
@@ -1495,18 +1852,27 @@ bool normalize_fuses(int W, int C)
     return W <= kFuseThreads * kFuseMaxCols && W - C + 1 <= 16 * kFuseOut;
 }
 
-hipError_t launch_normalize_fused(hipStream_t s, const float *amp, int pairs, int W, int C,
-                                  const float *partial, int nsplit, const double2 *stats,
+hipError_t launch_normalize_fused(hipStream_t s, int form, const void *target, int pairs, int W,
+                                  int C, const float *partial, int nsplit, const double2 *stats,
                                   const MixedStats *mstats, float *gamma)
 {
-    if (!normalize_fuses(W, C) || (!stats && !mstats)) return hipErrorInvalidValue;
+    if (!normalize_fuses(W, C) || (form == kFuseFP32 ? !stats : !mstats))
+        return hipErrorInvalidValue;
     const size_t lds = 2 * (size_t)kFuseRows * (W + 1) * sizeof(double);
-    if (mstats)
-        hipLaunchKernelGGL(k_normalize_fused<true>, dim3(pairs), dim3(kFuseThreads), lds, s, amp, W,
-                           C, partial, nsplit, stats, mstats, gamma);
-    else
-        hipLaunchKernelGGL(k_normalize_fused<false>, dim3(pairs), dim3(kFuseThreads), lds, s, amp,
-                           W, C, partial, nsplit, stats, mstats, gamma);
+    const int P = W - C + 1, nblocks = (P + kFuseRows - 1) / kFuseRows;
+    const int span = nblocks;   // one workgroup per pair: the box sums slide down all rows
+    const dim3 grid(pairs, (nblocks + span - 1) / span);
+#define AMPCOR_FUSE(F)                                                                              \
+    hipLaunchKernelGGL(k_normalize_fused<F>, grid, dim3(kFuseThreads), lds, s, target, W, C,        \
+                       partial, nsplit, stats, mstats, gamma, span)
+    switch (form) {
+    case kFuseFP32: AMPCOR_FUSE(kFuseFP32); break;
+    case kFuseMixed: AMPCOR_FUSE(kFuseMixed); break;
+    case kFuseBF16: AMPCOR_FUSE(kFuseBF16); break;
+    case kFuseFP16: AMPCOR_FUSE(kFuseFP16); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef AMPCOR_FUSE
     return hipGetLastError();
 }
 
@@ -1519,15 +1885,55 @@ hipError_t launch_coarse_peak(hipStream_t s, const float *gamma, int pairs, int 
     return hipGetLastError();
 }
 
-bool upsample_fuses_quantization(int N, int rf) { return rf == 2 && (N + 15) / 16 * 16 <= kUp2MaxN; }
+bool upsample_fuses_quantization(int N, int rf) { return rf == 2 && N <= kUpBandMaxN; }
+
+bool upsample_needs_scratch(int N, int rf)
+{
+    return rf == 2 && (N + 15) / 16 * 16 > kUp2MaxN && N <= kUpBandMaxN;
+}
 
 int upsample_fragment_count(int N) { return ((N + 31) / 32 * 32 / 16) * ((N + 31) / 32) * 64 * 8; }
 
 hipError_t launch_upsample(hipStream_t s, const PairDesc *desc, int pairs, int dtype, int role,
                            const int2 *window, int N, int rf, const float *U, const float *g,
                            float *amp, int qmode, uint16_t *q, int cs, MixedStats *stats,
-                           uint16_t *qlo, const uint16_t *hfrag)
+                           uint16_t *qlo, const uint16_t *hfrag, const UpsampleScratch *scratch)
 {
+    if (upsample_needs_scratch(N, rf) && g && scratch && scratch->T) {
+        const int Np = (N + 15) / 16 * 16, nb = Np / 16;
+        const dim3 grid(pairs, nb), block(kUpBandThreads);
+        const size_t lds1 = (2 * (size_t)Np * 16 + 2 * Np) * sizeof(float);
+        const size_t lds2 = (4 * (size_t)16 * (Np + 1) + 2 * Np) * sizeof(float);
+        if (dtype == 0)
+            hipLaunchKernelGGL(k_up2_cols<0>, grid, block, lds1, s, desc, role, window, N, Np, g,
+                               scratch->T, scratch->band);
+        else
+            hipLaunchKernelGGL(k_up2_cols<1>, grid, block, lds1, s, desc, role, window, N, Np, g,
+                               scratch->T, scratch->band);
+#define AMPCOR_ROWS(DT, QM)                                                                        \
+    hipLaunchKernelGGL((k_up2_rows<DT, QM>), grid, block, lds2, s, desc, role, window, N, Np, g,    \
+                       scratch->T, scratch->band, nb, amp, q, qlo, cs, scratch->rows, stats)
+        if (dtype == 0) {
+            if (qmode == kUpBF16) AMPCOR_ROWS(0, kUpBF16);
+            else if (qmode == kUpFP16) AMPCOR_ROWS(0, kUpFP16);
+            else AMPCOR_ROWS(0, kUpFP32);
+        } else {
+            if (qmode == kUpBF16) AMPCOR_ROWS(1, kUpBF16);
+            else if (qmode == kUpFP16) AMPCOR_ROWS(1, kUpFP16);
+            else AMPCOR_ROWS(1, kUpFP32);
+        }
+#undef AMPCOR_ROWS
+        if (role == 0 && qmode != kUpFP32) {
+            const dim3 fg((pairs + 255) / 256), fb(256);
+            if (qmode == kUpFP16)
+                hipLaunchKernelGGL(k_up2_finish<kUpFP16>, fg, fb, 0, s, pairs, N, scratch->band, nb,
+                                   scratch->rows, nb, stats);
+            else
+                hipLaunchKernelGGL(k_up2_finish<kUpBF16>, fg, fb, 0, s, pairs, N, scratch->band, nb,
+                                   scratch->rows, nb, stats);
+        }
+        return hipGetLastError();
+    }
     const int Np = (N + 15) / 16 * 16;
     const int Np32 = (N + 31) / 32 * 32;
     if (rf == 2 && hfrag && qmode != kUpFP32 && !qlo && Np32 <= kUp2hMaxN) {

@@ -380,10 +380,28 @@ int ampcor_hip_plan_create(int device, size_t pairs, size_t ref_lines, size_t re
     }
     if (int rc = upload_upsampler(p->U_ref, p->g_ref, p->C, p->rf)) return cleanup(rc);
     if (int rc = upload_upsampler(p->U_tgt, p->g_tgt, p->E, p->rf)) return cleanup(rc);
-    // the BF16/FP16 forms upsample on FP16 matrix cores (the split forms keep FP32)
+    // the BF16/FP16 forms upsample tiles up to 96 samples on FP16 matrix cores
+    // (the split forms keep FP32)
     if (p->mixed && !p->split && p->rf == 2) {
-        if (int rc = upload_fragments(p->hfrag_ref, p->C)) return cleanup(rc);
-        if (int rc = upload_fragments(p->hfrag_tgt, p->E)) return cleanup(rc);
+        if (p->C <= 96)
+            if (int rc = upload_fragments(p->hfrag_ref, p->C)) return cleanup(rc);
+        if (p->E <= 96)
+            if (int rc = upload_fragments(p->hfrag_tgt, p->E)) return cleanup(rc);
+    }
+    // larger tiles upsample in two banded passes through scratch, shared by
+    // the two roles (they run one after the other on the plan stream)
+    {
+        int nmax = 0;
+        if (upsample_needs_scratch(p->C, p->rf)) nmax = std::max(nmax, p->C);
+        if (upsample_needs_scratch(p->E, p->rf)) nmax = std::max(nmax, p->E);
+        if (nmax) {
+            const size_t nb = (nmax + 15) / 16;
+            hipError_t ea = p->up_T.alloc(N * (size_t)nmax * nmax);
+            if (ea == hipSuccess) ea = p->up_band.alloc(N * nb);
+            if (ea == hipSuccess) ea = p->up_rows.alloc(N * nb);
+            if (ea != hipSuccess) return cleanup(hip_fail(ea, "allocating the upsampling scratch"));
+            p->up_scratch = UpsampleScratch{p->up_T.ptr, p->up_band.ptr, p->up_rows.ptr};
+        }
     }
     {
         std::vector<double> d(2 * p->Z + 1);
@@ -503,9 +521,10 @@ int plan_enqueue(ampcor_hip_plan *plan, int N)
     const bool fp16 = p->fp16;
     const int qmode = fp16 ? kUpFP16 : kUpBF16;
     if (p->mixed && gather_prep_fits(p->C, p->W)) {
+        // the fused normalizer takes the window sums from the quantized target
         STEP(launch_gather_prep_mixed(s, p->d_desc.ptr, N, p->dtype, p->C, p->W, p->mixed_coarse.cs,
-                                      fp16, p->amp_tgt.ptr, p->chip_q.ptr, p->tgt_q.ptr,
-                                      p->mstats.ptr),
+                                      fp16, normalize_fuses(p->W, p->C) ? nullptr : p->amp_tgt.ptr,
+                                      p->chip_q.ptr, p->tgt_q.ptr, p->mstats.ptr),
              "gather_prep");
     } else {
         STEP(launch_gather_detect(s, p->d_desc.ptr, N, p->dtype, p->C, p->W, p->amp_ref.ptr,
@@ -531,9 +550,9 @@ int plan_enqueue(ampcor_hip_plan *plan, int N)
                                     fp16, p->partial.ptr),
              "correlate");
         if (fuse_c)
-            STEP(launch_normalize_fused(s, p->amp_tgt.ptr, N, p->W, p->C, p->partial.ptr,
-                                        p->mixed_coarse.nsplit, nullptr, p->mstats.ptr,
-                                        p->gamma.ptr),
+            STEP(launch_normalize_fused(s, fp16 ? kFuseFP16 : kFuseBF16, p->tgt_q.ptr, N, p->W,
+                                        p->C, p->partial.ptr, p->mixed_coarse.nsplit, nullptr,
+                                        p->mstats.ptr, p->gamma.ptr),
                  "normalize");
         else
             STEP(launch_normalize_mixed(s, p->partial.ptr, p->mixed_coarse.nsplit, p->sums.ptr,
@@ -552,22 +571,28 @@ int plan_enqueue(ampcor_hip_plan *plan, int N)
          "coarse_peak");
     const bool fused = p->mixed && upsample_fuses_quantization(p->C, p->rf) &&
                        upsample_fuses_quantization(p->E, p->rf);
+    // plain mixed forms with the fused upsampling: the refined window sums come
+    // from the quantized target, and no FP32 target amplitudes are written
+    const bool quant_sums_r = fused && !p->split && normalize_fuses(p->Wr, p->Cr);
     if (fused) {
         STEP(launch_upsample(s, p->d_desc.ptr, N, p->dtype, 0, p->window.ptr, p->C, p->rf,
                              p->U_ref.ptr, p->g_ref.ptr, nullptr, qmode, p->chip_q_r.ptr,
                              p->mixed_refined.cs, p->mstats_r.ptr, p->chip_lo_r.ptr,
-                             p->hfrag_ref.ptr),
+                             p->hfrag_ref.ptr, &p->up_scratch),
              "upsample_ref");
         STEP(launch_upsample(s, p->d_desc.ptr, N, p->dtype, 1, p->window.ptr, p->E, p->rf,
-                             p->U_tgt.ptr, p->g_tgt.ptr, p->amp_tgt_r.ptr, qmode, p->tgt_q_r.ptr, 0,
-                             p->mstats_r.ptr, p->tgt_lo_r.ptr, p->hfrag_tgt.ptr),
+                             p->U_tgt.ptr, p->g_tgt.ptr, quant_sums_r ? nullptr : p->amp_tgt_r.ptr,
+                             qmode, p->tgt_q_r.ptr, 0, p->mstats_r.ptr, p->tgt_lo_r.ptr,
+                             p->hfrag_tgt.ptr, &p->up_scratch),
              "upsample_tgt");
     } else {
         STEP(launch_upsample(s, p->d_desc.ptr, N, p->dtype, 0, p->window.ptr, p->C, p->rf,
-                             p->U_ref.ptr, p->g_ref.ptr, p->amp_ref_r.ptr),
+                             p->U_ref.ptr, p->g_ref.ptr, p->amp_ref_r.ptr, kUpFP32, nullptr, 0,
+                             nullptr, nullptr, nullptr, &p->up_scratch),
              "upsample_ref");
         STEP(launch_upsample(s, p->d_desc.ptr, N, p->dtype, 1, p->window.ptr, p->E, p->rf,
-                             p->U_tgt.ptr, p->g_tgt.ptr, p->amp_tgt_r.ptr),
+                             p->U_tgt.ptr, p->g_tgt.ptr, p->amp_tgt_r.ptr, kUpFP32, nullptr, 0,
+                             nullptr, nullptr, nullptr, &p->up_scratch),
              "upsample_tgt");
         if (p->mixed)
             STEP(launch_prep_mixed(s, p->amp_ref_r.ptr, p->amp_tgt_r.ptr, N, p->Cr, p->Wr,
@@ -587,9 +612,11 @@ int plan_enqueue(ampcor_hip_plan *plan, int N)
                                     p->tgt_lo_r.ptr),
              "correlate_refined");
         if (fuse_r)
-            STEP(launch_normalize_fused(s, p->amp_tgt_r.ptr, N, p->Wr, p->Cr, p->partial_r.ptr,
-                                        p->mixed_refined.nsplit, nullptr, p->mstats_r.ptr,
-                                        p->gamma_r.ptr),
+            STEP(launch_normalize_fused(s, quant_sums_r ? (fp16 ? kFuseFP16 : kFuseBF16) : kFuseMixed,
+                                        quant_sums_r ? (const void *)p->tgt_q_r.ptr
+                                                     : (const void *)p->amp_tgt_r.ptr,
+                                        N, p->Wr, p->Cr, p->partial_r.ptr, p->mixed_refined.nsplit,
+                                        nullptr, p->mstats_r.ptr, p->gamma_r.ptr),
                  "normalize_refined");
         else
             STEP(launch_normalize_mixed(s, p->partial_r.ptr, p->mixed_refined.nsplit, p->sums_r.ptr,
@@ -600,9 +627,9 @@ int plan_enqueue(ampcor_hip_plan *plan, int N)
                               p->partial_r.ptr),
              "correlate_refined");
         if (fuse_r)
-            STEP(launch_normalize_fused(s, p->amp_tgt_r.ptr, N, p->Wr, p->Cr, p->partial_r.ptr,
-                                        p->refined_shape.nsplit, p->stats_r.ptr, nullptr,
-                                        p->gamma_r.ptr),
+            STEP(launch_normalize_fused(s, kFuseFP32, p->amp_tgt_r.ptr, N, p->Wr, p->Cr,
+                                        p->partial_r.ptr, p->refined_shape.nsplit, p->stats_r.ptr,
+                                        nullptr, p->gamma_r.ptr),
                  "normalize_refined");
         else
             STEP(launch_normalize(s, p->partial_r.ptr, p->refined_shape.nsplit, p->sums_r.ptr,
@@ -816,6 +843,11 @@ size_t plan_bytes_per_pair(int C, int W, int rf, int rm, int precision)
              2 * sizeof(MixedStats);
     if (precision == AMPCOR_HIP_BF16X3 || precision == AMPCOR_HIP_FP16X3)
         b += 2 * Cr * (Cr + 48) + 2 * Wr * Wr;
+    // banded upsampling scratch for tiles above 96 samples
+    size_t nmax = 0;
+    if (upsample_needs_scratch(C, rf)) nmax = std::max(nmax, (size_t)C);
+    if (upsample_needs_scratch((int)E, rf)) nmax = std::max(nmax, E);
+    if (nmax) b += sizeof(float2) * nmax * nmax + 2 * sizeof(double2) * ((nmax + 15) / 16);
     return b;
 }
 

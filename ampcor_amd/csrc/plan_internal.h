@@ -92,6 +92,10 @@ struct ampcor_hip_plan {
     ampcor_hip::DeviceBuffer<double> D;
     ampcor_hip::DeviceBuffer<uint16_t> chip_q, tgt_q, chip_q_r, tgt_q_r, chip_lo_r, tgt_lo_r;
     ampcor_hip::DeviceBuffer<uint16_t> hfrag_ref, hfrag_tgt;  // FP16 circulant fragments
+    // scratch of the banded upsampler (tiles above 96 samples)
+    ampcor_hip::DeviceBuffer<float2> up_T;
+    ampcor_hip::DeviceBuffer<double2> up_band, up_rows;
+    ampcor_hip::UpsampleScratch up_scratch{};
     ampcor_hip::DeviceBuffer<ampcor_hip::MixedStats> mstats, mstats_r;
 
     bool timing = false;
@@ -117,6 +121,9 @@ struct ampcor_hip_plan {
         tgt_lo_r.release();
         hfrag_ref.release();
         hfrag_tgt.release();
+        up_T.release();
+        up_band.release();
+        up_rows.release();
         mstats.release();
         mstats_r.release();
         D.release();

@@ -377,16 +377,23 @@ __device__ inline float fuse_load(const void *t, size_t k)
         return static_cast<const float *>(t)[k];
 }
 
-template <int FORM>
+// LDS-only barrier: the waves of a workgroup exchange data through LDS alone
+// (global memory is read-only input or write-once output here), so the
+// global loads in flight -- the next block's rows -- need not complete
+__device__ inline void lds_barrier()
+{
+    __builtin_amdgcn_s_waitcnt(0xc07f);   // lgkmcnt(0); vmcnt, expcnt untouched
+    __builtin_amdgcn_s_barrier();
+}
+
+template <int FORM, int COLS>
 __global__ __launch_bounds__(kFuseThreads) void k_normalize_fused(
     const void *__restrict__ target, int W, int C, const float *__restrict__ partial, int nsplit,
     const double2 *__restrict__ stats, const MixedStats *__restrict__ mstats,
-    float *__restrict__ gamma, int span)
+    float *__restrict__ gamma)
 {
     extern __shared__ double fl[];   // [2][kFuseRows][W + 1]
     const int pair = blockIdx.x, t = threadIdx.x;
-    // this workgroup's placement rows: [ub, ue), `span` blocks of kFuseRows
-    const int ub = blockIdx.y * span * kFuseRows;
     const int P = W - C + 1, ws = W + 1;
     double *p1 = fl, *p2 = fl + kFuseRows * ws;
     const size_t tbase = (size_t)pair * W * W;
@@ -413,18 +420,34 @@ __global__ __launch_bounds__(kFuseThreads) void k_normalize_fused(
         }
     }
 
-    const int P_ = W - C + 1;
-    const int ue = min(P_, ub + span * kFuseRows);
-    // vertical box sums at u = ub, summed directly
-    double v1[kFuseMaxCols], v2[kFuseMaxCols];
+    // the rows entering (u + C - 1) and leaving (u - 1) the box of each of a
+    // block's placement rows u = u0 + k, for this thread's columns
+    // (rows and columns clamped into the window: every load is in bounds and
+    // unconditional; the values of rows past the last placement are unused)
+    auto fetch = [&](int u0, float (&ra)[COLS][kFuseRows], float (&rb)[COLS][kFuseRows]) {
 #pragma unroll
-    for (int q = 0; q < kFuseMaxCols; ++q) {
+        for (int q = 0; q < COLS; ++q) {
+            const int x = min(t + kFuseThreads * q, W - 1);
+#pragma unroll
+            for (int k = 0; k < kFuseRows; ++k) {
+                const int u = u0 + k;
+                ra[q][k] = tval(min(u + C - 1, W - 1), x);
+                rb[q][k] = tval(max(min(u - 1, W - 1), 0), x);
+            }
+        }
+    };
+    float ra[COLS][kFuseRows], rb[COLS][kFuseRows];
+    fetch(0, ra, rb);
+    // vertical box sums at u = 0
+    double v1[COLS], v2[COLS];
+#pragma unroll
+    for (int q = 0; q < COLS; ++q) {
         const int x = t + kFuseThreads * q;
         v1[q] = v2[q] = 0.0;
         if (x < W)
-#pragma unroll 16
+#pragma unroll 8
             for (int i = 0; i < C; ++i) {
-                const double d = (double)tval(ub + i, x);
+                const double d = (double)tval(i, x);
                 v1[q] = v1[q] + d;
                 v2[q] = v2[q] + d * d;
             }
@@ -434,10 +457,12 @@ __global__ __launch_bounds__(kFuseThreads) void k_normalize_fused(
     const int cw = ((W + 15) / 16) | 1;
     const int x0 = min(W, sl * cw), x1 = min(W, x0 + cw);
 
-    for (int u0 = ub; u0 < ue; u0 += kFuseRows) {
-        const int rows = min(kFuseRows, ue - u0);
-        // the numerators of this thread's outputs (row r, columns sl + 16 j),
-        // loaded first so that their latency overlaps the box sums below
+    for (int u0 = 0; u0 < P; u0 += kFuseRows) {
+        const int rows = min(kFuseRows, P - u0);
+        // the next block's rows, in flight while this block is processed
+        float na[COLS][kFuseRows], nb[COLS][kFuseRows];
+        fetch(u0 + kFuseRows, na, nb);
+        // the numerators of this thread's outputs (row r, columns sl + 16 j)
         float num[kFuseOut];
 #pragma unroll
         for (int j = 0; j < kFuseOut; ++j) num[j] = 0.0f;
@@ -450,24 +475,14 @@ __global__ __launch_bounds__(kFuseThreads) void k_normalize_fused(
         }
         // slide V down to the block's rows and park them in LDS
 #pragma unroll
-        for (int q = 0; q < kFuseMaxCols; ++q) {
+        for (int q = 0; q < COLS; ++q) {
             const int x = t + kFuseThreads * q;
             if (x >= W) continue;
-            float a[kFuseRows], b[kFuseRows];
-#pragma unroll
-            for (int k = 0; k < kFuseRows; ++k) {
-                const int u = u0 + k;
-                a[k] = b[k] = 0.0f;
-                if (k < rows && u > ub) {
-                    a[k] = tval(u + C - 1, x);
-                    b[k] = tval(u - 1, x);
-                }
-            }
 #pragma unroll
             for (int k = 0; k < kFuseRows; ++k) {
                 if (k >= rows) break;
-                if (u0 + k > ub) {
-                    const double da = a[k], db = b[k];
+                if (u0 + k > 0) {
+                    const double da = ra[q][k], db = rb[q][k];
                     v1[q] = (v1[q] + da) - db;
                     v2[q] = (v2[q] + da * da) - db * db;
                 }
@@ -475,7 +490,7 @@ __global__ __launch_bounds__(kFuseThreads) void k_normalize_fused(
                 p2[k * ws + x] = v2[q];
             }
         }
-        __syncthreads();
+        lds_barrier();
         // row prefix sums: chunk prefix, then the 16-lane scan of chunk totals
         // (eight entries in flight per step)
         double run1 = 0.0, run2 = 0.0;
@@ -523,10 +538,9 @@ __global__ __launch_bounds__(kFuseThreads) void k_normalize_fused(
                         p2[r * ws + xb + i] = e2[i] + o2;
                     }
             }
-        __syncthreads();
+        lds_barrier();
         if (r < rows) {
             const int u = u0 + r;
-#pragma unroll
             for (int j = 0; j < kFuseOut; ++j) {
                 const int v = sl + 16 * j;
                 if (v >= P) break;
@@ -544,7 +558,14 @@ __global__ __launch_bounds__(kFuseThreads) void k_normalize_fused(
                 gamma[(size_t)pair * P * P + (size_t)u * P + v] = g;
             }
         }
-        __syncthreads();
+        lds_barrier();
+#pragma unroll
+        for (int q = 0; q < COLS; ++q)
+#pragma unroll
+            for (int k = 0; k < kFuseRows; ++k) {
+                ra[q][k] = na[q][k];
+                rb[q][k] = nb[q][k];
+            }
     }
 }
 
@@ -1859,12 +1880,16 @@ hipError_t launch_normalize_fused(hipStream_t s, int form, const void *target, i
     if (!normalize_fuses(W, C) || (form == kFuseFP32 ? !stats : !mstats))
         return hipErrorInvalidValue;
     const size_t lds = 2 * (size_t)kFuseRows * (W + 1) * sizeof(double);
-    const int P = W - C + 1, nblocks = (P + kFuseRows - 1) / kFuseRows;
-    const int span = nblocks;   // one workgroup per pair: the box sums slide down all rows
-    const dim3 grid(pairs, (nblocks + span - 1) / span);
+    const bool one = W <= kFuseThreads;   // one target column per thread
 #define AMPCOR_FUSE(F)                                                                              \
-    hipLaunchKernelGGL(k_normalize_fused<F>, grid, dim3(kFuseThreads), lds, s, target, W, C,        \
-                       partial, nsplit, stats, mstats, gamma, span)
+    do {                                                                                            \
+        if (one)                                                                                    \
+            hipLaunchKernelGGL((k_normalize_fused<F, 1>), dim3(pairs), dim3(kFuseThreads), lds, s,  \
+                               target, W, C, partial, nsplit, stats, mstats, gamma);                \
+        else                                                                                        \
+            hipLaunchKernelGGL((k_normalize_fused<F, 2>), dim3(pairs), dim3(kFuseThreads), lds, s,  \
+                               target, W, C, partial, nsplit, stats, mstats, gamma);                \
+    } while (0)
     switch (form) {
     case kFuseFP32: AMPCOR_FUSE(kFuseFP32); break;
     case kFuseMixed: AMPCOR_FUSE(kFuseMixed); break;

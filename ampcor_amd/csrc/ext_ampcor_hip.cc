@@ -267,26 +267,35 @@ PyObject *addPairs(PyObject *, PyObject *args)
     return result;
 }
 
-PyObject *adjust(PyObject *, PyObject *args)
+PyObject *adjust(PyObject *, PyObject *args, PyObject *kwds)
 {
+    // offsets=False skips building the tuple of (dy, dx) pairs: a caller that
+    // reads the products afterwards (products()) does not pay for the objects
+    static const char *kw[] = {"worker", "offsets", nullptr};
     PyObject *pyWorker;
-    if (!PyArg_ParseTuple(args, "O!:adjust", &PyCapsule_Type, &pyWorker)) return nullptr;
+    int want = 1;
+    if (!PyArg_ParseTupleAndKeywords(args, kwds, "O!|p:adjust", const_cast<char **>(kw),
+                                     &PyCapsule_Type, &pyWorker, &want))
+        return nullptr;
     Worker *w = get_worker(pyWorker);
     if (!w) return nullptr;
-    std::vector<float> offsets(2 * w->pairs);
+    std::vector<float> offsets(want ? 2 * w->pairs : 0);
     int rc;
     Py_BEGIN_ALLOW_THREADS
-    rc = ampcor_hip_adjust(w->plan, offsets.data());
+    rc = ampcor_hip_adjust(w->plan, want ? offsets.data() : nullptr);
     Py_END_ALLOW_THREADS
     if (rc) return raise(rc);
+    if (!want) Py_RETURN_NONE;
     PyObject *results = PyTuple_New(w->pairs);
     if (!results) return nullptr;
     for (size_t p = 0; p < w->pairs; ++p) {
-        PyObject *shift = Py_BuildValue("(dd)", (double)offsets[2 * p], (double)offsets[2 * p + 1]);
+        PyObject *shift = PyTuple_New(2);
         if (!shift) {
             Py_DECREF(results);
             return nullptr;
         }
+        PyTuple_SET_ITEM(shift, 0, PyFloat_FromDouble(offsets[2 * p]));
+        PyTuple_SET_ITEM(shift, 1, PyFloat_FromDouble(offsets[2 * p + 1]));
         PyTuple_SET_ITEM(results, p, shift);
     }
     return results;
@@ -577,7 +586,8 @@ PyMethodDef methods[] = {
     {"addReference", addReference, METH_VARARGS, "process a reference tile"},
     {"addTarget", addTarget, METH_VARARGS, "process a target tile"},
     {"addPairs", addPairs, METH_VARARGS, "register a run of pairs from int64 corner arrays"},
-    {"adjust", adjust, METH_VARARGS, "perform pixel adjustments to the registration map"},
+    {"adjust", (PyCFunction)(void (*)(void))adjust, METH_VARARGS | METH_KEYWORDS,
+     "perform pixel adjustments to the registration map"},
     {"products", products, METH_VARARGS, "offsets, correlation peaks and SNR of the last adjust"},
     {"dims", dims, METH_VARARGS, "(coarse, refined, zoomed) surface edges"},
     {"stage_gamma", stage_gamma, METH_VARARGS, "coarse correlation surfaces of the last adjust"},

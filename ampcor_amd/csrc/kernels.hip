@@ -1032,6 +1032,9 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2(
 constexpr int kUp2hMaxN = 96;
 
 typedef _Float16 halfx8_t __attribute__((ext_vector_type(8)));
+typedef _Float16 halfx2_t __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float floatx2_t __attribute__((ext_vector_type(2)));
 
 __device__ inline floatx4 mfma_f16(uint4 a, uint4 b, floatx4 c)
 {
@@ -1086,27 +1089,38 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2h(
         }
     };
 
-    // pass 1: the sum and peak of the coarse amplitudes
+    // the tile in 2 x 2 blocks, read once into registers: a wave takes 8 x 8
+    // blocks at a time (block row lane / 8, block column lane % 8), so each
+    // row segment a wave reads is 128 contiguous bytes and the row- and
+    // column-major LDS writes below hit 64 different banks
+    constexpr int kMaxJ = (kUp2hMaxN / 16) * (kUp2hMaxN / 16) / kUp2Waves + 1;
+    const int TB = Np / 16, ntb = TB * TB;
+    const int rsub = 2 * (lane >> 3), csub = 2 * (lane & 7);
+    float bre[kMaxJ][4], bim[kMaxJ][4];
+#pragma unroll
+    for (int j = 0; j < kMaxJ; ++j) {
+        const int tb = wave + kUp2Waves * j;
+        const int r = 16 * (tb / TB) + rsub, c = 16 * (tb % TB) + csub;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int rr = r + (e >> 1), cc = c + (e & 1);
+            bre[j][e] = bim[j][e] = 0.0f;
+            if (tb < ntb && rr < N && cc < N) load(rr, cc, bre[j][e], bim[j][e]);
+        }
+    }
+    // the sum and peak of the coarse amplitudes (padding adds zeros)
     double asum = 0.0;
     float amax = 0.0f;
-    const int nn = N * N;
-    for (int k0 = threadIdx.x; k0 < nn; k0 += kUp2Batch * blockDim.x) {
-        float re[kUp2Batch], im[kUp2Batch];
 #pragma unroll
-        for (int j = 0; j < kUp2Batch; ++j) {
-            const int k = k0 + j * blockDim.x;
-            re[j] = im[j] = 0.0f;
-            if (k < nn) load(k / N, k % N, re[j], im[j]);
-        }
+    for (int j = 0; j < kMaxJ; ++j) {
+        float part = 0.0f;
 #pragma unroll
-        for (int j = 0; j < kUp2Batch; ++j) {
-            const int k = k0 + j * blockDim.x;
-            if (k < nn) {
-                const float a = amplitude(re[j], im[j]);
-                asum += a;
-                amax = fmaxf(amax, a);
-            }
+        for (int e = 0; e < 4; ++e) {
+            const float a = amplitude(bre[j][e], bim[j][e]);
+            part += a;
+            amax = fmaxf(amax, a);
         }
+        asum += (double)part;
     }
     block_sum_max(red, redm, asum, amax);
     const int M = 2 * N;
@@ -1116,35 +1130,36 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2h(
     int ex = 0;
     if (amax > 0.0f) frexpf(amax, &ex);
     const float sx = amax > 0.0f ? ldexpf(1.0f, 13 - ex) : 1.0f, inv_sx = 1.0f / sx;
-    if (role == 0)
-        for (int k = threadIdx.x; k < M * (cs - M); k += blockDim.x) {
-            const int p = k / (cs - M), c = k - p * (cs - M);
-            const int col = c < 16 ? c : M + c;
-            qout[((size_t)pair * M + p) * cs + col] = 0;
-        }
-    // pass 2: the scaled FP16 tile, both layouts, zero-padded to Np x Np
-    const int npp = Np * Np;
-    for (int k0 = threadIdx.x; k0 < npp; k0 += kUp2Batch * blockDim.x) {
-        float re[kUp2Batch], im[kUp2Batch];
-#pragma unroll
-        for (int j = 0; j < kUp2Batch; ++j) {
-            const int k = k0 + j * blockDim.x;
-            const int r = k / Np, c = k - r * Np;
-            re[j] = im[j] = 0.0f;
-            if (k < npp && r < N && c < N) load(r, c, re[j], im[j]);
-        }
-#pragma unroll
-        for (int j = 0; j < kUp2Batch; ++j) {
-            const int k = k0 + j * blockDim.x;
-            if (k < npp) {
-                const int r = k / Np, c = k - r * Np;
-                const _Float16 hr = (_Float16)(re[j] * sx), hi = (_Float16)(im[j] * sx);
-                xr[r * Ls + c] = hr;
-                xi[r * Ls + c] = hi;
-                cr[c * Ls + r] = hr;
-                ci[c * Ls + r] = hi;
+    if (role == 0) {
+        // zero padding of the chip rows: columns [0, 16) and [M + 16, cs), in
+        // 32-bit words, 32 threads per row
+        const int pw = (cs - M) / 2, sub = threadIdx.x & 31;
+        if (sub < pw)
+            for (int p = threadIdx.x >> 5; p < M; p += (int)blockDim.x >> 5) {
+                const int col = sub < 8 ? 2 * sub : M + 2 * sub;
+                *reinterpret_cast<uint32_t *>(qout + ((size_t)pair * M + p) * cs + col) = 0u;
             }
-        }
+    }
+    // the scaled FP16 tile, zero-padded to Np x Np, row- and column-major
+    // (pairs of values as 32-bit words)
+    auto pack = [](float lo, float hi) {
+        const halfx2_t h = __builtin_convertvector((floatx2_t){lo, hi}, halfx2_t);
+        return __builtin_bit_cast(uint32_t, h);
+    };
+#pragma unroll
+    for (int j = 0; j < kMaxJ; ++j) {
+        const int tb = wave + kUp2Waves * j;
+        if (tb >= ntb) continue;
+        const int r = 16 * (tb / TB) + rsub, c = 16 * (tb % TB) + csub;
+        const float *v = bre[j], *w = bim[j];
+        *reinterpret_cast<uint32_t *>(xr + r * Ls + c) = pack(v[0] * sx, v[1] * sx);
+        *reinterpret_cast<uint32_t *>(xr + (r + 1) * Ls + c) = pack(v[2] * sx, v[3] * sx);
+        *reinterpret_cast<uint32_t *>(xi + r * Ls + c) = pack(w[0] * sx, w[1] * sx);
+        *reinterpret_cast<uint32_t *>(xi + (r + 1) * Ls + c) = pack(w[2] * sx, w[3] * sx);
+        *reinterpret_cast<uint32_t *>(cr + c * Ls + r) = pack(v[0] * sx, v[2] * sx);
+        *reinterpret_cast<uint32_t *>(cr + (c + 1) * Ls + r) = pack(v[1] * sx, v[3] * sx);
+        *reinterpret_cast<uint32_t *>(ci + c * Ls + r) = pack(w[0] * sx, w[2] * sx);
+        *reinterpret_cast<uint32_t *>(ci + (c + 1) * Ls + r) = pack(w[1] * sx, w[3] * sx);
     }
     __syncthreads();
 
@@ -1189,16 +1204,26 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2h(
     __syncthreads();
 
     float *out = (role == 1 && amp) ? amp + (size_t)pair * M * M : nullptr;
+    uint16_t *qbase = qout + (size_t)pair * M * (role == 0 ? cs : M) + (role == 0 ? 16 : 0);
+    const int qstride = role == 0 ? cs : M;
+    // (a / sx - qc) qs == fma(a, qs / sx, -qc qs): sx and qs are powers of two
+    const float ka = inv_sx * qs, kb = -qc * qs;
+    // chip sums in the operand's units (exact: qs is a power of two), scaled back at the end
     double s1 = 0.0, s2 = 0.0;
-    auto emit2 = [&](int p, int q, float v0, float v1) {
-        if (out) *reinterpret_cast<float2 *>(out + (size_t)p * M + q) = make_float2(v0, v1);
-        const uint16_t h0 = quantize(QMODE, (v0 - qc) * qs), h1 = quantize(QMODE, (v1 - qc) * qs);
-        const size_t at = role == 0 ? ((size_t)pair * M + p) * cs + 16 + q
-                                    : ((size_t)pair * M + p) * M + q;
-        *reinterpret_cast<uint32_t *>(qout + at) = (uint32_t)h0 | ((uint32_t)h1 << 16);
+    // y[p][q], y[p][q + 1] (q even) from amplitudes in the tile's FP16 scale
+    auto emit2 = [&](int p, int q, float a0, float a1) {
+        if (out)
+            *reinterpret_cast<float2 *>(out + (p * M + q)) = make_float2(a0 * inv_sx, a1 * inv_sx);
+        const floatx2_t x = {fmaf(a0, ka, kb), fmaf(a1, ka, kb)};
+        uint32_t h;
+        if constexpr (QMODE == kUpFP16)
+            h = __builtin_bit_cast(uint32_t, __builtin_convertvector(x, halfx2_t));
+        else
+            h = __builtin_bit_cast(uint32_t, __builtin_convertvector(x, bf16x2_t));
+        *reinterpret_cast<uint32_t *>(qbase + (p * qstride + q)) = h;
         if (role == 0) {
-            const double b0 = (double)dequantize(QMODE, h0) / qs;
-            const double b1 = (double)dequantize(QMODE, h1) / qs;
+            const double b0 = (double)dequantize(QMODE, (uint16_t)h);
+            const double b1 = (double)dequantize(QMODE, (uint16_t)(h >> 16));
             s1 += b0 + b1;
             s2 += b0 * b0 + b1 * b1;
         }
@@ -1229,10 +1254,10 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2h(
             const int a = 16 * at + 4 * lk + i;
             if (a < N && bcol < N) {
                 const int t = a * Ls + bcol;
-                const float x0 = amplitude((float)xr[t], (float)xi[t]) * inv_sx;
-                const float t1 = amplitude((float)cr[t], (float)ci[t]) * inv_sx;
-                emit2(2 * a, 2 * bcol, x0, amplitude(xR[i], xI[i]) * inv_sx);
-                emit2(2 * a + 1, 2 * bcol, t1, amplitude(yR[i], yI[i]) * inv_sx);
+                emit2(2 * a, 2 * bcol, amplitude((float)xr[t], (float)xi[t]),
+                      amplitude(xR[i], xI[i]));
+                emit2(2 * a + 1, 2 * bcol, amplitude((float)cr[t], (float)ci[t]),
+                      amplitude(yR[i], yI[i]));
             }
         }
     }
@@ -1241,8 +1266,8 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2h(
         block_sum_max(red, redm, s1, unused);
         block_sum_max(red2, redm, s2, unused);
         if (threadIdx.x == 0) {
-            stats[pair].sumr = s1;
-            stats[pair].sumr2 = s2;
+            stats[pair].sumr = s1 / qs;
+            stats[pair].sumr2 = s2 / ((double)qs * qs);
             stats[pair].scale_r = qs;
         }
     } else if (threadIdx.x == 0) {

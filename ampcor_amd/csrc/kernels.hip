@@ -365,6 +365,16 @@ constexpr int kFuseRows = 16;
 constexpr int kFuseThreads = 256;
 constexpr int kFuseMaxCols = 2;  // target columns per thread: W <= 512
 constexpr int kFuseOut = 9;      // outputs per thread and block: P <= 144
+constexpr int kFuseMaxSplit = 4; // numerator partials loaded a block ahead
+constexpr int kFuseScan = 6;     // LDS entries in flight per step of the row scans
+
+constexpr int kFuseInit = 4;     // rows in flight per thread in the first box sums
+
+// LDS row stride (doubles) of the normalizer's sums: at least W + 1 and 16 mod
+// 32, so that two consecutive rows sit 32 banks apart and the half-wave
+// accesses of two rows (16 lanes each, consecutive or 18- / 22-dword strided
+// chunks) fall in distinct banks
+__host__ __device__ inline int fuse_row_stride(int W) { return (W + 1 + 15) / 32 * 32 + 16; }
 
 template <int FORM>
 __device__ inline float fuse_load(const void *t, size_t k)
@@ -377,6 +387,28 @@ __device__ inline float fuse_load(const void *t, size_t k)
         return static_cast<const float *>(t)[k];
 }
 
+// values per 16-byte chunk of the target
+template <int FORM>
+__host__ __device__ constexpr int fuse_chunk()
+{
+    return (FORM == kFuseBF16 || FORM == kFuseFP16) ? 8 : 4;
+}
+
+// value q of a 16-byte chunk of the target
+template <int FORM>
+__device__ inline float fuse_unpack(const uint4 &w, int q)
+{
+    const uint32_t d = q < 2 ? (q ? w.y : w.x) : (q == 2 ? w.z : w.w);
+    if constexpr (FORM == kFuseBF16 || FORM == kFuseFP16) {
+        const uint32_t d2 = (q >> 1) < 2 ? ((q >> 1) ? w.y : w.x) : ((q >> 1) == 2 ? w.z : w.w);
+        const uint32_t h = (d2 >> (16 * (q & 1))) & 0xffffu;
+        return FORM == kFuseBF16 ? __uint_as_float(h << 16)
+                                 : (float)__builtin_bit_cast(_Float16, (uint16_t)h);
+    } else {
+        return __uint_as_float(d);
+    }
+}
+
 // LDS-only barrier: the waves of a workgroup exchange data through LDS alone
 // (global memory is read-only input or write-once output here), so the
 // global loads in flight -- the next block's rows -- need not complete
@@ -386,19 +418,19 @@ __device__ inline void lds_barrier()
     __builtin_amdgcn_s_barrier();
 }
 
-template <int FORM, int COLS>
-__global__ __launch_bounds__(kFuseThreads) void k_normalize_fused(
+template <int FORM, int COLS, int JO>
+__global__ __launch_bounds__(kFuseThreads, COLS == 1 && JO < 9 ? 4 : 2) void k_normalize_fused(
     const void *__restrict__ target, int W, int C, const float *__restrict__ partial, int nsplit,
     const double2 *__restrict__ stats, const MixedStats *__restrict__ mstats,
     float *__restrict__ gamma)
 {
     extern __shared__ double fl[];   // [2][kFuseRows][W + 1]
     const int pair = blockIdx.x, t = threadIdx.x;
-    const int P = W - C + 1, ws = W + 1;
+    const int P = W - C + 1, ws = fuse_row_stride(W);
     double *p1 = fl, *p2 = fl + kFuseRows * ws;
     const size_t tbase = (size_t)pair * W * W;
     auto tval = [&](int row, int x) { return fuse_load<FORM>(target, tbase + (size_t)row * W + x); };
-    const double n = (double)C * C;
+    const double n = (double)C * C, inv_n = 1.0 / n;
     // per-pair constants: the chip sum and the squared centred chip norm in the
     // units of the target sums, and for the FP32-amplitude mixed form the
     // operand scales and the window constant
@@ -438,40 +470,117 @@ __global__ __launch_bounds__(kFuseThreads) void k_normalize_fused(
     };
     float ra[COLS][kFuseRows], rb[COLS][kFuseRows];
     fetch(0, ra, rb);
-    // vertical box sums at u = 0
+    // vertical box sums at u = 0 (rows [0, C)): each thread takes a 16-byte
+    // chunk of columns (CH values) and every G-th row of it, kFuseInit rows in
+    // flight at a time; the G row groups are then summed in order through LDS
+    constexpr int CH = fuse_chunk<FORM>();
     double v1[COLS], v2[COLS];
+    {
+        const int ncc = (W + CH - 1) / CH;
+        const int G = min(kFuseRows, kFuseThreads / ncc);
+        const int g = t / ncc, cc = t - g * ncc;
+        if (g < G) {
+            double s1[CH], s2[CH];
 #pragma unroll
-    for (int q = 0; q < COLS; ++q) {
-        const int x = t + kFuseThreads * q;
-        v1[q] = v2[q] = 0.0;
-        if (x < W)
-#pragma unroll 8
-            for (int i = 0; i < C; ++i) {
-                const double d = (double)tval(i, x);
-                v1[q] = v1[q] + d;
-                v2[q] = v2[q] + d * d;
+            for (int q = 0; q < CH; ++q) s1[q] = s2[q] = 0.0;
+            const bool vec = W % CH == 0;
+            for (int i0 = g; i0 < C; i0 += kFuseInit * G) {
+                uint4 e[kFuseInit];   // raw chunks: four registers each
+#pragma unroll
+                for (int b = 0; b < kFuseInit; ++b) {
+                    const int i = min(i0 + b * G, C - 1);   // clamped: in bounds, unused
+                    if (vec) {
+                        e[b] = *reinterpret_cast<const uint4 *>(
+                            static_cast<const char *>(target) +
+                            (tbase + (size_t)i * W + CH * cc) * (16 / CH));
+                    } else {
+                        uint32_t v[CH];
+#pragma unroll
+                        for (int q = 0; q < CH; ++q)
+                            v[q] = FORM == kFuseBF16 || FORM == kFuseFP16
+                                       ? static_cast<const uint16_t *>(target)[tbase + (size_t)i * W + min(CH * cc + q, W - 1)]
+                                       : __float_as_uint(tval(i, min(CH * cc + q, W - 1)));
+                        if constexpr (CH == 8)
+                            e[b] = make_uint4(v[0] | v[1] << 16, v[2] | v[3] << 16,
+                                              v[4] | v[5] << 16, v[6] | v[7] << 16);
+                        else
+                            e[b] = make_uint4(v[0], v[1], v[2], v[3]);
+                    }
+                }
+#pragma unroll
+                for (int b = 0; b < kFuseInit; ++b)
+                    if (i0 + b * G < C)
+#pragma unroll
+                        for (int q = 0; q < CH; ++q) {
+                            const double d = (double)fuse_unpack<FORM>(e[b], q);
+                            s1[q] = s1[q] + d;
+                            s2[q] = s2[q] + d * d;
+                        }
             }
+#pragma unroll
+            for (int q = 0; q < CH; ++q)
+                if (CH * cc + q < W) {
+                    p1[g * ws + CH * cc + q] = s1[q];
+                    p2[g * ws + CH * cc + q] = s2[q];
+                }
+        }
+        lds_barrier();
+#pragma unroll
+        for (int q = 0; q < COLS; ++q) {
+            const int x = t + kFuseThreads * q;
+            v1[q] = v2[q] = 0.0;
+            if (x < W)
+                for (int h = 0; h < G; ++h) {
+                    v1[q] = v1[q] + p1[h * ws + x];
+                    v2[q] = v2[q] + p2[h * ws + x];
+                }
+        }
+        lds_barrier();   // the first block rewrites p1, p2
     }
     const int r = t >> 4, sl = t & 15;      // row of the block, slot in the row
+    // the partials of a block's numerators, loaded a block ahead when there are
+    // at most kFuseMaxSplit of them (row and column clamped: in bounds, unused
+    // past the last placement)
+    const bool ahead = nsplit <= kFuseMaxSplit;
+    float raw[kFuseMaxSplit][JO];
+    auto fetch_num = [&](int u0) {
+        const float *pp = partial + (size_t)pair * nsplit * P * P + (size_t)min(u0 + r, P - 1) * P;
+#pragma unroll
+        for (int k = 0; k < kFuseMaxSplit; ++k)
+#pragma unroll
+            for (int j = 0; j < JO; ++j)
+                raw[k][j] = k < nsplit ? pp[(size_t)k * P * P + min(sl + 16 * j, P - 1)] : 0.0f;
+    };
+    if (ahead) fetch_num(0);
     // prefix chunk per slot, odd: the 16 slots of a row start in distinct LDS banks
     const int cw = ((W + 15) / 16) | 1;
     const int x0 = min(W, sl * cw), x1 = min(W, x0 + cw);
 
     for (int u0 = 0; u0 < P; u0 += kFuseRows) {
         const int rows = min(kFuseRows, P - u0);
-        // the next block's rows, in flight while this block is processed
-        float na[COLS][kFuseRows], nb[COLS][kFuseRows];
-        fetch(u0 + kFuseRows, na, nb);
-        // the numerators of this thread's outputs (row r, columns sl + 16 j)
-        float num[kFuseOut];
+        // the numerators of this thread's outputs (row r, columns sl + 16 j):
+        // the partials fetched a block ahead, summed in split order, then the
+        // next block's fetched
+        float num[JO];
+        if (ahead) {
 #pragma unroll
-        for (int j = 0; j < kFuseOut; ++j) num[j] = 0.0f;
-        if (r < rows) {
-            const float *pp = partial + (size_t)pair * nsplit * P * P + (size_t)(u0 + r) * P + sl;
-            for (int k = 0; k < nsplit; ++k)
+            for (int j = 0; j < JO; ++j) {
+                num[j] = raw[0][j];
 #pragma unroll
-                for (int j = 0; j < kFuseOut; ++j)
-                    if (sl + 16 * j < P) num[j] += pp[(size_t)k * P * P + 16 * j];
+                for (int k = 1; k < kFuseMaxSplit; ++k)
+                    if (k < nsplit) num[j] += raw[k][j];
+            }
+            fetch_num(u0 + kFuseRows);
+        } else {
+#pragma unroll
+            for (int j = 0; j < JO; ++j) num[j] = 0.0f;
+            if (r < rows) {
+                const float *pp = partial + (size_t)pair * nsplit * P * P + (size_t)(u0 + r) * P + sl;
+                for (int k = 0; k < nsplit; ++k)
+#pragma unroll
+                    for (int j = 0; j < JO; ++j)
+                        if (sl + 16 * j < P) num[j] += pp[(size_t)k * P * P + 16 * j];
+            }
         }
         // slide V down to the block's rows and park them in LDS
 #pragma unroll
@@ -490,20 +599,22 @@ __global__ __launch_bounds__(kFuseThreads) void k_normalize_fused(
                 p2[k * ws + x] = v2[q];
             }
         }
+        // the next block's rows, in flight during this block's scans and outputs
+        fetch(u0 + kFuseRows, ra, rb);
         lds_barrier();
         // row prefix sums: chunk prefix, then the 16-lane scan of chunk totals
-        // (eight entries in flight per step)
+        // (kFuseScan entries in flight per step)
         double run1 = 0.0, run2 = 0.0;
         if (r < rows)
-            for (int xb = x0; xb < x1; xb += 8) {
-                double e1[8], e2[8];
+            for (int xb = x0; xb < x1; xb += kFuseScan) {
+                double e1[kFuseScan], e2[kFuseScan];
 #pragma unroll
-                for (int i = 0; i < 8; ++i) {
+                for (int i = 0; i < kFuseScan; ++i) {
                     e1[i] = xb + i < x1 ? p1[r * ws + xb + i] : 0.0;
                     e2[i] = xb + i < x1 ? p2[r * ws + xb + i] : 0.0;
                 }
 #pragma unroll
-                for (int i = 0; i < 8; ++i) {
+                for (int i = 0; i < kFuseScan; ++i) {
                     run1 += e1[i];
                     run2 += e2[i];
                     if (xb + i < x1) {
@@ -524,15 +635,15 @@ __global__ __launch_bounds__(kFuseThreads) void k_normalize_fused(
         o1 -= run1;   // exclusive: the totals of the chunks to the left
         o2 -= run2;
         if (r < rows && sl > 0)
-            for (int xb = x0; xb < x1; xb += 8) {
-                double e1[8], e2[8];
+            for (int xb = x0; xb < x1; xb += kFuseScan) {
+                double e1[kFuseScan], e2[kFuseScan];
 #pragma unroll
-                for (int i = 0; i < 8; ++i) {
+                for (int i = 0; i < kFuseScan; ++i) {
                     e1[i] = xb + i < x1 ? p1[r * ws + xb + i] : 0.0;
                     e2[i] = xb + i < x1 ? p2[r * ws + xb + i] : 0.0;
                 }
 #pragma unroll
-                for (int i = 0; i < 8; ++i)
+                for (int i = 0; i < kFuseScan; ++i)
                     if (xb + i < x1) {
                         p1[r * ws + xb + i] = e1[i] + o1;
                         p2[r * ws + xb + i] = e2[i] + o2;
@@ -541,14 +652,16 @@ __global__ __launch_bounds__(kFuseThreads) void k_normalize_fused(
         lds_barrier();
         if (r < rows) {
             const int u = u0 + r;
-            for (int j = 0; j < kFuseOut; ++j) {
+            for (int j = 0; j < JO; ++j) {
                 const int v = sl + 16 * j;
                 if (v >= P) break;
                 const double S1 = p1[r * ws + v + C - 1] - (v > 0 ? p1[r * ws + v - 1] : 0.0);
                 const double S2 = p2[r * ws + v + C - 1] - (v > 0 ? p2[r * ws + v - 1] : 0.0);
                 // FP64 throughout: the centred sums cancel, and a tiny variance
                 // must not flush to zero
-                const double mu = S1 / n;
+                // (the BF16 / FP16 / mixed forms multiply by 1 / n: one FP64
+                // rounding more, far below their operand rounding)
+                const double mu = FORM == kFuseFP32 ? S1 / n : S1 * inv_n;
                 const double var = S2 - mu * S1;
                 const double num_c = FORM == kFuseMixed
                                          ? (double)num[j] * inv_ab + (mean_t - mu) * sumr
@@ -559,13 +672,6 @@ __global__ __launch_bounds__(kFuseThreads) void k_normalize_fused(
             }
         }
         lds_barrier();
-#pragma unroll
-        for (int q = 0; q < COLS; ++q)
-#pragma unroll
-            for (int k = 0; k < kFuseRows; ++k) {
-                ra[q][k] = na[q][k];
-                rb[q][k] = nb[q][k];
-            }
     }
 }
 
@@ -1904,16 +2010,23 @@ hipError_t launch_normalize_fused(hipStream_t s, int form, const void *target, i
 {
     if (!normalize_fuses(W, C) || (form == kFuseFP32 ? !stats : !mstats))
         return hipErrorInvalidValue;
-    const size_t lds = 2 * (size_t)kFuseRows * (W + 1) * sizeof(double);
+    const size_t lds = 2 * (size_t)kFuseRows * fuse_row_stride(W) * sizeof(double);
     const bool one = W <= kFuseThreads;   // one target column per thread
+    // outputs per thread and block: 3, 5 or 9 (P <= 48, 80, 144)
+    const int P = W - C + 1, jo = P <= 48 ? 3 : P <= 80 ? 5 : 9;
+#define AMPCOR_FUSE_J(F, CO, J)                                                                     \
+    hipLaunchKernelGGL((k_normalize_fused<F, CO, J>), dim3(pairs), dim3(kFuseThreads), lds, s,      \
+                       target, W, C, partial, nsplit, stats, mstats, gamma)
+#define AMPCOR_FUSE_C(F, CO)                                                                        \
+    do {                                                                                            \
+        if (jo == 3) AMPCOR_FUSE_J(F, CO, 3);                                                       \
+        else if (jo == 5) AMPCOR_FUSE_J(F, CO, 5);                                                  \
+        else AMPCOR_FUSE_J(F, CO, 9);                                                               \
+    } while (0)
 #define AMPCOR_FUSE(F)                                                                              \
     do {                                                                                            \
-        if (one)                                                                                    \
-            hipLaunchKernelGGL((k_normalize_fused<F, 1>), dim3(pairs), dim3(kFuseThreads), lds, s,  \
-                               target, W, C, partial, nsplit, stats, mstats, gamma);                \
-        else                                                                                        \
-            hipLaunchKernelGGL((k_normalize_fused<F, 2>), dim3(pairs), dim3(kFuseThreads), lds, s,  \
-                               target, W, C, partial, nsplit, stats, mstats, gamma);                \
+        if (one) AMPCOR_FUSE_C(F, 1);                                                               \
+        else AMPCOR_FUSE_C(F, 2);                                                                   \
     } while (0)
     switch (form) {
     case kFuseFP32: AMPCOR_FUSE(kFuseFP32); break;
@@ -1923,6 +2036,8 @@ hipError_t launch_normalize_fused(hipStream_t s, int form, const void *target, i
     default: return hipErrorInvalidValue;
     }
 #undef AMPCOR_FUSE
+#undef AMPCOR_FUSE_C
+#undef AMPCOR_FUSE_J
     return hipGetLastError();
 }
 

@@ -1671,99 +1671,204 @@ __global__ void k_up2_finish(int pairs, int N, const double2 *__restrict__ band,
 constexpr int kGatherPrepThreads = 256;
 constexpr size_t kGatherPrepLds = 64 * 1024;
 
-template <int DTYPE, int QMODE>
-__global__ __launch_bounds__(kGatherPrepThreads) void k_gather_prep_mixed(
-    const PairDesc *__restrict__ desc, int C, int W, int CS, float *__restrict__ tgt_amp,
-    uint16_t *__restrict__ chip_q, uint16_t *__restrict__ tgt_q, MixedStats *__restrict__ stats)
+// the pair a workgroup takes: consecutive workgroups go to the eight XCDs in
+// turn, so each XCD is given a contiguous run of pairs -- neighbours in the
+// scene, whose windows overlap, then meet in the same L2
+__device__ inline int xcd_pair(int b, int pairs)
 {
-    extern __shared__ float lds[];
-    __shared__ double red[kGatherPrepThreads];
-    __shared__ double red2[kGatherPrepThreads];
-    const int pair = blockIdx.x;
-    const int t = threadIdx.x;
-    const PairDesc d = desc[pair];
-    float *ra = lds;
-    auto reduce = [&](double &a, double &b, bool maxb) {
-        red[t] = a;
-        red2[t] = b;
-        __syncthreads();
-        for (int s = kGatherPrepThreads / 2; s > 0; s >>= 1) {
-            if (t < s) {
-                red[t] += red[t + s];
-                red2[t] = maxb ? fmax(red2[t], red2[t + s]) : red2[t] + red2[t + s];
-            }
-            __syncthreads();
-        }
-        a = red[0];
-        b = red2[0];
-        __syncthreads();
-    };
-    double rs = 0.0, rm = 0.0;
-    constexpr int kB = 8;   // global loads in flight per thread
-    for (int k0 = t; k0 < C * C; k0 += kB * kGatherPrepThreads) {
-        float v[kB];
-#pragma unroll
-        for (int q = 0; q < kB; ++q) {
-            const int k = k0 + q * kGatherPrepThreads;
-            const int i = k / C, j = k - i * C;
-            v[q] = k < C * C ? detect(DTYPE, d.ref, (size_t)(d.ref_line + i) * d.ref_stride +
-                                                        d.ref_sample + j)
-                             : 0.0f;
-        }
-#pragma unroll
-        for (int q = 0; q < kB; ++q) {
-            const int k = k0 + q * kGatherPrepThreads;
-            if (k < C * C) {
-                ra[k] = v[q];
-                rs += v[q];
-                rm = fmax(rm, (double)v[q]);
-            }
+    const int x = b & 7, q = pairs >> 3, r = pairs & 7;
+    return x * q + min(x, r) + (b >> 3);
+}
+
+// a thread's walk over the column pairs (i, 2 m), (i, 2 m + 1) of an N-row
+// tile with hw column pairs per row, kGatherPrepThreads pairs apart
+struct PairWalk {
+    int i, m;
+    int di, dm, hw;
+    __device__ PairWalk(int t, int hw_) : hw(hw_)
+    {
+        i = t / hw;
+        m = t - i * hw;
+        di = kGatherPrepThreads / hw;
+        dm = kGatherPrepThreads - di * hw;
+    }
+    __device__ void next()
+    {
+        i += di;
+        m += dm;
+        if (m >= hw) {
+            m -= hw;
+            ++i;
         }
     }
-    reduce(rs, rm, true);
+};
+
+constexpr int kGatherBatch = 8;   // column pairs per thread in flight
+
+// the amplitudes of column pairs: |z| (FP32 square root) or |x|
+template <int DTYPE>
+__device__ inline void load_pairs(const void *base, size_t stride, int line, int sample, int N,
+                                  PairWalk &w, float2 (&z)[kGatherBatch][2])
+{
+#pragma unroll
+    for (int q = 0; q < kGatherBatch; ++q) {
+        // rows and columns clamped into the tile: in bounds, the extras unused
+        const int i = min(w.i, N - 1), j0 = min(2 * w.m, N - 1), j1 = min(2 * w.m + 1, N - 1);
+        const size_t row = (size_t)(line + i) * stride + sample;
+        if (DTYPE == 0) {
+            z[q][0] = static_cast<const float2 *>(base)[row + j0];
+            z[q][1] = static_cast<const float2 *>(base)[row + j1];
+        } else {
+            z[q][0] = make_float2(static_cast<const float *>(base)[row + j0], 0.0f);
+            z[q][1] = make_float2(static_cast<const float *>(base)[row + j1], 0.0f);
+        }
+        w.next();
+    }
+}
+
+__device__ inline float pair_amp(float2 z) { return amplitude(z.x, z.y); }
+
+// sum (FP64) and max over the workgroup; every thread gets the results
+__device__ inline void gp_reduce(double (*red)[kGatherPrepThreads / kWave], double &a, double &b,
+                                 bool maxb)
+{
+    for (int d = kWave / 2; d > 0; d >>= 1) {
+        a += __shfl_xor(a, d, kWave);
+        const double o = __shfl_xor(b, d, kWave);
+        b = maxb ? fmax(b, o) : b + o;
+    }
+    const int wave = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        red[0][wave] = a;
+        red[1][wave] = b;
+    }
+    __syncthreads();
+    a = red[0][0];
+    b = red[1][0];
+    for (int k = 1; k < kGatherPrepThreads / kWave; ++k) {
+        a += red[0][k];
+        b = maxb ? fmax(b, red[1][k]) : b + red[1][k];
+    }
+}
+
+template <int DTYPE, int QMODE>
+__global__ __launch_bounds__(kGatherPrepThreads, 4) void k_gather_prep_mixed(
+    const PairDesc *__restrict__ desc, int pairs, int C, int W, int CS,
+    float *__restrict__ tgt_amp, uint16_t *__restrict__ chip_q, uint16_t *__restrict__ tgt_q,
+    MixedStats *__restrict__ stats)
+{
+    extern __shared__ float ra[];   // the chip's amplitudes, [C][C]
+    __shared__ double red[2][kGatherPrepThreads / kWave];
+    const int pair = xcd_pair(blockIdx.x, pairs);
+    const int t = threadIdx.x;
+    const PairDesc d = desc[pair];
+    const int hc = (C + 1) / 2, hwin = (W + 1) / 2;
+    // the chip, and the first batch of the window (in flight meanwhile)
+    PairWalk wc(t, hc), ww(t, hwin);
+    const int nbc = (C * hc + kGatherBatch * kGatherPrepThreads - 1) / (kGatherBatch * kGatherPrepThreads);
+    const int nbw = (W * hwin + kGatherBatch * kGatherPrepThreads - 1) / (kGatherBatch * kGatherPrepThreads);
+    float2 zw[kGatherBatch][2];
+    double rs = 0.0, rm = 0.0;
+    for (int b = 0; b < nbc; ++b) {
+        float2 zc[kGatherBatch][2];
+        const int i0 = wc.i, m0 = wc.m;
+        load_pairs<DTYPE>(d.ref, d.ref_stride, d.ref_line, d.ref_sample, C, wc, zc);
+        if (b == 0) load_pairs<DTYPE>(d.tgt, d.tgt_stride, d.tgt_line, d.tgt_sample, W, ww, zw);
+        PairWalk w = wc;
+        w.i = i0;
+        w.m = m0;
+        float part = 0.0f, mx = 0.0f;
+#pragma unroll
+        for (int q = 0; q < kGatherBatch; ++q) {
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const int j = 2 * w.m + e;
+                if (w.i < C && j < C) {
+                    const float a = pair_amp(zc[q][e]);
+                    ra[w.i * C + j] = a;
+                    part += a;
+                    mx = fmaxf(mx, a);
+                }
+            }
+            w.next();
+        }
+        rs += (double)part;
+        rm = fmax(rm, (double)mx);
+    }
+    gp_reduce(red, rs, rm, true);   // also orders the LDS writes before the reads below
     const float cr = (float)(rs / ((double)C * C));
     const float ar = (float)quant_scale(QMODE, (float)rm);
     // headroom for window values far above the chip's
     const float at = (float)quant_scale(QMODE, 16.0f * (float)rm);
-    uint16_t *cq = chip_q + (size_t)pair * C * CS;
+    auto pack = [](float lo, float hi) -> uint32_t {
+        const floatx2_t x = {lo, hi};
+        if constexpr (QMODE == kUpFP16)
+            return __builtin_bit_cast(uint32_t, __builtin_convertvector(x, halfx2_t));
+        else
+            return __builtin_bit_cast(uint32_t, __builtin_convertvector(x, bf16x2_t));
+    };
+    // the chip operand rows: 16 zeros, the C values, zeros up to CS (CS and
+    // the row starts even: 32-bit words); its sums in quantized units
+    uint32_t *cq = reinterpret_cast<uint32_t *>(chip_q + (size_t)pair * C * CS);
     double s1 = 0.0, s2 = 0.0;
-    for (int k = t; k < C * CS; k += kGatherPrepThreads) {
-        const int i = k / CS, j = k - i * CS - 16;
-        uint16_t h = 0;
-        if (j >= 0 && j < C) {
-            h = quantize(QMODE, (ra[i * C + j] - cr) * ar);
-            const double back = (double)dequantize(QMODE, h) / ar;
-            s1 += back;
-            s2 += back * back;
+    {
+        PairWalk w(t, CS / 2);
+        for (int k = t; k < C * (CS / 2); k += kGatherPrepThreads) {
+            const int j = 2 * w.m - 16;
+            uint32_t h = 0;
+            if (j >= 0 && j < C) {
+                const float x0 = (ra[w.i * C + j] - cr) * ar;
+                const float x1 = j + 1 < C ? (ra[w.i * C + j + 1] - cr) * ar : 0.0f;
+                h = pack(x0, x1);
+                if (j + 1 >= C) h &= 0xffffu;
+                const double b0 = (double)dequantize(QMODE, (uint16_t)h);
+                const double b1 = (double)dequantize(QMODE, (uint16_t)(h >> 16));
+                s1 += b0 + b1;
+                s2 += b0 * b0 + b1 * b1;
+            }
+            cq[k] = h;
+            w.next();
         }
-        cq[k] = h;
     }
+    // the window: quantized with the chip's mean removed, two columns per
+    // 32-bit word when W is even; the next batch in flight while one is written
     float *ga = tgt_amp ? tgt_amp + (size_t)pair * W * W : nullptr;
     uint16_t *tq = tgt_q + (size_t)pair * W * W;
-    for (int k0 = t; k0 < W * W; k0 += kB * kGatherPrepThreads) {
-        float v[kB];
+    PairWalk wo(t, hwin);
+    for (int b = 0; b < nbw; ++b) {
+        float2 zn[kGatherBatch][2];
+        if (b + 1 < nbw) load_pairs<DTYPE>(d.tgt, d.tgt_stride, d.tgt_line, d.tgt_sample, W, ww, zn);
 #pragma unroll
-        for (int q = 0; q < kB; ++q) {
-            const int k = k0 + q * kGatherPrepThreads;
-            const int i = k / W, j = k - i * W;
-            v[q] = k < W * W ? detect(DTYPE, d.tgt, (size_t)(d.tgt_line + i) * d.tgt_stride +
-                                                        d.tgt_sample + j)
-                             : 0.0f;
+        for (int q = 0; q < kGatherBatch; ++q) {
+            const int i = wo.i, j = 2 * wo.m;
+            if (i < W) {
+                const float a0 = pair_amp(zw[q][0]), a1 = pair_amp(zw[q][1]);
+                const uint32_t h = pack((a0 - cr) * at, (a1 - cr) * at);
+                if (ga) {
+                    ga[i * W + j] = a0;
+                    if (j + 1 < W) ga[i * W + j + 1] = a1;
+                }
+                if ((W & 1) == 0) {
+                    *reinterpret_cast<uint32_t *>(tq + i * W + j) = h;
+                } else {
+                    tq[i * W + j] = (uint16_t)h;
+                    if (j + 1 < W) tq[i * W + j + 1] = (uint16_t)(h >> 16);
+                }
+            }
+            wo.next();
         }
 #pragma unroll
-        for (int q = 0; q < kB; ++q) {
-            const int k = k0 + q * kGatherPrepThreads;
-            if (k < W * W) {
-                if (ga) ga[k] = v[q];
-                tq[k] = quantize(QMODE, (v[q] - cr) * at);
-            }
+        for (int q = 0; q < kGatherBatch; ++q) {
+            zw[q][0] = zn[q][0];
+            zw[q][1] = zn[q][1];
         }
     }
-    reduce(s1, s2, false);
+    gp_reduce(red, s1, s2, false);
     if (t == 0) {
         MixedStats st;
-        st.sumr = s1;
-        st.sumr2 = s2;
+        st.sumr = s1 / ar;
+        st.sumr2 = s2 / ((double)ar * ar);
         st.scale_r = ar;
         st.scale_t = at;
         st.mean_t = cr;
@@ -2158,7 +2263,8 @@ hipError_t launch_gather_prep_mixed(hipStream_t s, const PairDesc *desc, int pai
     const size_t lds = (size_t)C * C * sizeof(float);
     const dim3 grid(pairs), block(kGatherPrepThreads);
 #define AMPCOR_GP(DT, QM)                                                                            \
-    hipLaunchKernelGGL((k_gather_prep_mixed<DT, QM>), grid, block, lds, s, desc, C, W, CS, tgt_amp, \
+    hipLaunchKernelGGL((k_gather_prep_mixed<DT, QM>), grid, block, lds, s, desc, pairs, C, W, CS,     \
+                       tgt_amp,                                                                    \
                        chip_q, tgt_q, stats)
     if (dtype == 0) {
         if (fp16) AMPCOR_GP(0, kUpFP16);

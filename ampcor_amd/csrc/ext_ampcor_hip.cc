@@ -103,6 +103,7 @@ int parse_precision(const char *name, int *precision)
     else if (!strcmp(name, "fp16")) *precision = AMPCOR_HIP_FP16;
     else if (!strcmp(name, "bf16x3")) *precision = AMPCOR_HIP_BF16X3;
     else if (!strcmp(name, "fp16x3")) *precision = AMPCOR_HIP_FP16X3;
+    else if (!strcmp(name, "fp8")) *precision = AMPCOR_HIP_FP8;
     else {
         PyErr_Format(PyExc_ValueError, "unknown precision '%s'", name);
         return -1;

@@ -30,6 +30,9 @@ struct MixedStats {
 // quantization modes of the upsampler: detected FP32 only, or BF16 / FP16
 // operands of the mixed contraction as well
 enum UpsampleOutput { kUpFP32 = 0, kUpBF16 = 1, kUpFP16 = 2 };
+// OCP e4m3 operands (one byte each) of the FP8 coarse contraction; written by
+// the mixed gather only
+constexpr int kUpFP8 = 3;
 
 // tiling of one correlation launch: placements [u0, u0+ub) x [v0, v0+vb) of one
 // pair, chip columns split into `nsplit` ranges, each its own workgroup
@@ -61,9 +64,10 @@ hipError_t launch_normalize(hipStream_t s, const float *partial, int nsplit, con
                             const double2 *stats, int pairs, int C, int W, float *gamma);
 // window sums and normalization in one pass (W <= 512, P <= 144; tolerance-checked
 // stages).  form kFuseFP32: FP32 amplitudes and `stats` (sum r', |r'|);
-// kFuseMixed: FP32 amplitudes and `mstats`; kFuseBF16 / kFuseFP16: the sums
-// come from the quantized target (the contraction's operand) and `mstats`
-enum { kFuseFP32 = 0, kFuseMixed = 1, kFuseBF16 = 2, kFuseFP16 = 3 };
+// kFuseMixed: FP32 amplitudes and `mstats`; kFuseBF16 / kFuseFP16 / kFuseFP8:
+// the sums come from the quantized target (the contraction's operand, two
+// bytes or one byte per value) and `mstats`
+enum { kFuseFP32 = 0, kFuseMixed = 1, kFuseBF16 = 2, kFuseFP16 = 3, kFuseFP8 = 4 };
 bool normalize_fuses(int W, int C);
 hipError_t launch_normalize_fused(hipStream_t s, int form, const void *target, int pairs, int W,
                                   int C, const float *partial, int nsplit, const double2 *stats,
@@ -104,9 +108,11 @@ hipError_t launch_upsample(hipStream_t s, const PairDesc *desc, int pairs, int d
 // mixed coarse stage in one pass when the tiles fit in LDS: gather, detect,
 // write the FP32 window (for the window sums) and both quantized operands
 bool gather_prep_fits(int C, int W);
+// qmode kUpBF16 / kUpFP16 / kUpFP8; with kUpFP8 chip_q and tgt_q hold bytes
+// (rows of CS bytes and W x W bytes)
 hipError_t launch_gather_prep_mixed(hipStream_t s, const PairDesc *desc, int pairs, int dtype,
-                                    int C, int W, int CS, bool fp16, float *tgt_amp,
-                                    uint16_t *chip_q, uint16_t *tgt_q, MixedStats *stats);
+                                    int C, int W, int CS, int qmode, float *tgt_amp,
+                                    void *chip_q, void *tgt_q, MixedStats *stats);
 // ---- mixed precision (mixed.hip) ----
 
 // tiling of the MFMA contraction: 16 x 16 placement tiles, nk K-steps of 32
@@ -132,6 +138,21 @@ hipError_t launch_correlate_mixed(hipStream_t s, const MixedShape &shape, const 
 hipError_t launch_normalize_mixed(hipStream_t s, const float *partial, int nsplit,
                                   const double2 *sums, const MixedStats *stats, int pairs, int C,
                                   int W, float *gamma);
+
+// ---- FP8 coarse contraction (fp8.hip) ----
+
+// tiling of the FP8 contraction (v_mfma_f32_16x16x128_f8f6f4): K-steps of 128
+// = eight chip rows x 16 target columns; chip operand rows of cs bytes (16
+// zero bytes, C values, zeros); chip rows split into nsplit ranges of ilen (a
+// multiple of 8, zero past C)
+struct Fp8Shape {
+    int nk, cs;
+    int wg_ut, wg_vt, tiles_u, tiles_v, nsplit, ilen;
+    size_t lds_bytes;
+};
+Fp8Shape choose_fp8_shape(int C, int P, int nsplit = 0);
+hipError_t launch_correlate_fp8(hipStream_t s, const Fp8Shape &shape, const uint8_t *chip_q,
+                                const uint8_t *tgt_q, int pairs, int C, int W, float *partial);
 
 hipError_t launch_zoom(hipStream_t s, const float *gamma_refined, int pairs, int R, int zf,
                        const double *dirichlet, const int2 *window, const float *coarse_peak,

@@ -18,6 +18,8 @@
 
 #include <math.h>
 
+#include <type_traits>
+
 namespace ampcor_hip {
 
 namespace {
@@ -34,6 +36,17 @@ __device__ inline float detect(int dtype, const void *base, size_t offset)
     }
     return fabsf(static_cast<const float *>(base)[offset]);
 }
+
+// two values as OCP e4m3 bytes (round to nearest even; clamped to the finite
+// range first, where the conversion would give NaN), in the low 16 bits
+__device__ inline uint32_t pack_fp8(float lo, float hi)
+{
+    lo = fminf(fmaxf(lo, -448.0f), 448.0f);
+    hi = fminf(fmaxf(hi, -448.0f), 448.0f);
+    return (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(lo, hi, 0, false) & 0xffffu;
+}
+
+__device__ inline float dequant_fp8(uint32_t byte) { return __builtin_amdgcn_cvt_f32_fp8((int)byte, 0); }
 
 // ---------------------------------------------------------------------------
 // gather + detect: chips and windows straight from the resident rasters
@@ -383,28 +396,39 @@ __device__ inline float fuse_load(const void *t, size_t k)
         return __uint_as_float((uint32_t) static_cast<const uint16_t *>(t)[k] << 16);
     else if constexpr (FORM == kFuseFP16)
         return (float)__builtin_bit_cast(_Float16, static_cast<const uint16_t *>(t)[k]);
+    else if constexpr (FORM == kFuseFP8)
+        return dequant_fp8(static_cast<const uint8_t *>(t)[k]);
     else
         return static_cast<const float *>(t)[k];
 }
 
-// values per 16-byte chunk of the target
+// bytes per target value, and values per chunk read at once (16 bytes; 8
+// bytes for FP8, which keeps eight running sums per thread)
+template <int FORM>
+__host__ __device__ constexpr int fuse_elem_bytes()
+{
+    return FORM == kFuseFP8 ? 1 : (FORM == kFuseBF16 || FORM == kFuseFP16) ? 2 : 4;
+}
 template <int FORM>
 __host__ __device__ constexpr int fuse_chunk()
 {
-    return (FORM == kFuseBF16 || FORM == kFuseFP16) ? 8 : 4;
+    return fuse_elem_bytes<FORM>() == 4 ? 4 : 8;
 }
 
-// value q of a 16-byte chunk of the target
+// value q of a chunk of the target (w.z, w.w unused for FP8)
 template <int FORM>
 __device__ inline float fuse_unpack(const uint4 &w, int q)
 {
-    const uint32_t d = q < 2 ? (q ? w.y : w.x) : (q == 2 ? w.z : w.w);
-    if constexpr (FORM == kFuseBF16 || FORM == kFuseFP16) {
+    if constexpr (FORM == kFuseFP8) {
+        const uint32_t d = q < 4 ? w.x : w.y;
+        return dequant_fp8((d >> (8 * (q & 3))) & 0xffu);
+    } else if constexpr (FORM == kFuseBF16 || FORM == kFuseFP16) {
         const uint32_t d2 = (q >> 1) < 2 ? ((q >> 1) ? w.y : w.x) : ((q >> 1) == 2 ? w.z : w.w);
         const uint32_t h = (d2 >> (16 * (q & 1))) & 0xffffu;
         return FORM == kFuseBF16 ? __uint_as_float(h << 16)
                                  : (float)__builtin_bit_cast(_Float16, (uint16_t)h);
     } else {
+        const uint32_t d = q < 2 ? (q ? w.y : w.x) : (q == 2 ? w.z : w.w);
         return __uint_as_float(d);
     }
 }
@@ -489,18 +513,32 @@ __global__ __launch_bounds__(kFuseThreads, COLS == 1 && JO < 9 ? 4 : 2) void k_n
 #pragma unroll
                 for (int b = 0; b < kFuseInit; ++b) {
                     const int i = min(i0 + b * G, C - 1);   // clamped: in bounds, unused
+                    const size_t at = tbase + (size_t)i * W;
                     if (vec) {
-                        e[b] = *reinterpret_cast<const uint4 *>(
-                            static_cast<const char *>(target) +
-                            (tbase + (size_t)i * W + CH * cc) * (16 / CH));
+                        const char *src = static_cast<const char *>(target) +
+                                          (at + CH * cc) * fuse_elem_bytes<FORM>();
+                        if constexpr (FORM == kFuseFP8) {
+                            const uint2 w2 = *reinterpret_cast<const uint2 *>(src);
+                            e[b] = make_uint4(w2.x, w2.y, 0u, 0u);
+                        } else {
+                            e[b] = *reinterpret_cast<const uint4 *>(src);
+                        }
                     } else {
                         uint32_t v[CH];
 #pragma unroll
-                        for (int q = 0; q < CH; ++q)
-                            v[q] = FORM == kFuseBF16 || FORM == kFuseFP16
-                                       ? static_cast<const uint16_t *>(target)[tbase + (size_t)i * W + min(CH * cc + q, W - 1)]
-                                       : __float_as_uint(tval(i, min(CH * cc + q, W - 1)));
-                        if constexpr (CH == 8)
+                        for (int q = 0; q < CH; ++q) {
+                            const size_t k = at + min(CH * cc + q, W - 1);
+                            if constexpr (FORM == kFuseFP8)
+                                v[q] = static_cast<const uint8_t *>(target)[k];
+                            else if constexpr (FORM == kFuseBF16 || FORM == kFuseFP16)
+                                v[q] = static_cast<const uint16_t *>(target)[k];
+                            else
+                                v[q] = __float_as_uint(static_cast<const float *>(target)[k]);
+                        }
+                        if constexpr (FORM == kFuseFP8)
+                            e[b] = make_uint4(v[0] | v[1] << 8 | v[2] << 16 | v[3] << 24,
+                                              v[4] | v[5] << 8 | v[6] << 16 | v[7] << 24, 0u, 0u);
+                        else if constexpr (CH == 8)
                             e[b] = make_uint4(v[0] | v[1] << 16, v[2] | v[3] << 16,
                                               v[4] | v[5] << 16, v[6] | v[7] << 16);
                         else
@@ -887,14 +925,16 @@ __device__ inline float dequantize(int qmode, uint16_t h)
     return __uint_as_float((uint32_t)h << 16);
 }
 
-// power of two bringing max-abs into [0.5, 1); 1 for BF16, whose range is FP32's
+// power of two bringing max-abs into [0.5, 1) for FP16 and into [128, 256)
+// for FP8 (e4m3 tops out at 448); 1 for BF16, whose range is FP32's
 __device__ inline double quant_scale(int qmode, float maxabs)
 {
-    if (qmode != kUpFP16 || !(maxabs > 0.0f)) return 1.0;
+    if ((qmode != kUpFP16 && qmode != kUpFP8) || !(maxabs > 0.0f)) return 1.0;
     int e;
     frexp((double)maxabs, &e);
-    return ldexp(1.0, -e);
+    return ldexp(1.0, qmode == kUpFP8 ? 8 - e : -e);
 }
+
 
 // sum (FP64) and max over the workgroup; every thread gets the results
 __device__ inline void block_sum_max(double *red, float *redm, double &sum, float &mx)
@@ -1755,9 +1795,15 @@ __device__ inline void gp_reduce(double (*red)[kGatherPrepThreads / kWave], doub
 template <int DTYPE, int QMODE>
 __global__ __launch_bounds__(kGatherPrepThreads, 4) void k_gather_prep_mixed(
     const PairDesc *__restrict__ desc, int pairs, int C, int W, int CS,
-    float *__restrict__ tgt_amp, uint16_t *__restrict__ chip_q, uint16_t *__restrict__ tgt_q,
+    float *__restrict__ tgt_amp, void *__restrict__ chip_qv, void *__restrict__ tgt_qv,
     MixedStats *__restrict__ stats)
 {
+    // operand elements: two bytes (BF16, FP16) or one (FP8); pairs of them
+    // are written as one word
+    constexpr bool kByte = QMODE == kUpFP8;
+    using Elem = std::conditional_t<kByte, uint8_t, uint16_t>;
+    using Word = std::conditional_t<kByte, uint16_t, uint32_t>;
+    Elem *chip_q = static_cast<Elem *>(chip_qv), *tgt_q = static_cast<Elem *>(tgt_qv);
     extern __shared__ float ra[];   // the chip's amplitudes, [C][C]
     __shared__ double red[2][kGatherPrepThreads / kWave];
     const int pair = xcd_pair(blockIdx.x, pairs);
@@ -1803,14 +1849,22 @@ __global__ __launch_bounds__(kGatherPrepThreads, 4) void k_gather_prep_mixed(
     const float at = (float)quant_scale(QMODE, 16.0f * (float)rm);
     auto pack = [](float lo, float hi) -> uint32_t {
         const floatx2_t x = {lo, hi};
-        if constexpr (QMODE == kUpFP16)
+        if constexpr (QMODE == kUpFP8)
+            return pack_fp8(lo, hi);
+        else if constexpr (QMODE == kUpFP16)
             return __builtin_bit_cast(uint32_t, __builtin_convertvector(x, halfx2_t));
         else
             return __builtin_bit_cast(uint32_t, __builtin_convertvector(x, bf16x2_t));
     };
+    constexpr int kBits = 8 * sizeof(Elem);
+    constexpr uint32_t kLow = (1u << kBits) - 1u;
+    auto dq = [](uint32_t e) -> float {
+        if constexpr (QMODE == kUpFP8) return dequant_fp8(e);
+        else return dequantize(QMODE, (uint16_t)e);
+    };
     // the chip operand rows: 16 zeros, the C values, zeros up to CS (CS and
-    // the row starts even: 32-bit words); its sums in quantized units
-    uint32_t *cq = reinterpret_cast<uint32_t *>(chip_q + (size_t)pair * C * CS);
+    // the row starts even: pairs as words); its sums in quantized units
+    Word *cq = reinterpret_cast<Word *>(chip_q + (size_t)pair * C * CS);
     double s1 = 0.0, s2 = 0.0;
     {
         PairWalk w(t, CS / 2);
@@ -1821,20 +1875,20 @@ __global__ __launch_bounds__(kGatherPrepThreads, 4) void k_gather_prep_mixed(
                 const float x0 = (ra[w.i * C + j] - cr) * ar;
                 const float x1 = j + 1 < C ? (ra[w.i * C + j + 1] - cr) * ar : 0.0f;
                 h = pack(x0, x1);
-                if (j + 1 >= C) h &= 0xffffu;
-                const double b0 = (double)dequantize(QMODE, (uint16_t)h);
-                const double b1 = (double)dequantize(QMODE, (uint16_t)(h >> 16));
+                if (j + 1 >= C) h &= kLow;
+                const double b0 = (double)dq(h & kLow);
+                const double b1 = (double)dq((h >> kBits) & kLow);
                 s1 += b0 + b1;
                 s2 += b0 * b0 + b1 * b1;
             }
-            cq[k] = h;
+            cq[k] = (Word)h;
             w.next();
         }
     }
     // the window: quantized with the chip's mean removed, two columns per
     // 32-bit word when W is even; the next batch in flight while one is written
     float *ga = tgt_amp ? tgt_amp + (size_t)pair * W * W : nullptr;
-    uint16_t *tq = tgt_q + (size_t)pair * W * W;
+    Elem *tq = tgt_q + (size_t)pair * W * W;
     PairWalk wo(t, hwin);
     for (int b = 0; b < nbw; ++b) {
         float2 zn[kGatherBatch][2];
@@ -1850,10 +1904,10 @@ __global__ __launch_bounds__(kGatherPrepThreads, 4) void k_gather_prep_mixed(
                     if (j + 1 < W) ga[i * W + j + 1] = a1;
                 }
                 if ((W & 1) == 0) {
-                    *reinterpret_cast<uint32_t *>(tq + i * W + j) = h;
+                    *reinterpret_cast<Word *>(tq + i * W + j) = (Word)h;
                 } else {
-                    tq[i * W + j] = (uint16_t)h;
-                    if (j + 1 < W) tq[i * W + j + 1] = (uint16_t)(h >> 16);
+                    tq[i * W + j] = (Elem)h;
+                    if (j + 1 < W) tq[i * W + j + 1] = (Elem)(h >> kBits);
                 }
             }
             wo.next();
@@ -2138,6 +2192,7 @@ hipError_t launch_normalize_fused(hipStream_t s, int form, const void *target, i
     case kFuseMixed: AMPCOR_FUSE(kFuseMixed); break;
     case kFuseBF16: AMPCOR_FUSE(kFuseBF16); break;
     case kFuseFP16: AMPCOR_FUSE(kFuseFP16); break;
+    case kFuseFP8: AMPCOR_FUSE(kFuseFP8); break;
     default: return hipErrorInvalidValue;
     }
 #undef AMPCOR_FUSE
@@ -2257,22 +2312,24 @@ hipError_t launch_upsample(hipStream_t s, const PairDesc *desc, int pairs, int d
 bool gather_prep_fits(int C, int W) { return (size_t)C * C * sizeof(float) <= kGatherPrepLds; }
 
 hipError_t launch_gather_prep_mixed(hipStream_t s, const PairDesc *desc, int pairs, int dtype,
-                                    int C, int W, int CS, bool fp16, float *tgt_amp,
-                                    uint16_t *chip_q, uint16_t *tgt_q, MixedStats *stats)
+                                    int C, int W, int CS, int qmode, float *tgt_amp,
+                                    void *chip_q, void *tgt_q, MixedStats *stats)
 {
     const size_t lds = (size_t)C * C * sizeof(float);
     const dim3 grid(pairs), block(kGatherPrepThreads);
 #define AMPCOR_GP(DT, QM)                                                                            \
     hipLaunchKernelGGL((k_gather_prep_mixed<DT, QM>), grid, block, lds, s, desc, pairs, C, W, CS,     \
-                       tgt_amp,                                                                    \
-                       chip_q, tgt_q, stats)
-    if (dtype == 0) {
-        if (fp16) AMPCOR_GP(0, kUpFP16);
-        else AMPCOR_GP(0, kUpBF16);
-    } else {
-        if (fp16) AMPCOR_GP(1, kUpFP16);
-        else AMPCOR_GP(1, kUpBF16);
-    }
+                       tgt_amp, chip_q, tgt_q, stats)
+#define AMPCOR_GP_Q(DT)                                                                              \
+    do {                                                                                             \
+        if (qmode == kUpFP8) AMPCOR_GP(DT, kUpFP8);                                                  \
+        else if (qmode == kUpFP16) AMPCOR_GP(DT, kUpFP16);                                           \
+        else AMPCOR_GP(DT, kUpBF16);                                                                 \
+    } while (0)
+    if (qmode != kUpBF16 && qmode != kUpFP16 && qmode != kUpFP8) return hipErrorInvalidValue;
+    if (dtype == 0) AMPCOR_GP_Q(0);
+    else AMPCOR_GP_Q(1);
+#undef AMPCOR_GP_Q
 #undef AMPCOR_GP
     return hipGetLastError();
 }

@@ -275,11 +275,16 @@ int ampcor_hip_plan_create(int device, size_t pairs, size_t ref_lines, size_t re
     const size_t R = Wr - Cr + 1;
     if (Wr > 512) return fail(AMPCOR_HIP_EINVAL, "refined window edge must be <= 512");
     if (R > 129) return fail(AMPCOR_HIP_EINVAL, "refined surface too large (2 rf rm + 1 > 129)");
-    if (precision < AMPCOR_HIP_FP32 || precision > AMPCOR_HIP_FP16X3)
-        return fail(AMPCOR_HIP_EINVAL, "unknown precision (FP32, BF16, FP16, BF16X3 or FP16X3)");
+    if (precision < AMPCOR_HIP_FP32 || precision > AMPCOR_HIP_FP8)
+        return fail(AMPCOR_HIP_EINVAL, "unknown precision (FP32, BF16, FP16, BF16X3, FP16X3 or FP8)");
     if (precision != AMPCOR_HIP_FP32 && (C % 16 || C * refine_factor > 256))
         return fail(AMPCOR_HIP_EINVAL, "the mixed forms need a chip edge that is a multiple of 16 "
                                        "and a refined chip edge <= 256");
+    // the FP8 coarse stage runs as one gather-quantize pass and the fused
+    // normalizer (its window sums come from the FP8 operand)
+    if (precision == AMPCOR_HIP_FP8 &&
+        (!gather_prep_fits((int)C, (int)W) || !normalize_fuses((int)W, (int)C)))
+        return fail(AMPCOR_HIP_EINVAL, "the FP8 form needs a chip edge <= 128 and (window - chip + 1) <= 144");
     if (int rc = check_device(device)) return rc;
 
     auto *p = new (std::nothrow) ampcor_hip_plan();
@@ -312,11 +317,13 @@ int ampcor_hip_plan_create(int device, size_t pairs, size_t ref_lines, size_t re
     }
     p->mixed = precision != AMPCOR_HIP_FP32;
     p->split = precision == AMPCOR_HIP_BF16X3 || precision == AMPCOR_HIP_FP16X3;
-    p->fp16 = precision == AMPCOR_HIP_FP16 || precision == AMPCOR_HIP_FP16X3;
+    p->fp8 = precision == AMPCOR_HIP_FP8;
+    p->fp16 = precision == AMPCOR_HIP_FP16 || precision == AMPCOR_HIP_FP16X3 || p->fp8;
     if (p->mixed) {
         p->mixed_coarse = choose_mixed_shape(p->C, p->P);
         p->mixed_refined = choose_mixed_shape(p->Cr, p->R, 0, p->split);
     }
+    if (p->fp8) p->fp8_coarse = choose_fp8_shape(p->C, p->P);
 
     DeviceGuard guard(device);
     auto cleanup = [&](int rc) {
@@ -343,7 +350,8 @@ int ampcor_hip_plan_create(int device, size_t pairs, size_t ref_lines, size_t re
     A(p->sums_r.alloc(N * R * R));
     // partial buffers leave room for the autotuner's largest candidate splits
     const size_t ns_c = std::max({p->coarse_shape.nsplit, p->mixed ? p->mixed_coarse.nsplit : 1,
-                                  p->mixed ? kTuneMaxSplitCoarse : 1});
+                                  p->mixed ? kTuneMaxSplitCoarse : 1,
+                                  p->fp8 ? p->fp8_coarse.nsplit : 1});
     const size_t ns_r = std::max({p->refined_shape.nsplit, p->mixed ? p->mixed_refined.nsplit : 1,
                                   kTuneMaxSplitRefined});
     A(p->partial.alloc(N * ns_c * p->P * p->P));
@@ -358,6 +366,7 @@ int ampcor_hip_plan_create(int device, size_t pairs, size_t ref_lines, size_t re
     A(p->stats_r.alloc(N));
     A(p->partial_r.alloc(N * ns_r * R * R));
     if (p->mixed) {
+        // (the FP8 operands, one byte per value, fit the same buffers)
         A(p->chip_q.alloc(N * C * p->mixed_coarse.cs));
         A(p->tgt_q.alloc(N * W * W));
         A(p->chip_q_r.alloc(N * Cr * p->mixed_refined.cs));
@@ -522,8 +531,10 @@ int plan_enqueue(ampcor_hip_plan *plan, int N)
     const int qmode = fp16 ? kUpFP16 : kUpBF16;
     if (p->mixed && gather_prep_fits(p->C, p->W)) {
         // the fused normalizer takes the window sums from the quantized target
-        STEP(launch_gather_prep_mixed(s, p->d_desc.ptr, N, p->dtype, p->C, p->W, p->mixed_coarse.cs,
-                                      fp16, normalize_fuses(p->W, p->C) ? nullptr : p->amp_tgt.ptr,
+        STEP(launch_gather_prep_mixed(s, p->d_desc.ptr, N, p->dtype, p->C, p->W,
+                                      p->fp8 ? p->fp8_coarse.cs : p->mixed_coarse.cs,
+                                      p->fp8 ? kUpFP8 : qmode,
+                                      normalize_fuses(p->W, p->C) ? nullptr : p->amp_tgt.ptr,
                                       p->chip_q.ptr, p->tgt_q.ptr, p->mstats.ptr),
              "gather_prep");
     } else {
@@ -545,7 +556,15 @@ int plan_enqueue(ampcor_hip_plan *plan, int N)
     const bool fuse_r = normalize_fuses(p->Wr, p->Cr);
     if (!fuse_c)
         STEP(launch_window_sums(s, p->amp_tgt.ptr, N, p->W, p->C, p->sums.ptr), "window_sums");
-    if (p->mixed) {
+    if (p->fp8) {
+        const auto *cq8 = reinterpret_cast<const uint8_t *>(p->chip_q.ptr);
+        const auto *tq8 = reinterpret_cast<const uint8_t *>(p->tgt_q.ptr);
+        STEP(launch_correlate_fp8(s, p->fp8_coarse, cq8, tq8, N, p->C, p->W, p->partial.ptr),
+             "correlate");
+        STEP(launch_normalize_fused(s, kFuseFP8, tq8, N, p->W, p->C, p->partial.ptr,
+                                    p->fp8_coarse.nsplit, nullptr, p->mstats.ptr, p->gamma.ptr),
+             "normalize");
+    } else if (p->mixed) {
         STEP(launch_correlate_mixed(s, p->mixed_coarse, p->chip_q.ptr, p->tgt_q.ptr, N, p->C, p->W,
                                     fp16, p->partial.ptr),
              "correlate");
@@ -805,6 +824,7 @@ int ampcor_hip_plan_autotune(ampcor_hip_plan *plan, int reps)
                        plan->chip_q_r.ptr, plan->tgt_q_r.ptr, plan->chip_lo_r.ptr, plan->tgt_lo_r.ptr,
                        plan->partial_r.ptr, plan->split}};
         for (auto &st : stages) {
+            if (plan->fp8 && st.shape == &plan->mixed_coarse) continue;   // tuned below
             float best = 1e30f;
             MixedShape pick = *st.shape;
             for (int ns = 1; ns <= st.maxsplit && !rc; ns *= 2) {
@@ -819,6 +839,23 @@ int ampcor_hip_plan_autotune(ampcor_hip_plan *plan, int reps)
                 if (ms < best) best = ms, pick = sh;
             }
             *st.shape = pick;
+        }
+        if (plan->fp8 && !rc) {
+            float best = 1e30f;
+            Fp8Shape pick = plan->fp8_coarse;
+            const auto *cq8 = reinterpret_cast<const uint8_t *>(plan->chip_q.ptr);
+            const auto *tq8 = reinterpret_cast<const uint8_t *>(plan->tgt_q.ptr);
+            for (int ns = 1; ns <= kTuneMaxSplitCoarse && !rc; ns *= 2) {
+                Fp8Shape sh = choose_fp8_shape(plan->C, plan->P, ns);
+                if (sh.nsplit > kTuneMaxSplitCoarse || sh.lds_bytes > 150 * 1024) continue;
+                float ms;
+                if ((rc = timeit([&] { return launch_correlate_fp8(s, sh, cq8, tq8, N, plan->C,
+                                                                   plan->W, plan->partial.ptr); },
+                                 &ms)))
+                    break;
+                if (ms < best) best = ms, pick = sh;
+            }
+            plan->fp8_coarse = pick;
         }
     }
     (void)hipEventDestroy(e0);

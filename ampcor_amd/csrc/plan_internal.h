@@ -77,6 +77,8 @@ struct ampcor_hip_plan {
 
     ampcor_hip::CorrelateShape coarse_shape{}, refined_shape{};
     ampcor_hip::MixedShape mixed_coarse{}, mixed_refined{};
+    ampcor_hip::Fp8Shape fp8_coarse{};
+    bool fp8 = false;    // FP8 coarse contraction (refined stage as FP16)
     bool mixed = false;
     bool split = false;  // hi/lo operands in the refined contraction
     bool fp16 = false;

@@ -405,7 +405,7 @@ int ampcor_hip_server_submit(ampcor_hip_server *sv, const ampcor_hip_raster *ref
         window > 512 || refine_factor < 1 || refine_factor > 8 || zoom_factor < 1 || zoom_factor > 31 ||
         refine_factor * (chip + 2 * refine_margin) > 512 || 2 * refine_factor * refine_margin + 1 > 129)
         return fail(AMPCOR_HIP_EINVAL, "unsupported job geometry");
-    if (precision < AMPCOR_HIP_FP32 || precision > AMPCOR_HIP_FP16X3)
+    if (precision < AMPCOR_HIP_FP32 || precision > AMPCOR_HIP_FP8)
         return fail(AMPCOR_HIP_EINVAL, "unknown precision");
     if (precision != AMPCOR_HIP_FP32 && (chip % 16 || chip * refine_factor > 256))
         return fail(AMPCOR_HIP_EINVAL, "the mixed forms need chip % 16 == 0 and rf * chip <= 256");

@@ -54,6 +54,9 @@ extern "C" {
 /* mixed, with split hi/lo operands (three products) in the refined re-correlation */
 #define AMPCOR_HIP_BF16X3 3
 #define AMPCOR_HIP_FP16X3 4
+/* mixed: the coarse contraction on FP8 (OCP e4m3) matrix cores, the refined
+ * re-correlation as AMPCOR_HIP_FP16 */
+#define AMPCOR_HIP_FP8 5
 
 /* raster pixel types */
 #define AMPCOR_HIP_COMPLEX64 0 /* SLC: interleaved float re, im */

@@ -154,6 +154,8 @@ Fp8Shape choose_fp8_shape(int C, int P, int nsplit = 0);
 hipError_t launch_correlate_fp8(hipStream_t s, const Fp8Shape &shape, const uint8_t *chip_q,
                                 const uint8_t *tgt_q, int pairs, int C, int W, float *partial);
 
+// whether the zoom of an R x R refined surface by zf fits one workgroup's LDS
+bool zoom_fits(int R, int zf);
 hipError_t launch_zoom(hipStream_t s, const float *gamma_refined, int pairs, int R, int zf,
                        const double *dirichlet, const int2 *window, const float *coarse_peak,
                        int margin, int rf, float *offsets, float *refined_peak, int2 *zoom_loc);

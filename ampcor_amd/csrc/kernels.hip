@@ -746,14 +746,24 @@ __global__ __launch_bounds__(256) void k_coarse_peak(const float *__restrict__ g
     if (pair >= pairs) return;
     const int cells = P * P;
     const float *g = gamma + (size_t)pair * cells;
+    // kPeakBatch loads in flight per lane; each lane visits its cells in
+    // increasing order, so its ties resolve to the first
+    constexpr int kPeakBatch = 8;
     float best = -INFINITY;
     int bidx = 0x7fffffff;
-    for (int k = lane; k < cells; k += kWave) {
-        const float x = g[k];
-        if (x > best) {
-            best = x;
-            bidx = k;
+    for (int k0 = lane; k0 < cells; k0 += kPeakBatch * kWave) {
+        float x[kPeakBatch];
+#pragma unroll
+        for (int j = 0; j < kPeakBatch; ++j) {
+            const int k = k0 + j * kWave;
+            x[j] = k < cells ? g[k] : -INFINITY;
         }
+#pragma unroll
+        for (int j = 0; j < kPeakBatch; ++j)
+            if (x[j] > best) {
+                best = x[j];
+                bidx = k0 + j * kWave;
+            }
     }
     wave_argmax(best, bidx);
     const bool none = (bidx == 0x7fffffff);
@@ -763,13 +773,21 @@ __global__ __launch_bounds__(256) void k_coarse_peak(const float *__restrict__ g
 
     double sum = 0.0;
     int count = 0;
-    for (int k = lane; k < cells; k += kWave) {
-        const int u = k / P, v = k - u * P;
-        if (abs(u - row) <= exclude && abs(v - col) <= exclude) continue;
-        const float x = g[k];
-        if (x != x) continue;
-        sum += (double)x * (double)x;
-        ++count;
+    for (int k0 = lane; k0 < cells; k0 += kPeakBatch * kWave) {
+        float x[kPeakBatch];
+#pragma unroll
+        for (int j = 0; j < kPeakBatch; ++j) {
+            const int k = k0 + j * kWave;
+            x[j] = k < cells ? g[k] : __builtin_nanf("");
+        }
+#pragma unroll
+        for (int j = 0; j < kPeakBatch; ++j) {
+            const int k = k0 + j * kWave;
+            const int u = k / P, v = k - u * P;
+            if (x[j] != x[j] || (abs(u - row) <= exclude && abs(v - col) <= exclude)) continue;
+            sum += (double)x[j] * (double)x[j];
+            ++count;
+        }
     }
 #pragma unroll
     for (int d = kWave / 2; d > 0; d >>= 1) {
@@ -1937,71 +1955,121 @@ __global__ __launch_bounds__(kGatherPrepThreads, 4) void k_gather_prep_mixed(
 // ---------------------------------------------------------------------------
 constexpr int kZoomThreads = 256;
 
+// the first maximum over a workgroup of kZoomThreads: every thread gets it
+__device__ inline void block_argmax(float *rb, int *ri, float &best, int &bidx)
+{
+    wave_argmax(best, bidx);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        rb[wave] = best;
+        ri[wave] = bidx;
+    }
+    __syncthreads();
+    best = rb[0];
+    bidx = ri[0];
+    for (int w = 1; w < kZoomThreads / kWave; ++w) better(best, bidx, rb[w], ri[w]);
+}
+
+// LDS of k_zoom (bytes); the Dirichlet weights of the two passes are staged
+// when `stage_d`
+__host__ __device__ inline size_t zoom_lds_bytes(int R, int zf, bool stage_d)
+{
+    const size_t span = 2 * zf + 1;
+    return sizeof(double) * (span * R + span * span + (stage_d ? 2 * span * R : 0)) +
+           sizeof(float) * (size_t)R * R;
+}
+
 __global__ __launch_bounds__(kZoomThreads) void k_zoom(
     const float *__restrict__ gamma_refined, int R, int zf, const double *__restrict__ D,
-    const int2 *__restrict__ window, const float *__restrict__ coarse_peak, int margin, int rf,
-    float *__restrict__ offsets, float *__restrict__ refined_peak, int2 *__restrict__ zoom_loc)
+    int stage_d, const int2 *__restrict__ window, const float *__restrict__ coarse_peak,
+    int margin, int rf, float *__restrict__ offsets, float *__restrict__ refined_peak,
+    int2 *__restrict__ zoom_loc)
 {
     extern __shared__ double zs[];
     const int pair = blockIdx.x;
     const int t = threadIdx.x;
     const int Z = zf * R;
     const int span = 2 * zf + 1;
-    double *g = zs;                   // R*R
-    double *H = g + R * R;            // span*R
-    double *f = H + span * R;         // span*span
-    __shared__ float rb[kZoomThreads];
-    __shared__ int ri[kZoomThreads];
+    double *H = zs;                   // [span][R]
+    double *f = H + span * R;         // [span][span]
+    double *Dz = f + span * span;     // [span][R] row weights, [span][R] column weights
+    double *Dq = Dz + span * R;
+    float *g = reinterpret_cast<float *>(stage_d ? Dq + span * R : Dz);   // [R][R]
+    __shared__ float rb[kZoomThreads / kWave];
+    __shared__ int ri[kZoomThreads / kWave];
 
+    // the refined surface (NaN read as 0 by the zoom, never a maximum)
     const float *gr = gamma_refined + (size_t)pair * R * R;
     float best = -INFINITY;
     int bidx = 0x7fffffff;
     for (int k = t; k < R * R; k += kZoomThreads) {
         const float x = gr[k];
-        g[k] = (x != x) ? 0.0 : (double)x;
+        g[k] = (x != x) ? 0.0f : x;
         if (x > best) {
             best = x;
             bidx = k;
         }
     }
-    rb[t] = best;
-    ri[t] = bidx;
-    __syncthreads();
-    for (int s = kZoomThreads / 2; s > 0; s >>= 1) {
-        if (t < s) {
-            float b0 = rb[t];
-            int i0 = ri[t];
-            better(b0, i0, rb[t + s], ri[t + s]);
-            rb[t] = b0;
-            ri[t] = i0;
-        }
-        __syncthreads();
-    }
-    const bool none = (ri[0] == 0x7fffffff);
-    const int a = none ? 0 : ri[0] / R;
-    const int bcol = none ? 0 : ri[0] - a * R;
+    block_argmax(rb, ri, best, bidx);
+    const bool none = (bidx == 0x7fffffff);
+    const int a = none ? 0 : bidx / R;
+    const int bcol = none ? 0 : bidx - a * R;
 
     const int p0 = max(zf * a - zf, 0), p1 = min(zf * a + zf, Z - 1);
     const int q0 = max(zf * bcol - zf, 0), q1 = min(zf * bcol + zf, Z - 1);
     const int np = p1 - p0 + 1, nq = q1 - q0 + 1;
+    // Dirichlet weights of zoomed row p0 + ip and surface row aa (and columns)
+    if (stage_d) {
+        for (int k = t; k < np * R; k += kZoomThreads) {
+            const int ip = k / R, aa = k - ip * R;
+            Dz[k] = D[(p0 + ip) - zf * aa + Z];
+        }
+        for (int k = t; k < nq * R; k += kZoomThreads) {
+            const int iq = k / R, bb = k - iq * R;
+            Dq[k] = D[(q0 + iq) - zf * bb + Z];
+        }
+    }
+    __syncthreads();
     for (int k = t; k < np * R; k += kZoomThreads) {
         const int ip = k / R, bb = k - ip * R;
         double acc = 0.0;
-        for (int aa = 0; aa < R; ++aa) acc += D[(p0 + ip) - zf * aa + Z] * g[aa * R + bb];
+        if (stage_d)
+            for (int aa = 0; aa < R; ++aa) acc += Dz[ip * R + aa] * (double)g[aa * R + bb];
+        else
+            for (int aa = 0; aa < R; ++aa) acc += D[(p0 + ip) - zf * aa + Z] * (double)g[aa * R + bb];
         H[k] = acc;
     }
     __syncthreads();
     for (int k = t; k < np * nq; k += kZoomThreads) {
         const int ip = k / nq, iq = k - ip * nq;
         double acc = 0.0;
-        for (int bb = 0; bb < R; ++bb) acc += H[ip * R + bb] * D[(q0 + iq) - zf * bb + Z];
+        if (stage_d)
+            for (int bb = 0; bb < R; ++bb) acc += H[ip * R + bb] * Dq[iq * R + bb];
+        else
+            for (int bb = 0; bb < R; ++bb) acc += H[ip * R + bb] * D[(q0 + iq) - zf * bb + Z];
         f[k] = acc;
     }
     __syncthreads();
+    // the first maximum of the zoomed cells: wave 0
+    if (t >= kWave) return;
+    int bi = 0x7fffffff;
+    double fb = -INFINITY;
+    for (int k = t; k < np * nq; k += kWave)
+        if (f[k] > fb) {
+            fb = f[k];
+            bi = k;
+        }
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) {
+        const double ob = __shfl_xor(fb, d, kWave);
+        const int oi = __shfl_xor(bi, d, kWave);
+        if (ob > fb || (ob == fb && oi < bi)) {
+            fb = ob;
+            bi = oi;
+        }
+    }
     if (t != 0) return;
-    int bi = 0;
-    for (int k = 1; k < np * nq; ++k)
-        if (f[k] > f[bi]) bi = k;
+    if (bi == 0x7fffffff) bi = 0;   // every zoomed cell NaN: the serial scan's pick
     const int ip = bi / nq, iq = bi - ip * nq;
     double dp = 0.0, dq = 0.0;
     if (ip > 0 && ip < np - 1) {
@@ -2334,15 +2402,22 @@ hipError_t launch_gather_prep_mixed(hipStream_t s, const PairDesc *desc, int pai
     return hipGetLastError();
 }
 
+constexpr size_t kZoomLdsStage = 48 * 1024;   // LDS budget with staged weights
+constexpr size_t kZoomLdsMax = 160 * 1024;
+
+bool zoom_fits(int R, int zf) { return zoom_lds_bytes(R, zf, false) <= kZoomLdsMax; }
+
 hipError_t launch_zoom(hipStream_t s, const float *gamma_refined, int pairs, int R, int zf,
                        const double *dirichlet, const int2 *window, const float *coarse_peak,
                        int margin, int rf, float *offsets, float *refined_peak, int2 *zoom_loc)
 {
-    const int span = 2 * zf + 1;
-    const size_t lds = sizeof(double) * ((size_t)R * R + (size_t)span * R + (size_t)span * span);
+    if (!zoom_fits(R, zf)) return hipErrorInvalidValue;
+    // the Dirichlet weights of both passes go to LDS when they fit beside the rest
+    const bool stage_d = zoom_lds_bytes(R, zf, true) <= kZoomLdsStage;
+    const size_t lds = zoom_lds_bytes(R, zf, stage_d);
     hipLaunchKernelGGL(k_zoom, dim3(pairs), dim3(kZoomThreads), lds, s, gamma_refined, R, zf,
-                       dirichlet, window, coarse_peak, margin, rf, offsets, refined_peak,
-                       zoom_loc);
+                       dirichlet, (int)stage_d, window, coarse_peak, margin, rf, offsets,
+                       refined_peak, zoom_loc);
     return hipGetLastError();
 }
 

@@ -275,6 +275,9 @@ int ampcor_hip_plan_create(int device, size_t pairs, size_t ref_lines, size_t re
     const size_t R = Wr - Cr + 1;
     if (Wr > 512) return fail(AMPCOR_HIP_EINVAL, "refined window edge must be <= 512");
     if (R > 129) return fail(AMPCOR_HIP_EINVAL, "refined surface too large (2 rf rm + 1 > 129)");
+    if (!zoom_fits((int)R, (int)zoom_factor))
+        return fail(AMPCOR_HIP_EINVAL, "zoom factor too large for this refined surface (the zoom "
+                                       "of one pair must fit a workgroup's 160 KiB of LDS)");
     if (precision < AMPCOR_HIP_FP32 || precision > AMPCOR_HIP_FP8)
         return fail(AMPCOR_HIP_EINVAL, "unknown precision (FP32, BF16, FP16, BF16X3, FP16X3 or FP8)");
     if (precision != AMPCOR_HIP_FP32 && (C % 16 || C * refine_factor > 256))
@@ -756,8 +759,8 @@ int ampcor_hip_stage_locations(ampcor_hip_plan *plan, int32_t *coarse, int32_t *
     return copy_out(plan, zoomed, plan->zoom_loc.ptr, bytes);
 }
 
-// Times candidate tilings of the two contractions on the data of the last
-// adjust and keeps the fastest.  The FP32 coarse contraction only varies its
+// Times candidate tilings of the two contractions, each followed by its
+// normalizer, on the data of the last adjust and keeps the fastest.  The FP32 coarse contraction only varies its
 // register block, which leaves its arithmetic order (and results) unchanged.
 int ampcor_hip_plan_autotune(ampcor_hip_plan *plan, int reps)
 {
@@ -779,6 +782,42 @@ int ampcor_hip_plan_autotune(ampcor_hip_plan *plan, int reps)
         HIP_TRY(hipEventSynchronize(e1), "tuning");
         HIP_TRY(hipEventElapsedTime(ms, e0, e1), "tuning");
         return AMPCOR_HIP_OK;
+    };
+    // the normalizer reads every partial of the split contraction: candidate
+    // splits are timed together with it, launched as adjust launches it
+    const bool fuse_c = plan->mixed && normalize_fuses(plan->W, plan->C);
+    const bool fuse_r = normalize_fuses(plan->Wr, plan->Cr);
+    const bool quant_sums_r = plan->mixed && !plan->split && fuse_r &&
+                              upsample_fuses_quantization(plan->C, plan->rf) &&
+                              upsample_fuses_quantization(plan->E, plan->rf);
+    auto normalize_coarse = [&](int nsplit) -> hipError_t {
+        if (plan->fp8)
+            return launch_normalize_fused(s, kFuseFP8, plan->tgt_q.ptr, N, plan->W, plan->C,
+                                          plan->partial.ptr, nsplit, nullptr, plan->mstats.ptr,
+                                          plan->gamma.ptr);
+        if (fuse_c)
+            return launch_normalize_fused(s, fp16 ? kFuseFP16 : kFuseBF16, plan->tgt_q.ptr, N,
+                                          plan->W, plan->C, plan->partial.ptr, nsplit, nullptr,
+                                          plan->mstats.ptr, plan->gamma.ptr);
+        return launch_normalize_mixed(s, plan->partial.ptr, nsplit, plan->sums.ptr,
+                                      plan->mstats.ptr, N, plan->C, plan->W, plan->gamma.ptr);
+    };
+    auto normalize_refined = [&](int nsplit) -> hipError_t {
+        if (!plan->mixed)
+            return fuse_r ? launch_normalize_fused(s, kFuseFP32, plan->amp_tgt_r.ptr, N, plan->Wr,
+                                                   plan->Cr, plan->partial_r.ptr, nsplit,
+                                                   plan->stats_r.ptr, nullptr, plan->gamma_r.ptr)
+                          : launch_normalize(s, plan->partial_r.ptr, nsplit, plan->sums_r.ptr,
+                                             plan->stats_r.ptr, N, plan->Cr, plan->Wr,
+                                             plan->gamma_r.ptr);
+        if (fuse_r)
+            return launch_normalize_fused(
+                s, quant_sums_r ? (fp16 ? kFuseFP16 : kFuseBF16) : kFuseMixed,
+                quant_sums_r ? (const void *)plan->tgt_q_r.ptr : (const void *)plan->amp_tgt_r.ptr,
+                N, plan->Wr, plan->Cr, plan->partial_r.ptr, nsplit, nullptr, plan->mstats_r.ptr,
+                plan->gamma_r.ptr);
+        return launch_normalize_mixed(s, plan->partial_r.ptr, nsplit, plan->sums_r.ptr,
+                                      plan->mstats_r.ptr, N, plan->Cr, plan->Wr, plan->gamma_r.ptr);
     };
     int rc = AMPCOR_HIP_OK;
     if (!plan->mixed) {
@@ -803,10 +842,11 @@ int ampcor_hip_plan_autotune(ampcor_hip_plan *plan, int reps)
                 CorrelateShape sh = choose_correlate_shape(plan->Cr, plan->R, false, bu, ns);
                 if (sh.lds_bytes > 150 * 1024) continue;
                 float ms;
-                if ((rc = timeit([&] { return launch_correlate(s, sh, plan->rT_r.ptr,
-                                                               plan->amp_tgt_r.ptr, N, plan->Cr,
-                                                               plan->Wr, plan->partial_r.ptr); },
-                                 &ms)))
+                if ((rc = timeit([&] {
+                         hipError_t e = launch_correlate(s, sh, plan->rT_r.ptr, plan->amp_tgt_r.ptr,
+                                                         N, plan->Cr, plan->Wr, plan->partial_r.ptr);
+                         return e != hipSuccess ? e : normalize_refined(sh.nsplit);
+                     }, &ms)))
                     break;
                 if (ms < best) best = ms, pick = sh;
             }
@@ -831,10 +871,13 @@ int ampcor_hip_plan_autotune(ampcor_hip_plan *plan, int reps)
                 MixedShape sh = choose_mixed_shape(st.C, st.P, ns, st.split);
                 if (sh.nsplit > st.maxsplit || sh.lds_bytes > 150 * 1024) continue;
                 float ms;
-                if ((rc = timeit([&] { return launch_correlate_mixed(s, sh, st.cq, st.tq, N, st.C,
-                                                                     st.W, fp16, st.partial, st.cl,
-                                                                     st.tl); },
-                                 &ms)))
+                const bool coarse = st.shape == &plan->mixed_coarse;
+                if ((rc = timeit([&] {
+                         hipError_t e = launch_correlate_mixed(s, sh, st.cq, st.tq, N, st.C, st.W,
+                                                               fp16, st.partial, st.cl, st.tl);
+                         if (e != hipSuccess) return e;
+                         return coarse ? normalize_coarse(sh.nsplit) : normalize_refined(sh.nsplit);
+                     }, &ms)))
                     break;
                 if (ms < best) best = ms, pick = sh;
             }
@@ -849,9 +892,11 @@ int ampcor_hip_plan_autotune(ampcor_hip_plan *plan, int reps)
                 Fp8Shape sh = choose_fp8_shape(plan->C, plan->P, ns);
                 if (sh.nsplit > kTuneMaxSplitCoarse || sh.lds_bytes > 150 * 1024) continue;
                 float ms;
-                if ((rc = timeit([&] { return launch_correlate_fp8(s, sh, cq8, tq8, N, plan->C,
-                                                                   plan->W, plan->partial.ptr); },
-                                 &ms)))
+                if ((rc = timeit([&] {
+                         hipError_t e = launch_correlate_fp8(s, sh, cq8, tq8, N, plan->C, plan->W,
+                                                             plan->partial.ptr);
+                         return e != hipSuccess ? e : normalize_coarse(sh.nsplit);
+                     }, &ms)))
                     break;
                 if (ms < best) best = ms, pick = sh;
             }

@@ -433,6 +433,20 @@ __device__ inline float fuse_unpack(const uint4 &w, int q)
     }
 }
 
+// 1 / sqrt(p) for p > 0 in FP64: the hardware estimate refined by two Newton
+// steps, y <- y + y (1 - p y^2) / 2 (each at least doubles the correct bits;
+// the result is rounded to FP32 by the callers)
+__device__ inline double rsqrt_fp64(double p)
+{
+    double y = __builtin_amdgcn_rsq(p);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const double e = fma(-p * y, y, 1.0);
+        y = fma(0.5 * y, e, y);
+    }
+    return y;
+}
+
 // LDS-only barrier: the waves of a workgroup exchange data through LDS alone
 // (global memory is read-only input or write-once output here), so the
 // global loads in flight -- the next block's rows -- need not complete
@@ -705,7 +719,7 @@ __global__ __launch_bounds__(kFuseThreads, COLS == 1 && JO < 9 ? 4 : 2) void k_n
                                          ? (double)num[j] * inv_ab + (mean_t - mu) * sumr
                                          : (double)num[j] - mu * sumr;
                 float g = __builtin_nanf("");
-                if (var > 0.0 && chip2 > 0.0) g = (float)(num_c / sqrt(chip2 * var));
+                if (var > 0.0 && chip2 > 0.0) g = (float)(num_c * rsqrt_fp64(chip2 * var));
                 gamma[(size_t)pair * P * P + (size_t)u * P + v] = g;
             }
         }

@@ -1342,16 +1342,18 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2h(
     __syncthreads();
 
     // stage 1: T = H x, tiles (mt, ct); a wave keeps its tiles until every
-    // wave is done with the column-major copy
+    // wave is done with the column-major copy.  Only the NT x NT tiles that
+    // hold samples are computed: the rest of that copy (x[n][a] with n >= N)
+    // is zero, which is what T holds there and what the products below read
     constexpr int kMaxT = (kUp2hMaxN / 16) * (kUp2hMaxN / 16) / kUp2Waves + 1;
     floatx4 tR[kMaxT], tI[kMaxT];
-    const int ntile = MT * MT;
+    const int NT = (N + 15) / 16, ntile = NT * NT;
 #pragma unroll
     for (int q = 0; q < kMaxT; ++q) {
         tR[q] = tI[q] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
         const int tile = wave + kUp2Waves * q;
         if (tile >= ntile) continue;
-        const int mt = tile / MT, ct = tile - mt * MT;
+        const int mt = tile / NT, ct = tile - mt * NT;
         const _Float16 *br = cr + (16 * ct + lm) * Ls + 8 * lk;
         const _Float16 *bi = ci + (16 * ct + lm) * Ls + 8 * lk;
         uint4 fa[kUp2hMaxN / 32];
@@ -1371,7 +1373,7 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2h(
     for (int q = 0; q < kMaxT; ++q) {
         const int tile = wave + kUp2Waves * q;
         if (tile >= ntile) continue;
-        const int mt = tile / MT, ct = tile - mt * MT;
+        const int mt = tile / NT, ct = tile - mt * NT;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int at = (16 * mt + 4 * lk + i) * Ls + 16 * ct + lm;
@@ -1409,7 +1411,7 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2h(
     // stage 2: odd columns, Y_s = T_s H^T for T_0 = x and T_1 = T, then the
     // four phases of each output 2 x 2 block
     for (int tile = wave; tile < ntile; tile += kUp2Waves) {
-        const int at = tile / MT, bt = tile - at * MT;
+        const int at = tile / NT, bt = tile - at * NT;
         const int arow = (16 * at + lm) * Ls + 8 * lk;
         floatx4 xR = {0.0f, 0.0f, 0.0f, 0.0f}, xI = xR, yR = xR, yI = xR;
         uint4 fb[kUp2hMaxN / 32];

@@ -49,7 +49,7 @@ def parse():
     ap.add_argument("--precision", default="bf16",
                     help="form whose throughput is the headline value (fp32, bf16, fp16, fp8); "
                          "the FP32 drop-in form is always timed beside it")
-    ap.add_argument("--extra-forms", default="bf16x3,fp8",
+    ap.add_argument("--extra-forms", default="fp16,bf16x3,fp8",
                     help="further forms timed beside FP32 and the headline (comma list, or '')")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target duration of the CPU-baseline sample")

@@ -84,3 +84,15 @@ def test_split_refinement_is_closer_to_fp64(lib):
     split = np.abs(run(lib, s, "bf16x3")["offsets"] - exact["offsets"]).max()
     print(f"bf16 {plain:.2e} px, bf16x3 {split:.2e} px", flush=True)
     assert split <= plain
+
+
+@pytest.mark.parametrize("precision", ["bf16", "fp8"])
+def test_mixed_window_not_a_multiple_of_8(lib, precision):
+    """W = 52: window rows are 4-byte but not 16-byte aligned (the contractions'
+    dword staging, the gathers' odd-chunk paths)."""
+    s = synth.shifted_scene(chip=32, margin=10, refine_margin=8)
+    hip, exact = run(lib, s, precision), fp64(s)
+    err = np.abs(hip["offsets"] - exact["offsets"])
+    print(f"{precision} W=52: max |offset - fp64| = {err.max():.2e} px", flush=True)
+    assert err.max() <= MIXED_OFFSET_TOL
+    assert np.mean(np.all(hip["coarse_loc"] == exact["coarse_loc"], axis=1)) >= 0.98

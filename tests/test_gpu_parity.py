@@ -121,3 +121,15 @@ def test_float32_amplitude_rasters(lib):
     hip = run_hip(lib, ref, tgt, s.ref_begin, s.tgt_begin, s.chip, s.window, 2, 8, 8)
     twin, exact = oracles(s)
     check_against_oracles(hip, twin, exact)
+
+
+def test_wide_refined_surface_unstaged_zoom(lib):
+    """R = 2 rf rm + 1 = 125 with zoom 4: the zoom's Dirichlet weights no longer
+    fit its staging budget and are read from global memory (the oracle takes
+    R <= 128)."""
+    s = synth.shifted_scene(size=384, grid=(3, 3), chip=32, margin=32, refine_factor=2,
+                            refine_margin=31, zoom_factor=4)
+    hip = run_hip(lib, s.reference, s.target, s.ref_begin, s.tgt_begin, s.chip, s.window,
+                  s.refine_factor, s.refine_margin, s.zoom_factor)
+    twin, exact = oracles(s)
+    check_against_oracles(hip, twin, exact)

@@ -1986,18 +1986,19 @@ __device__ inline void block_argmax(float *rb, int *ri, float &best, int &bidx)
     for (int w = 1; w < kZoomThreads / kWave; ++w) better(best, bidx, rb[w], ri[w]);
 }
 
-// LDS of k_zoom (bytes); the Dirichlet weights of the two passes are staged
+// LDS of k_zoom (bytes); the Dirichlet table (2 zf R + 1 weights) is staged
 // when `stage_d`
 __host__ __device__ inline size_t zoom_lds_bytes(int R, int zf, bool stage_d)
 {
     const size_t span = 2 * zf + 1;
-    return sizeof(double) * (span * R + span * span + (stage_d ? 2 * span * R : 0)) +
+    return sizeof(double) * (span * R + span * span + (stage_d ? 2 * (size_t)zf * R + 1 : 0)) +
            sizeof(float) * (size_t)R * R;
 }
 
+template <bool STAGE>
 __global__ __launch_bounds__(kZoomThreads) void k_zoom(
     const float *__restrict__ gamma_refined, int R, int zf, const double *__restrict__ D,
-    int stage_d, const int2 *__restrict__ window, const float *__restrict__ coarse_peak,
+    const int2 *__restrict__ window, const float *__restrict__ coarse_peak,
     int margin, int rf, float *__restrict__ offsets, float *__restrict__ refined_peak,
     int2 *__restrict__ zoom_loc)
 {
@@ -2008,12 +2009,15 @@ __global__ __launch_bounds__(kZoomThreads) void k_zoom(
     const int span = 2 * zf + 1;
     double *H = zs;                   // [span][R]
     double *f = H + span * R;         // [span][span]
-    double *Dz = f + span * span;     // [span][R] row weights, [span][R] column weights
-    double *Dq = Dz + span * R;
-    float *g = reinterpret_cast<float *>(stage_d ? Dq + span * R : Dz);   // [R][R]
+    double *Dt = f + span * span;     // the Dirichlet table, [2 Z + 1], when staged
+    float *g = reinterpret_cast<float *>(STAGE ? Dt + 2 * Z + 1 : Dt);   // [R][R]
     __shared__ float rb[kZoomThreads / kWave];
     __shared__ int ri[kZoomThreads / kWave];
 
+    // the Dirichlet table (loads in flight with the surface's)
+    if (STAGE)
+        for (int k = t; k < 2 * Z + 1; k += kZoomThreads) Dt[k] = D[k];
+    const double *Dw = STAGE ? Dt : D;   // weights, LDS or global
     // the refined surface (NaN read as 0 by the zoom, never a maximum)
     const float *gr = gamma_refined + (size_t)pair * R * R;
     float best = -INFINITY;
@@ -2034,35 +2038,19 @@ __global__ __launch_bounds__(kZoomThreads) void k_zoom(
     const int p0 = max(zf * a - zf, 0), p1 = min(zf * a + zf, Z - 1);
     const int q0 = max(zf * bcol - zf, 0), q1 = min(zf * bcol + zf, Z - 1);
     const int np = p1 - p0 + 1, nq = q1 - q0 + 1;
-    // Dirichlet weights of zoomed row p0 + ip and surface row aa (and columns)
-    if (stage_d) {
-        for (int k = t; k < np * R; k += kZoomThreads) {
-            const int ip = k / R, aa = k - ip * R;
-            Dz[k] = D[(p0 + ip) - zf * aa + Z];
-        }
-        for (int k = t; k < nq * R; k += kZoomThreads) {
-            const int iq = k / R, bb = k - iq * R;
-            Dq[k] = D[(q0 + iq) - zf * bb + Z];
-        }
-    }
-    __syncthreads();
     for (int k = t; k < np * R; k += kZoomThreads) {
         const int ip = k / R, bb = k - ip * R;
+        const double *w = Dw + (p0 + ip) + Z;   // weight of surface row aa: w[-zf aa]
         double acc = 0.0;
-        if (stage_d)
-            for (int aa = 0; aa < R; ++aa) acc += Dz[ip * R + aa] * (double)g[aa * R + bb];
-        else
-            for (int aa = 0; aa < R; ++aa) acc += D[(p0 + ip) - zf * aa + Z] * (double)g[aa * R + bb];
+        for (int aa = 0; aa < R; ++aa) acc += w[-zf * aa] * (double)g[aa * R + bb];
         H[k] = acc;
     }
     __syncthreads();
     for (int k = t; k < np * nq; k += kZoomThreads) {
         const int ip = k / nq, iq = k - ip * nq;
+        const double *w = Dw + (q0 + iq) + Z;
         double acc = 0.0;
-        if (stage_d)
-            for (int bb = 0; bb < R; ++bb) acc += H[ip * R + bb] * Dq[iq * R + bb];
-        else
-            for (int bb = 0; bb < R; ++bb) acc += H[ip * R + bb] * D[(q0 + iq) - zf * bb + Z];
+        for (int bb = 0; bb < R; ++bb) acc += H[ip * R + bb] * w[-zf * bb];
         f[k] = acc;
     }
     __syncthreads();
@@ -2431,9 +2419,14 @@ hipError_t launch_zoom(hipStream_t s, const float *gamma_refined, int pairs, int
     // the Dirichlet weights of both passes go to LDS when they fit beside the rest
     const bool stage_d = zoom_lds_bytes(R, zf, true) <= kZoomLdsStage;
     const size_t lds = zoom_lds_bytes(R, zf, stage_d);
-    hipLaunchKernelGGL(k_zoom, dim3(pairs), dim3(kZoomThreads), lds, s, gamma_refined, R, zf,
-                       dirichlet, (int)stage_d, window, coarse_peak, margin, rf, offsets,
-                       refined_peak, zoom_loc);
+    if (stage_d)
+        hipLaunchKernelGGL(k_zoom<true>, dim3(pairs), dim3(kZoomThreads), lds, s, gamma_refined, R,
+                           zf, dirichlet, window, coarse_peak, margin, rf, offsets, refined_peak,
+                           zoom_loc);
+    else
+        hipLaunchKernelGGL(k_zoom<false>, dim3(pairs), dim3(kZoomThreads), lds, s, gamma_refined,
+                           R, zf, dirichlet, window, coarse_peak, margin, rf, offsets,
+                           refined_peak, zoom_loc);
     return hipGetLastError();
 }
 

@@ -11,7 +11,8 @@ Fp8Shape choose_fp8_shape(int C, int P, int nsplit)
     const int tiles = (P + 15) / 16;
     const int vblocks = (tiles + fp8_detail::kMaxVT - 1) / fp8_detail::kMaxVT;
     s.wg_vt = (tiles + vblocks - 1) / vblocks;
-    s.wg_ut = tiles < fp8_detail::kMaxWaves ? tiles : fp8_detail::kMaxWaves;
+    const int ublocks = (tiles + fp8_detail::kMaxWaves - 1) / fp8_detail::kMaxWaves;
+    s.wg_ut = (tiles + ublocks - 1) / ublocks;   // waves, spread evenly over the blocks
     // chip rows in steps of eight: ranges of a multiple of 8 rows (zero past C)
     const int c8 = (C + 7) / 8 * 8;
     auto can_split = [&](int ns) { return c8 % (8 * ns) == 0; };

@@ -192,8 +192,10 @@ MixedShape choose_mixed_shape(int C, int P, int nsplit, bool split)
     // v-tiles per workgroup: at most kMaxVT, spread evenly over the blocks
     const int vblocks = (tiles + mixed_detail::kMaxVT - 1) / mixed_detail::kMaxVT;
     s.wg_vt = (tiles + vblocks - 1) / vblocks;
+    // u-tiles (waves) per workgroup: at most max_waves, spread evenly over the blocks
     const int max_ut = mixed_detail::max_waves(s.nk);
-    s.wg_ut = tiles < max_ut ? tiles : max_ut;
+    const int ublocks = (tiles + max_ut - 1) / max_ut;
+    s.wg_ut = (tiles + ublocks - 1) / ublocks;
     s.nsplit = 1;
     const size_t budget = 120 * 1024;
     auto lds = [&](int ut, int ns) { return mixed_lds_bytes(C, s.nk, s.wg_vt, ut, ns, split); };

@@ -92,6 +92,8 @@ bool upsample_fuses_quantization(int N, int rf);
 // FP16 MFMA fragments (upsample_fragments); the products then run on FP16
 // matrix cores instead of FP32 ones
 int upsample_fragment_count(int N);
+// whether a tile edge N can use them (up to 96 for complex rasters, 160 for real)
+bool upsample_uses_fragments(int N);
 // tiles with 96 < N <= 512 (rf = 2) upsample in two banded passes through
 // global scratch: T [pairs][N][N], band and row statistics [pairs][ceil(N / 16)]
 struct UpsampleScratch {

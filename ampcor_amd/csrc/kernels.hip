@@ -1207,7 +1207,11 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2(
 // stays well below the BF16/FP16 rounding of the operands written for the
 // contraction.
 // ---------------------------------------------------------------------------
-constexpr int kUp2hMaxN = 96;
+constexpr int kUp2hMaxN = 96;       // complex tiles
+constexpr int kUp2hMaxNReal = 160;  // real tiles: two planes instead of four
+
+// largest tile edge (padded to 32) k_upsample2h takes for the raster type
+__host__ __device__ constexpr int up2h_max_n(int dtype) { return dtype == 1 ? kUp2hMaxNReal : kUp2hMaxN; }
 
 typedef _Float16 halfx8_t __attribute__((ext_vector_type(8)));
 typedef _Float16 halfx2_t __attribute__((ext_vector_type(2)));
@@ -1220,9 +1224,12 @@ __device__ inline floatx4 mfma_f16(uint4 a, uint4 b, floatx4 c)
                                                   __builtin_bit_cast(halfx8_t, b), c, 0, 0, 0);
 }
 
-// LDS of k_upsample2h (bytes): four FP16 planes [Np][Np + 8]
+// LDS of k_upsample2h (bytes): four FP16 planes [Np][Np + 8], two for real tiles
 __host__ __device__ inline size_t up2h_plane_bytes(int Np) { return (size_t)Np * (Np + 8) * 2; }
-__host__ __device__ inline size_t up2h_lds_bytes(int Np) { return 4 * up2h_plane_bytes(Np); }
+__host__ __device__ inline size_t up2h_lds_bytes(int Np, int dtype)
+{
+    return (dtype == 1 ? 2 : 4) * up2h_plane_bytes(Np);
+}
 
 template <int DTYPE, int QMODE>
 __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2h(
@@ -1230,16 +1237,19 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2h(
     const uint4 *__restrict__ hfrag, float *__restrict__ amp, uint16_t *__restrict__ qout, int cs,
     MixedStats *__restrict__ stats)
 {
+    // real tiles (DTYPE 1) have no imaginary planes, and may be larger
+    constexpr bool kReal = DTYPE == 1;
+    constexpr int kMaxN = up2h_max_n(DTYPE);
     extern __shared__ uint4 lds4[];
     __shared__ double red[kUp2Waves], red2[kUp2Waves];
     __shared__ float redm[kUp2Waves];
     unsigned char *lb = reinterpret_cast<unsigned char *>(lds4);
     const int Ls = Np + 8;
     const size_t pb = up2h_plane_bytes(Np);
-    _Float16 *xr = reinterpret_cast<_Float16 *>(lb);            // x, row-major
-    _Float16 *xi = reinterpret_cast<_Float16 *>(lb + pb);
-    _Float16 *cr = reinterpret_cast<_Float16 *>(lb + 2 * pb);   // x column-major, then T row-major
-    _Float16 *ci = reinterpret_cast<_Float16 *>(lb + 3 * pb);
+    _Float16 *xr = reinterpret_cast<_Float16 *>(lb);                         // x, row-major
+    _Float16 *cr = reinterpret_cast<_Float16 *>(lb + pb);                    // x column-major, then T row-major
+    _Float16 *xi = kReal ? nullptr : reinterpret_cast<_Float16 *>(lb + 2 * pb);
+    _Float16 *ci = kReal ? nullptr : reinterpret_cast<_Float16 *>(lb + 3 * pb);
     const int MT = Np / 16, KC = Np / 32;
     const int pair = blockIdx.x;
     const int lane = threadIdx.x & (kWave - 1);
@@ -1271,7 +1281,7 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2h(
     // blocks at a time (block row lane / 8, block column lane % 8), so each
     // row segment a wave reads is 128 contiguous bytes and the row- and
     // column-major LDS writes below hit 64 different banks
-    constexpr int kMaxJ = (kUp2hMaxN / 16) * (kUp2hMaxN / 16) / kUp2Waves + 1;
+    constexpr int kMaxJ = (kMaxN / 16) * (kMaxN / 16) / kUp2Waves + 1;
     const int TB = Np / 16, ntb = TB * TB;
     const int rsub = 2 * (lane >> 3), csub = 2 * (lane & 7);
     float bre[kMaxJ][4], bim[kMaxJ][4];
@@ -1332,12 +1342,14 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2h(
         const float *v = bre[j], *w = bim[j];
         *reinterpret_cast<uint32_t *>(xr + r * Ls + c) = pack(v[0] * sx, v[1] * sx);
         *reinterpret_cast<uint32_t *>(xr + (r + 1) * Ls + c) = pack(v[2] * sx, v[3] * sx);
-        *reinterpret_cast<uint32_t *>(xi + r * Ls + c) = pack(w[0] * sx, w[1] * sx);
-        *reinterpret_cast<uint32_t *>(xi + (r + 1) * Ls + c) = pack(w[2] * sx, w[3] * sx);
         *reinterpret_cast<uint32_t *>(cr + c * Ls + r) = pack(v[0] * sx, v[2] * sx);
         *reinterpret_cast<uint32_t *>(cr + (c + 1) * Ls + r) = pack(v[1] * sx, v[3] * sx);
-        *reinterpret_cast<uint32_t *>(ci + c * Ls + r) = pack(w[0] * sx, w[2] * sx);
-        *reinterpret_cast<uint32_t *>(ci + (c + 1) * Ls + r) = pack(w[1] * sx, w[3] * sx);
+        if constexpr (!kReal) {
+            *reinterpret_cast<uint32_t *>(xi + r * Ls + c) = pack(w[0] * sx, w[1] * sx);
+            *reinterpret_cast<uint32_t *>(xi + (r + 1) * Ls + c) = pack(w[2] * sx, w[3] * sx);
+            *reinterpret_cast<uint32_t *>(ci + c * Ls + r) = pack(w[0] * sx, w[2] * sx);
+            *reinterpret_cast<uint32_t *>(ci + (c + 1) * Ls + r) = pack(w[1] * sx, w[3] * sx);
+        }
     }
     __syncthreads();
 
@@ -1345,7 +1357,7 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2h(
     // wave is done with the column-major copy.  Only the NT x NT tiles that
     // hold samples are computed: the rest of that copy (x[n][a] with n >= N)
     // is zero, which is what T holds there and what the products below read
-    constexpr int kMaxT = (kUp2hMaxN / 16) * (kUp2hMaxN / 16) / kUp2Waves + 1;
+    constexpr int kMaxT = (kMaxN / 16) * (kMaxN / 16) / kUp2Waves + 1;
     floatx4 tR[kMaxT], tI[kMaxT];
     const int NT = (N + 15) / 16, ntile = NT * NT;
 #pragma unroll
@@ -1355,17 +1367,18 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2h(
         if (tile >= ntile) continue;
         const int mt = tile / NT, ct = tile - mt * NT;
         const _Float16 *br = cr + (16 * ct + lm) * Ls + 8 * lk;
-        const _Float16 *bi = ci + (16 * ct + lm) * Ls + 8 * lk;
-        uint4 fa[kUp2hMaxN / 32];
+        const _Float16 *bi = kReal ? br : ci + (16 * ct + lm) * Ls + 8 * lk;
+        uint4 fa[kMaxN / 32];
 #pragma unroll
-        for (int kc = 0; kc < kUp2hMaxN / 32; ++kc)
+        for (int kc = 0; kc < kMaxN / 32; ++kc)
             if (kc < KC) fa[kc] = hfrag[(mt * KC + kc) * kWave + lane];
 #pragma unroll
-        for (int kc = 0; kc < kUp2hMaxN / 32; ++kc) {
+        for (int kc = 0; kc < kMaxN / 32; ++kc) {
             if (kc >= KC) break;
             const uint4 a = fa[kc];
             tR[q] = mfma_f16(a, *reinterpret_cast<const uint4 *>(br + 32 * kc), tR[q]);
-            tI[q] = mfma_f16(a, *reinterpret_cast<const uint4 *>(bi + 32 * kc), tI[q]);
+            if constexpr (!kReal)
+                tI[q] = mfma_f16(a, *reinterpret_cast<const uint4 *>(bi + 32 * kc), tI[q]);
         }
     }
     __syncthreads();
@@ -1378,7 +1391,7 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2h(
         for (int i = 0; i < 4; ++i) {
             const int at = (16 * mt + 4 * lk + i) * Ls + 16 * ct + lm;
             cr[at] = (_Float16)tR[q][i];
-            ci[at] = (_Float16)tI[q][i];
+            if constexpr (!kReal) ci[at] = (_Float16)tI[q][i];
         }
     }
     __syncthreads();
@@ -1414,19 +1427,21 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2h(
         const int at = tile / NT, bt = tile - at * NT;
         const int arow = (16 * at + lm) * Ls + 8 * lk;
         floatx4 xR = {0.0f, 0.0f, 0.0f, 0.0f}, xI = xR, yR = xR, yI = xR;
-        uint4 fb[kUp2hMaxN / 32];
+        uint4 fb[kMaxN / 32];
 #pragma unroll
-        for (int kc = 0; kc < kUp2hMaxN / 32; ++kc)
+        for (int kc = 0; kc < kMaxN / 32; ++kc)
             if (kc < KC) fb[kc] = hfrag[(bt * KC + kc) * kWave + lane];
 #pragma unroll
-        for (int kc = 0; kc < kUp2hMaxN / 32; ++kc) {
+        for (int kc = 0; kc < kMaxN / 32; ++kc) {
             if (kc >= KC) break;
             const uint4 b = fb[kc];
             const int o = arow + 32 * kc;
             xR = mfma_f16(*reinterpret_cast<const uint4 *>(xr + o), b, xR);
-            xI = mfma_f16(*reinterpret_cast<const uint4 *>(xi + o), b, xI);
             yR = mfma_f16(*reinterpret_cast<const uint4 *>(cr + o), b, yR);
-            yI = mfma_f16(*reinterpret_cast<const uint4 *>(ci + o), b, yI);
+            if constexpr (!kReal) {
+                xI = mfma_f16(*reinterpret_cast<const uint4 *>(xi + o), b, xI);
+                yI = mfma_f16(*reinterpret_cast<const uint4 *>(ci + o), b, yI);
+            }
         }
         const int bcol = 16 * bt + lm;
 #pragma unroll
@@ -1434,10 +1449,9 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2h(
             const int a = 16 * at + 4 * lk + i;
             if (a < N && bcol < N) {
                 const int t = a * Ls + bcol;
-                emit2(2 * a, 2 * bcol, amplitude((float)xr[t], (float)xi[t]),
-                      amplitude(xR[i], xI[i]));
-                emit2(2 * a + 1, 2 * bcol, amplitude((float)cr[t], (float)ci[t]),
-                      amplitude(yR[i], yI[i]));
+                const float xit = kReal ? 0.0f : (float)xi[t], cit = kReal ? 0.0f : (float)ci[t];
+                emit2(2 * a, 2 * bcol, amplitude((float)xr[t], xit), amplitude(xR[i], xI[i]));
+                emit2(2 * a + 1, 2 * bcol, amplitude((float)cr[t], cit), amplitude(yR[i], yI[i]));
             }
         }
     }
@@ -2289,6 +2303,8 @@ bool upsample_needs_scratch(int N, int rf)
     return rf == 2 && (N + 15) / 16 * 16 > kUp2MaxN && N <= kUpBandMaxN;
 }
 
+bool upsample_uses_fragments(int N) { return (N + 31) / 32 * 32 <= kUp2hMaxNReal; }
+
 int upsample_fragment_count(int N) { return ((N + 31) / 32 * 32 / 16) * ((N + 31) / 32) * 64 * 8; }
 
 hipError_t launch_upsample(hipStream_t s, const PairDesc *desc, int pairs, int dtype, int role,
@@ -2296,6 +2312,25 @@ hipError_t launch_upsample(hipStream_t s, const PairDesc *desc, int pairs, int d
                            float *amp, int qmode, uint16_t *q, int cs, MixedStats *stats,
                            uint16_t *qlo, const uint16_t *hfrag, const UpsampleScratch *scratch)
 {
+    const int Np = (N + 15) / 16 * 16;
+    const int Np32 = (N + 31) / 32 * 32;
+    if (rf == 2 && hfrag && qmode != kUpFP32 && !qlo && Np32 <= up2h_max_n(dtype)) {
+        const size_t lds = up2h_lds_bytes(Np32, dtype);
+        const dim3 grid(pairs), block(kUp2Waves * kWave);
+        const uint4 *hf = reinterpret_cast<const uint4 *>(hfrag);
+#define AMPCOR_UP2H(DT, QM)                                                                        \
+    hipLaunchKernelGGL((k_upsample2h<DT, QM>), grid, block, lds, s, desc, role, window, N, Np32, hf, \
+                       amp, q, cs, stats)
+        if (dtype == 0) {
+            if (qmode == kUpFP16) AMPCOR_UP2H(0, kUpFP16);
+            else AMPCOR_UP2H(0, kUpBF16);
+        } else {
+            if (qmode == kUpFP16) AMPCOR_UP2H(1, kUpFP16);
+            else AMPCOR_UP2H(1, kUpBF16);
+        }
+#undef AMPCOR_UP2H
+        return hipGetLastError();
+    }
     if (upsample_needs_scratch(N, rf) && g && scratch && scratch->T) {
         const int Np = (N + 15) / 16 * 16, nb = Np / 16;
         const dim3 grid(pairs, nb), block(kUpBandThreads);
@@ -2329,25 +2364,6 @@ hipError_t launch_upsample(hipStream_t s, const PairDesc *desc, int pairs, int d
                 hipLaunchKernelGGL(k_up2_finish<kUpBF16>, fg, fb, 0, s, pairs, N, scratch->band, nb,
                                    scratch->rows, nb, stats);
         }
-        return hipGetLastError();
-    }
-    const int Np = (N + 15) / 16 * 16;
-    const int Np32 = (N + 31) / 32 * 32;
-    if (rf == 2 && hfrag && qmode != kUpFP32 && !qlo && Np32 <= kUp2hMaxN) {
-        const size_t lds = up2h_lds_bytes(Np32);
-        const dim3 grid(pairs), block(kUp2Waves * kWave);
-        const uint4 *hf = reinterpret_cast<const uint4 *>(hfrag);
-#define AMPCOR_UP2H(DT, QM)                                                                        \
-    hipLaunchKernelGGL((k_upsample2h<DT, QM>), grid, block, lds, s, desc, role, window, N, Np32, hf, \
-                       amp, q, cs, stats)
-        if (dtype == 0) {
-            if (qmode == kUpFP16) AMPCOR_UP2H(0, kUpFP16);
-            else AMPCOR_UP2H(0, kUpBF16);
-        } else {
-            if (qmode == kUpFP16) AMPCOR_UP2H(1, kUpFP16);
-            else AMPCOR_UP2H(1, kUpBF16);
-        }
-#undef AMPCOR_UP2H
         return hipGetLastError();
     }
     if (rf == 2 && Np <= kUp2MaxN && g) {

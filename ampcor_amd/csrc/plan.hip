@@ -392,12 +392,12 @@ int ampcor_hip_plan_create(int device, size_t pairs, size_t ref_lines, size_t re
     }
     if (int rc = upload_upsampler(p->U_ref, p->g_ref, p->C, p->rf)) return cleanup(rc);
     if (int rc = upload_upsampler(p->U_tgt, p->g_tgt, p->E, p->rf)) return cleanup(rc);
-    // the BF16/FP16 forms upsample tiles up to 96 samples on FP16 matrix cores
-    // (the split forms keep FP32)
+    // the BF16/FP16/FP8 forms upsample tiles up to 96 samples (160 for real
+    // rasters) on FP16 matrix cores (the split forms keep FP32)
     if (p->mixed && !p->split && p->rf == 2) {
-        if (p->C <= 96)
+        if (upsample_uses_fragments(p->C))
             if (int rc = upload_fragments(p->hfrag_ref, p->C)) return cleanup(rc);
-        if (p->E <= 96)
+        if (upsample_uses_fragments(p->E))
             if (int rc = upload_fragments(p->hfrag_tgt, p->E)) return cleanup(rc);
     }
     // larger tiles upsample in two banded passes through scratch, shared by

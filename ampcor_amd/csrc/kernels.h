@@ -127,7 +127,8 @@ struct MixedShape {
 };
 // nsplit > 0 asks for at least that many chip-row ranges (tuning)
 // split: hi/lo operands (three products per pair of elements), twice the LDS
-MixedShape choose_mixed_shape(int C, int P, int nsplit = 0, bool split = false);
+// max_vt > 0 caps the v-tiles per workgroup (tuning)
+MixedShape choose_mixed_shape(int C, int P, int nsplit = 0, bool split = false, int max_vt = 0);
 // chip_lo / tgt_lo (optional): the rounding remainders, for the split form
 hipError_t launch_prep_mixed(hipStream_t s, const float *ref_amp, const float *tgt_amp, int pairs,
                              int C, int W, int CS, bool fp16, uint16_t *chip_q, uint16_t *tgt_q,

@@ -182,15 +182,16 @@ __global__ __launch_bounds__(256) void k_normalize_mixed(const float *__restrict
 
 }  // namespace
 
-MixedShape choose_mixed_shape(int C, int P, int nsplit, bool split)
+MixedShape choose_mixed_shape(int C, int P, int nsplit, bool split, int max_vt)
 {
     MixedShape s{};
     s.nk = (C + 15 + 15) / 16;
     s.cs = 16 * s.nk + 16;
     s.split = split;
     const int tiles = (P + 15) / 16;
-    // v-tiles per workgroup: at most kMaxVT, spread evenly over the blocks
-    const int vblocks = (tiles + mixed_detail::kMaxVT - 1) / mixed_detail::kMaxVT;
+    // v-tiles per workgroup: at most kMaxVT (or max_vt), spread evenly over the blocks
+    const int vmax = max_vt > 0 && max_vt < mixed_detail::kMaxVT ? max_vt : mixed_detail::kMaxVT;
+    const int vblocks = (tiles + vmax - 1) / vmax;
     s.wg_vt = (tiles + vblocks - 1) / vblocks;
     // u-tiles (waves) per workgroup: at most max_waves, spread evenly over the blocks
     const int max_ut = mixed_detail::max_waves(s.nk);

@@ -867,8 +867,12 @@ int ampcor_hip_plan_autotune(ampcor_hip_plan *plan, int reps)
             if (plan->fp8 && st.shape == &plan->mixed_coarse) continue;   // tuned below
             float best = 1e30f;
             MixedShape pick = *st.shape;
+            // v-tiles per workgroup: the widest, and 3 (which divides some tile
+            // counts more evenly, e.g. 9 = 3 x 3 rather than 5 + 5)
+            for (int vt : {0, 3})
             for (int ns = 1; ns <= st.maxsplit && !rc; ns *= 2) {
-                MixedShape sh = choose_mixed_shape(st.C, st.P, ns, st.split);
+                MixedShape sh = choose_mixed_shape(st.C, st.P, ns, st.split, vt);
+                if (vt && sh.wg_vt == choose_mixed_shape(st.C, st.P, ns, st.split).wg_vt) continue;
                 if (sh.nsplit > st.maxsplit || sh.lds_bytes > 150 * 1024) continue;
                 float ms;
                 const bool coarse = st.shape == &plan->mixed_coarse;

@@ -3,13 +3,14 @@
 
 namespace ampcor_hip {
 
-Fp8Shape choose_fp8_shape(int C, int P, int nsplit)
+Fp8Shape choose_fp8_shape(int C, int P, int nsplit, int max_vt)
 {
     Fp8Shape s{};
     s.nk = (C + 15 + 15) / 16;
     s.cs = 16 * s.nk + 16;
     const int tiles = (P + 15) / 16;
-    const int vblocks = (tiles + fp8_detail::kMaxVT - 1) / fp8_detail::kMaxVT;
+    const int vmax = max_vt > 0 && max_vt < fp8_detail::kMaxVT ? max_vt : fp8_detail::kMaxVT;
+    const int vblocks = (tiles + vmax - 1) / vmax;
     s.wg_vt = (tiles + vblocks - 1) / vblocks;
     const int ublocks = (tiles + fp8_detail::kMaxWaves - 1) / fp8_detail::kMaxWaves;
     s.wg_ut = (tiles + ublocks - 1) / ublocks;   // waves, spread evenly over the blocks

@@ -153,7 +153,7 @@ struct Fp8Shape {
     int wg_ut, wg_vt, tiles_u, tiles_v, nsplit, ilen;
     size_t lds_bytes;
 };
-Fp8Shape choose_fp8_shape(int C, int P, int nsplit = 0);
+Fp8Shape choose_fp8_shape(int C, int P, int nsplit = 0, int max_vt = 0);
 hipError_t launch_correlate_fp8(hipStream_t s, const Fp8Shape &shape, const uint8_t *chip_q,
                                 const uint8_t *tgt_q, int pairs, int C, int W, float *partial);
 

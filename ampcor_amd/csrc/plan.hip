@@ -892,8 +892,10 @@ int ampcor_hip_plan_autotune(ampcor_hip_plan *plan, int reps)
             Fp8Shape pick = plan->fp8_coarse;
             const auto *cq8 = reinterpret_cast<const uint8_t *>(plan->chip_q.ptr);
             const auto *tq8 = reinterpret_cast<const uint8_t *>(plan->tgt_q.ptr);
+            for (int vt : {0, 3})
             for (int ns = 1; ns <= kTuneMaxSplitCoarse && !rc; ns *= 2) {
-                Fp8Shape sh = choose_fp8_shape(plan->C, plan->P, ns);
+                Fp8Shape sh = choose_fp8_shape(plan->C, plan->P, ns, vt);
+                if (vt && sh.wg_vt == choose_fp8_shape(plan->C, plan->P, ns).wg_vt) continue;
                 if (sh.nsplit > kTuneMaxSplitCoarse || sh.lds_bytes > 150 * 1024) continue;
                 float ms;
                 if ((rc = timeit([&] {

@@ -365,10 +365,14 @@ __global__ __launch_bounds__(256) void k_normalize(const float *__restrict__ par
 // difference of two prefix entries.  The numerator partials (chip-row or
 // chip-column ranges) are summed in order, and the sums never leave the chip.
 //
+// The first box sums come from all threads (a chunk of columns and every
+// G-th row each, summed in order through LDS); each block's target rows and
+// numerator partials are fetched while the previous block is processed.
+//
 // Forms: kFuseFP32 (FP32 amplitudes; chip stats (sum r', |r'|)), kFuseMixed
 // (FP32 amplitudes; the mixed normalizer with the operand scales), and
-// kFuseBF16 / kFuseFP16, which take the window sums from the quantized
-// target itself.  With every sum in quantized units the scales and the
+// kFuseBF16 / kFuseFP16 / kFuseFP8, which take the window sums from the
+// quantized target itself.  With every sum in quantized units the scales and the
 // removed constants cancel, and gamma is the normalized correlation of
 // exactly the operands the contraction multiplied:
 //   gamma = (num - sum R sum T / n) /
@@ -1192,13 +1196,14 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2(
 }
 
 // ---------------------------------------------------------------------------
-// x2 upsampling for the BF16/FP16 forms: the same three circulant products as
-// k_upsample2, on v_mfma_f32_16x16x32_f16 with FP32 accumulation.  The tile
-// is scaled by a power of two into FP16 range (a first pass over it takes
-// its peak and mean amplitude); LDS holds it twice, row- and column-major,
-// because the first product contracts over its rows and the second over its
-// columns (78 KiB for N = 96: two workgroups share a CU, so that one's
-// global loads overlap the other's products).  The circulant comes as
+// x2 upsampling for the BF16/FP16/FP8 forms: the same three circulant
+// products as k_upsample2, on v_mfma_f32_16x16x32_f16 with FP32 accumulation.
+// The tile is read once into registers (its peak and mean amplitude taken
+// there) and scaled by a power of two into FP16 range; LDS holds it twice,
+// row- and column-major, because the first product contracts over its rows
+// and the second over its columns (complex tiles: four planes, 78 KiB for
+// N = 96, so two workgroups share a CU; real tiles: two planes, up to
+// N = 160).  Only the tiles that hold samples are multiplied.  The circulant comes as
 // precomputed FP16 fragments (hfrag, one 16 x 32 block per 64 lanes x 8
 // values: H[16 mt + l % 16][32 kc + 8 (l / 16) + e]), read from global
 // memory (L2-resident, shared by every pair), which serve as the A operand

@@ -31,6 +31,7 @@ sys.path.insert(0, ROOT)
 METRIC = "tile pairs/sec @64²/128² zoom8, mixed vs FP32 at ≤0.01px; p99 job latency"
 FP32_PEAK_TFLOPS = 157.3   # MI355X dense FP32 (matrix) peak
 MIXED_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 / FP16 matrix peak
+FP8_PEAK_TFLOPS = 5000.0    # MI355X dense FP8 matrix peak
 HBM_PEAK_GBS = 8000.0
 WORKLOADS = {
     "scene": "scene 8192x4096 complex64 speckle, smooth-field secondary +15 dB, 120x60 grid "
@@ -87,6 +88,68 @@ def cpu_baseline(scene, seconds):
     return ({"value": len(idx) / dt, "unit": "tile pairs/sec", "cores": cores, "kind": "port",
              "sample": f"FP64 oracle (oracle/ampcor_oracle.c, OpenMP over pairs) on {len(idx)} "
                        f"of the {scene.pairs} pairs, evenly spaced, {dt:.1f} s"}, idx, offsets)
+
+
+def stage_work(precision, scene, dtype):
+    """Algorithmic work of one launch of each stage: (bound, amount, unit).
+
+    HBM stages count the bytes they must move at least (each operand read once,
+    each product written once, per pair); the contractions count 2 flops per
+    multiply-add of the NCC numerator over all placements.  The fused stages
+    carry the work of the reference kernels they absorb: gather_prep = gather +
+    detect (detect.cu) + chip statistics (refStats.cu) + quantization;
+    normalize = window sums (sat.cu, tgtStats.cu) + normalization; the
+    upsamplers = migrate (migrate.cu) + x2 upsampling; coarse_peak = maxcor.cu
+    + nudge.cu.
+    """
+    N, C, W = scene.pairs, scene.chip, scene.window
+    rf, rm = scene.refine_factor, scene.refine_margin
+    P, E, R = W - C + 1, C + 2 * rm, 2 * rf * rm + 1
+    Cr, Er = rf * C, rf * (C + 2 * rm)
+    eb = 8 if dtype == "complex64" else 4            # raster element bytes
+    split = precision.endswith("x3")
+    qc = {"fp32": 4, "fp8": 1}.get(precision, 2)      # coarse operand bytes
+    qr = 4 if precision == "fp32" else 2              # refined operand bytes (fp8: FP16)
+    hbm = lambda b: ("hbm", float(b) * N, "B")
+    mfma = "valu" if precision == "fp32" else "mfma"
+    win_r = (4 * Er * Er + 2 * qr * Er * Er) if split else qr * Er * Er
+    return {
+        "gather_detect": hbm((eb + 4) * (C * C + W * W)),
+        "chip_stats": hbm(8 * C * C),
+        "window_sums": hbm(4 * W * W + 16 * P * P),
+        "gather_prep": hbm(eb * (C * C + W * W) + qc * (C * C + W * W)),
+        "correlate": (mfma, 2.0 * C * C * P * P * N, "FLOP"),
+        "normalize": hbm(24 * P * P if precision == "fp32" else qc * W * W + 8 * P * P),
+        "coarse_peak": hbm(4 * P * P),
+        "upsample_ref": hbm(eb * C * C + (2 if split else 1) * qr * Cr * Cr),
+        "upsample_tgt": hbm(eb * E * E + (4 * Er * Er if precision == "fp32" else win_r)),
+        "chip_stats_refined": hbm(8 * Cr * Cr),
+        "correlate_refined": (mfma, 2.0 * Cr * Cr * R * R * N, "FLOP"),
+        "normalize_refined": hbm((4 * Er * Er if precision == "fp32" or split else qr * Er * Er)
+                                 + 8 * R * R),
+        "zoom": hbm(4 * R * R + 16),
+    }
+
+
+def stage_roofline(precision, scene, dtype, stages):
+    """Per-stage achieved rate against MI355X's published peak."""
+    peaks = {"hbm": (HBM_PEAK_GBS, "GB/s", 1e9), "valu": (FP32_PEAK_TFLOPS, "TFLOP/s", 1e12),
+             "mfma": (FP8_PEAK_TFLOPS if precision == "fp8" else MIXED_PEAK_TFLOPS,
+                      "TFLOP/s", 1e12)}
+    work = stage_work(precision, scene, dtype)
+    out = {}
+    for name, ms in stages.items():
+        if name not in work:
+            continue
+        bound, amount, unit = work[name]
+        peak, punit, scale = peaks[bound]
+        if name == "correlate_refined" and bound == "mfma":
+            peak = MIXED_PEAK_TFLOPS          # the FP8 form refines in FP16
+        achieved = amount / (ms * 1e-3) / scale
+        out[name] = {"ms": round(ms, 4), "bound": bound, "work": amount, "work_unit": unit,
+                     "achieved": round(achieved, 2), "unit": punit, "peak": peak,
+                     "frac": round(achieved / peak, 4)}
+    return out
 
 
 def main():
@@ -170,6 +233,8 @@ def main():
         with open(tfile) as fh:
             traffic = json.load(fh).get("bytes_per_launch", {}).get(a.precision, {}).get(dom)
     peak = FP32_PEAK_TFLOPS if a.precision == "fp32" else MIXED_PEAK_TFLOPS
+    if a.precision == "fp8" and dom == "correlate":
+        peak = FP8_PEAK_TFLOPS
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
                 "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
                 "traffic": traffic, "kernel": dom, "kernel_ms": round(stages[dom], 4)}
@@ -209,6 +274,7 @@ def main():
                    "forms_pairs_per_sec": {f: round(N / runs[f][0], 1) for f in forms},
                    "speedup_vs_fp32": round(value / fp32_value, 3), "accuracy": accuracy},
         "roofline": roofline,
+        "stages": {f: stage_roofline(f, scene, dtype, runs[f][1]) for f in forms[:2]},
         "cpu_baseline": cpu,
         "serving": serving,
     }

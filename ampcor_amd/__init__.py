@@ -10,7 +10,7 @@ module ``ampcor_hip`` mirrors the reference's ``ampcor_cuda`` extension, and
 """
 
 from . import ext
-from .correlators import HIP, MGA, Constant, Grid, Plan, UniformGrid
+from .correlators import HIP, MGA, Constant, Field, Grid, Plan, UniformGrid
 from .dom import SLC, Slice
 
-__all__ = ["ext", "HIP", "MGA", "Constant", "Grid", "Plan", "UniformGrid", "SLC", "Slice"]
+__all__ = ["ext", "HIP", "MGA", "Constant", "Field", "Grid", "Plan", "UniformGrid", "SLC", "Slice"]

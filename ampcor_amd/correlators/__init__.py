@@ -2,6 +2,6 @@
 
 from .hip import HIP
 from .mga import MGA
-from .plan import Constant, Grid, OffsetMap, Plan, UniformGrid
+from .plan import Constant, Field, Grid, OffsetMap, Plan, UniformGrid
 
-__all__ = ["HIP", "MGA", "Constant", "Grid", "OffsetMap", "Plan", "UniformGrid"]
+__all__ = ["HIP", "MGA", "Constant", "Field", "Grid", "OffsetMap", "Plan", "UniformGrid"]

@@ -5,13 +5,16 @@ refineMargin, refineFactor, zoomFactor and the coarse map) and ``estimate``,
 which plans the pairs, opens the rasters and asks the strategy for the
 offsets.  ``estimate`` returns the offsets and products (the reference drops
 them and returns 0, MGA.py:91-98), and ``offset_field`` lays them out on the
-tile grid with NaN for invalid pairs.
+tile grid with NaN for invalid pairs.  ``next_pass`` turns the result into the
+coarse map of a further, finer pass (a ``Field`` functor): the offsets the
+strategy returns are relative to the initial guess, as in the reference, and
+the field adds the guess back.
 """
 
 import numpy as np
 
 from .hip import HIP
-from .plan import Grid, Plan
+from .plan import Field, Grid, Plan, UniformGrid
 
 
 class MGA:
@@ -36,19 +39,33 @@ class MGA:
         worker = self.makeWorker()
         coarse = self.coarse.map(reference=reference)
         plan = self.makePlan(regmap=coarse, rasters=(reference, target))
-        ref = reference.open().raster
-        tgt = target.open().raster
+        # rasters stay resident across passes: each is uploaded on its first open
+        ref = (reference.raster if reference.raster is not None else reference.open().raster)
+        tgt = (target.raster if target.raster is not None else target.open().raster)
         offsets = worker.adjust(manager=self, rasters=(ref, tgt), plan=plan, channel=channel)
         self.plan = plan
         self.products = worker.products()
         return offsets
 
-    def offset_field(self):
-        """Dense (grid..., 4) float32 raster: dLine, dSample, coarse peak, SNR; NaN where invalid."""
+    def offset_field(self, total=False):
+        """Dense (grid..., 4) float32 raster: dLine, dSample, coarse peak, SNR; NaN where invalid.
+
+        With ``total`` the offsets include the initial guess of each tile (the
+        coarse map's integer shift), i.e. the full displacement into the target."""
         valid = self.plan.valid
         field = np.full((valid.size, 4), np.nan, dtype=np.float32)
         p = self.products
-        field[valid, 0:2] = p["offsets"]
+        off = p["offsets"].astype(np.float64)
+        if total:
+            off = off + self.plan.shift[valid]
+        field[valid, 0:2] = off
         field[valid, 2] = p["coarse_peak"]
         field[valid, 3] = p["snr"]
         return field.reshape(tuple(self.plan.tile) + (4,))
+
+    def next_pass(self, shape, fallback=(0, 0)):
+        """The coarse map of a further pass: a ``shape`` grid of tiles whose
+        initial guesses follow this pass's total offsets."""
+        total = self.offset_field(total=True)[..., 0:2].astype(np.float64).reshape(-1, 2)
+        functor = Field(points=self.plan.domain, offsets=total, fallback=fallback)
+        return Grid(domain=UniformGrid(shape=shape), functor=functor)

@@ -5,6 +5,8 @@ Restates the caller side of the hot path from the reference:
   Constant.codomain    pkg/ampcor/correlators/Constant.py:32-43
   Grid.map             pkg/ampcor/correlators/Grid.py:35-50
   Plan / assemble      pkg/ampcor/correlators/Plan.py:15-185
+  Functor protocol     pkg/ampcor/correlators/Functor.py:15-38 (``Field`` is a
+                       second implementation beside ``Constant``)
 Pair ``idx`` is the position of a valid pair in ``plan.pairs`` (invalid pairs,
 whose chip or window leaves a raster, are dropped), exactly as the reference
 numbers them when it feeds its workers (CUDA.py:50-54).
@@ -41,6 +43,73 @@ class Constant:
             yield tuple(p + s for p, s in zip(point, self.shift))
 
 
+class Field:
+    """Maps reference points through a dense offset field: the coarse map of an
+    iterative pass, built from the offsets of the pass before it.
+
+    ``points`` are the (line, sample) reference points of the earlier pass, on
+    a rectilinear grid (what ``UniformGrid.points`` yields), and ``offsets``
+    their total displacements (dLine, dSample) into the target, NaN where the
+    earlier pass had no valid pair.  The image of a point is the point plus
+    the field interpolated bilinearly at it, each coordinate rounded half up
+    to an integer (window corners are whole pixels).  Outside the grid the
+    field is held at its edge value.  A NaN node takes the value of the
+    nearest valid node (grid-index distance, ties to the lowest row-major
+    index); a field with no valid node maps every point by ``fallback``.
+    """
+
+    def __init__(self, points, offsets, fallback=(0, 0)):
+        pts = np.asarray(points, dtype=np.float64).reshape(-1, 2)
+        off = np.asarray(offsets, dtype=np.float64).reshape(-1, 2)
+        if pts.shape != off.shape:
+            raise ValueError("points and offsets must have the same number of (line, sample) rows")
+        self.lines = np.unique(pts[:, 0])
+        self.samples = np.unique(pts[:, 1])
+        if self.lines.size * self.samples.size != pts.shape[0]:
+            raise ValueError("the field's points must form a full rectilinear grid")
+        order = np.lexsort((pts[:, 1], pts[:, 0]))     # row-major over (line, sample)
+        grid = off[order].reshape(self.lines.size, self.samples.size, 2)
+        self.fallback = tuple(int(f) for f in fallback)
+        self.grid = self._fill(grid)
+
+    @staticmethod
+    def _fill(grid):
+        bad = np.isnan(grid).any(axis=2)
+        if not bad.any() or bad.all():
+            return None if bad.all() else grid
+        good = np.argwhere(~bad)
+        out = grid.copy()
+        for i, j in np.argwhere(bad):
+            d = (good[:, 0] - i) ** 2 + (good[:, 1] - j) ** 2
+            gi, gj = good[int(np.argmin(d))]
+            out[i, j] = grid[gi, gj]
+        return out
+
+    def _axis(self, ticks, x):
+        """Cell index and weight of x on one axis, held at the ends."""
+        if ticks.size == 1:
+            return 0, 0, 0.0
+        x = min(max(x, ticks[0]), ticks[-1])
+        k = int(np.searchsorted(ticks, x, side="right")) - 1
+        k = min(k, ticks.size - 2)
+        return k, k + 1, (x - ticks[k]) / (ticks[k + 1] - ticks[k])
+
+    def at(self, point):
+        """The field (dLine, dSample) interpolated at a reference point."""
+        if self.grid is None:
+            return np.array(self.fallback, dtype=np.float64)
+        i0, i1, a = self._axis(self.lines, float(point[0]))
+        j0, j1, b = self._axis(self.samples, float(point[1]))
+        g = self.grid
+        return ((1 - a) * ((1 - b) * g[i0, j0] + b * g[i0, j1])
+                + a * ((1 - b) * g[i1, j0] + b * g[i1, j1]))
+
+    def codomain(self, domain):
+        for point in domain:
+            d = self.at(point)
+            yield tuple(int(p) + int(np.floor(v + 0.5)) for p, v in zip(point, d))
+
+
 class OffsetMap:
     """A coarse offset map: reference points (domain) and their target images."""
 
@@ -73,6 +142,11 @@ class Plan:
         self.chip = tuple(correlator.chip)
         self.padding = tuple(correlator.padding)
         self.window = tuple(c + 2 * p for c, p in zip(self.chip, self.padding))
+        self.domain = tuple(regmap.domain)
+        # the integer shift of each tile's initial guess: target point - reference point
+        self.shift = np.array([tuple(t - r for r, t in zip(rp, tp))
+                               for rp, tp in zip(regmap.domain, regmap.codomain)],
+                              dtype=np.int64).reshape(-1, 2)
         self.reference, self.target = self.assemble(regmap=regmap, rasters=rasters)
 
     @property

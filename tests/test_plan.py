@@ -4,7 +4,7 @@ pkg/ampcor/dom/SLC.py:55-76) restated without pyre."""
 import numpy as np
 import pytest
 
-from ampcor_amd import SLC, Constant, Grid, Plan, UniformGrid
+from ampcor_amd import SLC, Constant, Field, Grid, Plan, UniformGrid
 
 
 class Corr:
@@ -52,3 +52,67 @@ def test_shifted_coarse_map_moves_the_windows():
     plan = plan_for((512, 512), (4, 4), 32, 16, shift=(3, -2))
     ref, tgt = plan.corners()
     assert np.all(tgt - ref == np.array([3 - 16, -2 - 16]))
+
+
+def grid_points(shape, bounds):
+    return UniformGrid(shape=shape).points(bounds=bounds)
+
+
+def test_field_of_a_constant_is_the_constant_functor():
+    pts = grid_points((4, 5), (400, 500))
+    field = Field(points=pts, offsets=np.tile([3.4, -7.6], (len(pts), 1)))
+    dom = grid_points((7, 3), (400, 500))
+    assert tuple(field.codomain(dom)) == tuple(Constant(shift=(3, -8)).codomain(dom))
+
+
+def test_field_interpolates_a_linear_field_exactly():
+    # bilinear interpolation reproduces any field linear in (line, sample)
+    pts = np.array(grid_points((5, 6), (500, 600)), dtype=np.float64)
+    f = lambda p: np.stack([0.05 * p[..., 0] - 0.01 * p[..., 1] + 2.0,
+                            -0.02 * p[..., 0] + 0.03 * p[..., 1] - 5.0], axis=-1)
+    field = Field(points=pts, offsets=f(pts))
+    q = np.array([(60, 77), (123, 401), (449, 549), (250, 300)], dtype=np.float64)
+    for p in q:
+        assert np.allclose(field.at(p), f(p), atol=1e-12)
+    img = np.array(tuple(field.codomain([tuple(int(v) for v in p) for p in q])))
+    assert np.array_equal(img, q.astype(np.int64) + np.floor(f(q) + 0.5).astype(np.int64))
+
+
+def test_field_rounds_half_up_and_holds_its_edges():
+    pts = grid_points((2, 2), (200, 200))          # nodes at 50 and 150
+    off = np.array([[0.5, -0.5], [0.5, -0.5], [2.5, 1.5], [2.5, 1.5]])
+    field = Field(points=pts, offsets=off)
+    assert next(field.codomain([(0, 0)])) == (1, 0)            # held at the first row: (0.5, -0.5)
+    assert next(field.codomain([(199, 10)])) == (199 + 3, 12)  # held at the last row: (2.5, 1.5)
+    assert next(field.codomain([(100, 10)])) == (100 + 2, 11)  # midway: (1.5, 0.5)
+
+
+def test_field_fills_holes_from_the_nearest_valid_node():
+    pts = grid_points((3, 3), (300, 300))
+    off = np.full((9, 2), np.nan)
+    off[0] = (1.0, 2.0)
+    off[8] = (-4.0, 6.0)
+    field = Field(points=pts, offsets=off)
+    # node (0, 1) is nearest to node (0, 0); node (2, 1) to node (2, 2)
+    assert np.array_equal(field.grid[0, 1], [1.0, 2.0])
+    assert np.array_equal(field.grid[2, 1], [-4.0, 6.0])
+    # the centre is equidistant: the lowest row-major valid node wins
+    assert np.array_equal(field.grid[1, 1], [1.0, 2.0])
+    empty = Field(points=pts, offsets=np.full((9, 2), np.nan), fallback=(5, -1))
+    assert next(empty.codomain([(10, 10)])) == (15, 9)
+
+
+def test_field_rejects_scattered_points():
+    with pytest.raises(ValueError):
+        Field(points=[(0, 0), (1, 5), (7, 3)], offsets=np.zeros((3, 2)))
+
+
+def test_plan_records_each_tiles_initial_shift():
+    ref, tgt = SLC(shape=(400, 400)), SLC(shape=(400, 400))
+    pts = grid_points((2, 2), (400, 400))
+    field = Field(points=pts, offsets=[[1, 2], [3, 4], [5, 6], [7, 8]])
+    coarse = Grid(domain=UniformGrid(shape=(2, 2)), functor=field)
+    plan = Plan(correlator=Corr(32, 16), regmap=coarse.map(reference=ref), rasters=(ref, tgt))
+    assert plan.shift.tolist() == [[1, 2], [3, 4], [5, 6], [7, 8]]
+    r, t = plan.corners()
+    assert np.array_equal(t - r, plan.shift - 16)

@@ -9,7 +9,8 @@
 //   adjust(worker) -> tuple of (dLine, dSample), one per pair
 // plus the raster constructors that replace the host SLC capsule of the CPU
 // module (ext/ampcor/slc.cc:50-140): slc_map(filename, samples, lines),
-// raster(buffer, lines, samples[, dtype, device]), slc_pixelSize(); and the
+// raster(buffer, lines, samples[, dtype, device]), slc_pixelSize(),
+// slc_getitem(raster, offset); the CPU module's refine(worker); and the
 // batched and product accessors: addPairs, products, dims, stage_gamma,
 // stage_gamma_refined, stage_locations, set_timing, stage_times.
 //
@@ -114,6 +115,31 @@ int parse_precision(const char *name, int *precision)
 // ---- rasters -----------------------------------------------------------------
 
 PyObject *slc_pixelSize(PyObject *, PyObject *) { return PyLong_FromLong(8); }
+
+// slc_getitem(raster, offset) -> complex: the pixel at row-major offset
+// (ext/ampcor/slc.cc:105-140); a float32 raster's pixels have no imaginary part.
+// Past the end raises IndexError (the reference reads past its map).
+PyObject *slc_getitem(PyObject *, PyObject *args)
+{
+    PyObject *capsule;
+    unsigned long long offset;
+    if (!PyArg_ParseTuple(args, "OK:slc_getitem", &capsule, &offset)) return nullptr;
+    ampcor_hip_raster *r = get_raster(capsule);
+    if (!r) return nullptr;
+    size_t lines = 0, samples = 0;
+    if (int rc = ampcor_hip_raster_shape(r, &lines, &samples)) return raise(rc);
+    if (offset >= (unsigned long long)(lines * samples)) {
+        PyErr_SetString(PyExc_IndexError, "raster offset out of range");
+        return nullptr;
+    }
+    float px[2] = {0.0f, 0.0f};
+    int rc;
+    Py_BEGIN_ALLOW_THREADS
+    rc = ampcor_hip_raster_read(r, (size_t)offset, 1, px);
+    Py_END_ALLOW_THREADS
+    if (rc) return raise(rc);
+    return PyComplex_FromDoubles(px[0], px[1]);
+}
 
 PyObject *raster(PyObject *, PyObject *args, PyObject *kwds)
 {
@@ -300,6 +326,18 @@ PyObject *adjust(PyObject *, PyObject *args, PyObject *kwds)
         PyTuple_SET_ITEM(results, p, shift);
     }
     return results;
+}
+
+// refine(worker) -> None: the CPU module's separate refinement call
+// (ext/ampcor/sequential.cc:262-294, whose worker.refine() is empty,
+// lib/libampcor/correlators/Sequential.cc:101-104).  adjust already runs the
+// whole refinement chain, so this only checks the worker.
+PyObject *refine(PyObject *, PyObject *args)
+{
+    PyObject *pyWorker;
+    if (!PyArg_ParseTuple(args, "O!:refine", &PyCapsule_Type, &pyWorker)) return nullptr;
+    if (!get_worker(pyWorker)) return nullptr;
+    Py_RETURN_NONE;
 }
 
 PyObject *bytes_of(size_t n)
@@ -601,6 +639,9 @@ PyMethodDef methods[] = {
     {"slc_map", (PyCFunction)(void (*)(void))slc_map, METH_VARARGS | METH_KEYWORDS,
      "map an SLC raster file of a given shape onto the device"},
     {"slc_pixelSize", slc_pixelSize, METH_NOARGS, "the size of an SLC pixel"},
+    {"slc_getitem", slc_getitem, METH_VARARGS, "fetch the pixel at the given offset"},
+    {"refine", refine, METH_VARARGS,
+     "sub-pixel refinement of the registration map (adjust already refines)"},
     {"device_count", device_count, METH_NOARGS, "number of visible HIP devices"},
     {"autotune", autotune, METH_VARARGS, "time candidate tilings on the last adjust's data"},
     {"server", (PyCFunction)(void (*)(void))server, METH_VARARGS | METH_KEYWORDS,

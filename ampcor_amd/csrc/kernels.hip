@@ -752,6 +752,41 @@ __device__ inline void wave_argmax(float &best, int &bidx)
     }
 }
 
+// the wave's SNR sums reduced, then lane 0 writes the peak, SNR and nudged window
+__device__ inline void coarse_peak_finish(int pair, int lane, int P, int W, int E, int rm, bool none,
+                                          int row, int col, float pk, double sum, int count,
+                                          int2 *__restrict__ loc, int2 *__restrict__ window,
+                                          float *__restrict__ peak, float *__restrict__ snr)
+{
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) {
+        sum += __shfl_xor(sum, d, kWave);
+        count += __shfl_xor(count, d, kWave);
+    }
+    if (lane == 0) {
+        int left = col - rm;
+        if (left < 0) left = 0;
+        if (left + E > W) left = W - E;
+        int top = row - rm;
+        if (top < 0) top = 0;
+        if (top + E > W) top = W - E;
+        loc[pair] = make_int2(row, col);
+        window[pair] = make_int2(top, left);
+        peak[pair] = pk;
+        const double p = (double)pk;
+        float s;
+        if (none) s = __builtin_nanf("");
+        else if (count == 0 || sum <= 0.0) s = INFINITY;
+        else s = (float)(p * p / (sum / count));
+        snr[pair] = s;
+    }
+}
+
+// KEEP > 0: the surface has at most KEEP cells per lane and stays in registers
+// between the argmax and the SNR sums (one read of it); KEEP = 0 reads it twice.
+// Both visit each lane's cells in the same increasing order, so they give the
+// same bits.
+template <int KEEP>
 __global__ __launch_bounds__(256) void k_coarse_peak(const float *__restrict__ gamma, int pairs,
                                                      int P, int W, int E, int rm, int exclude,
                                                      int2 *__restrict__ loc,
@@ -764,6 +799,41 @@ __global__ __launch_bounds__(256) void k_coarse_peak(const float *__restrict__ g
     if (pair >= pairs) return;
     const int cells = P * P;
     const float *g = gamma + (size_t)pair * cells;
+    if constexpr (KEEP > 0) {
+        // every load in flight at once; NaN past the last cell never wins and is
+        // never summed
+        float x[KEEP];
+#pragma unroll
+        for (int j = 0; j < KEEP; ++j) {
+            const int k = lane + j * kWave;
+            x[j] = k < cells ? g[k] : __builtin_nanf("");
+        }
+        float best = -INFINITY;
+        int bidx = 0x7fffffff;
+#pragma unroll
+        for (int j = 0; j < KEEP; ++j)
+            if (x[j] > best) {
+                best = x[j];
+                bidx = lane + j * kWave;
+            }
+        wave_argmax(best, bidx);
+        const bool none = (bidx == 0x7fffffff);
+        const int row = none ? 0 : bidx / P;
+        const int col = none ? 0 : bidx - row * P;
+        double sum = 0.0;
+        int count = 0;
+#pragma unroll
+        for (int j = 0; j < KEEP; ++j) {
+            const int k = lane + j * kWave;
+            const int u = k / P, v = k - u * P;
+            if (x[j] != x[j] || (abs(u - row) <= exclude && abs(v - col) <= exclude)) continue;
+            sum += (double)x[j] * (double)x[j];
+            ++count;
+        }
+        coarse_peak_finish(pair, lane, P, W, E, rm, none, row, col, none ? __builtin_nanf("") : best,
+                           sum, count, loc, window, peak, snr);
+        return;
+    }
     // kPeakBatch loads in flight per lane; each lane visits its cells in
     // increasing order, so its ties resolve to the first
     constexpr int kPeakBatch = 8;
@@ -807,28 +877,8 @@ __global__ __launch_bounds__(256) void k_coarse_peak(const float *__restrict__ g
             ++count;
         }
     }
-#pragma unroll
-    for (int d = kWave / 2; d > 0; d >>= 1) {
-        sum += __shfl_xor(sum, d, kWave);
-        count += __shfl_xor(count, d, kWave);
-    }
-    if (lane == 0) {
-        int left = col - rm;
-        if (left < 0) left = 0;
-        if (left + E > W) left = W - E;
-        int top = row - rm;
-        if (top < 0) top = 0;
-        if (top + E > W) top = W - E;
-        loc[pair] = make_int2(row, col);
-        window[pair] = make_int2(top, left);
-        peak[pair] = pk;
-        const double p = (double)pk;
-        float s;
-        if (none) s = __builtin_nanf("");
-        else if (count == 0 || sum <= 0.0) s = INFINITY;
-        else s = (float)(p * p / (sum / count));
-        snr[pair] = s;
-    }
+    coarse_peak_finish(pair, lane, P, W, E, rm, none, row, col, pk, sum, count, loc, window, peak,
+                       snr);
 }
 
 // ---------------------------------------------------------------------------
@@ -1236,7 +1286,10 @@ __host__ __device__ inline size_t up2h_lds_bytes(int Np, int dtype)
     return (dtype == 1 ? 2 : 4) * up2h_plane_bytes(Np);
 }
 
-template <int DTYPE, int QMODE>
+// MAXN: the largest padded tile edge of the instantiation; the register arrays
+// are sized for it, so a smaller MAXN (64 for the chip of the scene) leaves
+// room for more workgroups per CU
+template <int DTYPE, int QMODE, int MAXN>
 __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2h(
     const PairDesc *__restrict__ desc, int role, const int2 *__restrict__ window, int N, int Np,
     const uint4 *__restrict__ hfrag, float *__restrict__ amp, uint16_t *__restrict__ qout, int cs,
@@ -1244,7 +1297,8 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2h(
 {
     // real tiles (DTYPE 1) have no imaginary planes, and may be larger
     constexpr bool kReal = DTYPE == 1;
-    constexpr int kMaxN = up2h_max_n(DTYPE);
+    constexpr int kMaxN = MAXN;
+    static_assert(MAXN % 32 == 0 && MAXN <= up2h_max_n(DTYPE), "tile edge of the instantiation");
     extern __shared__ uint4 lds4[];
     __shared__ double red[kUp2Waves], red2[kUp2Waves];
     __shared__ float redm[kUp2Waves];
@@ -2296,8 +2350,13 @@ hipError_t launch_coarse_peak(hipStream_t s, const float *gamma, int pairs, int 
                               int refine_margin, int exclude, int2 *loc, int2 *window,
                               float *peak, float *snr)
 {
-    hipLaunchKernelGGL(k_coarse_peak, dim3((pairs + 3) / 4), dim3(256), 0, s, gamma, pairs, P, W,
-                       E, refine_margin, exclude, loc, window, peak, snr);
+    constexpr int kKeep = 72;   // surfaces up to 67 x 67 stay in registers
+    if (P * P <= kKeep * kWave)
+        hipLaunchKernelGGL(k_coarse_peak<kKeep>, dim3((pairs + 3) / 4), dim3(256), 0, s, gamma,
+                           pairs, P, W, E, refine_margin, exclude, loc, window, peak, snr);
+    else
+        hipLaunchKernelGGL(k_coarse_peak<0>, dim3((pairs + 3) / 4), dim3(256), 0, s, gamma, pairs,
+                           P, W, E, refine_margin, exclude, loc, window, peak, snr);
     return hipGetLastError();
 }
 
@@ -2323,15 +2382,20 @@ hipError_t launch_upsample(hipStream_t s, const PairDesc *desc, int pairs, int d
         const size_t lds = up2h_lds_bytes(Np32, dtype);
         const dim3 grid(pairs), block(kUp2Waves * kWave);
         const uint4 *hf = reinterpret_cast<const uint4 *>(hfrag);
-#define AMPCOR_UP2H(DT, QM)                                                                        \
-    hipLaunchKernelGGL((k_upsample2h<DT, QM>), grid, block, lds, s, desc, role, window, N, Np32, hf, \
-                       amp, q, cs, stats)
+#define AMPCOR_UP2H(DT, QM, MN)                                                                    \
+    hipLaunchKernelGGL((k_upsample2h<DT, QM, MN>), grid, block, lds, s, desc, role, window, N, Np32, \
+                       hf, amp, q, cs, stats)
         if (dtype == 0) {
-            if (qmode == kUpFP16) AMPCOR_UP2H(0, kUpFP16);
-            else AMPCOR_UP2H(0, kUpBF16);
+            if (Np32 <= 64) {
+                if (qmode == kUpFP16) AMPCOR_UP2H(0, kUpFP16, 64);
+                else AMPCOR_UP2H(0, kUpBF16, 64);
+            } else {
+                if (qmode == kUpFP16) AMPCOR_UP2H(0, kUpFP16, kUp2hMaxN);
+                else AMPCOR_UP2H(0, kUpBF16, kUp2hMaxN);
+            }
         } else {
-            if (qmode == kUpFP16) AMPCOR_UP2H(1, kUpFP16);
-            else AMPCOR_UP2H(1, kUpBF16);
+            if (qmode == kUpFP16) AMPCOR_UP2H(1, kUpFP16, kUp2hMaxNReal);
+            else AMPCOR_UP2H(1, kUpBF16, kUp2hMaxNReal);
         }
 #undef AMPCOR_UP2H
         return hipGetLastError();

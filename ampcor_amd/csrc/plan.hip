@@ -239,6 +239,22 @@ int ampcor_hip_raster_shape(const ampcor_hip_raster *raster, size_t *lines, size
     return AMPCOR_HIP_OK;
 }
 
+int ampcor_hip_raster_read(const ampcor_hip_raster *raster, size_t offset, size_t count,
+                           void *host)
+{
+    if (!raster || (!host && count)) return fail(AMPCOR_HIP_EINVAL, "null argument");
+    const size_t size = raster->lines * raster->samples;
+    if (offset > size || count > size - offset)
+        return fail(AMPCOR_HIP_EINVAL, "read past the end of the raster");
+    if (count == 0) return AMPCOR_HIP_OK;
+    const size_t px = raster->dtype == AMPCOR_HIP_COMPLEX64 ? 8 : 4;
+    DeviceGuard guard(raster->device);
+    HIP_TRY(hipMemcpy(host, static_cast<const char *>(raster->data) + offset * px, count * px,
+                      hipMemcpyDeviceToHost),
+            "reading raster pixels");
+    return AMPCOR_HIP_OK;
+}
+
 int ampcor_hip_raster_destroy(ampcor_hip_raster *raster)
 {
     if (!raster) return AMPCOR_HIP_OK;

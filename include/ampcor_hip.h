@@ -79,6 +79,12 @@ int ampcor_hip_raster_create(int device, int dtype, const void *host, size_t lin
 int ampcor_hip_raster_wrap(int device, int dtype, const void *device_ptr, size_t lines,
                            size_t samples, ampcor_hip_raster **out);
 int ampcor_hip_raster_shape(const ampcor_hip_raster *raster, size_t *lines, size_t *samples);
+/* copy `count` pixels starting at row-major pixel `offset` to host memory (8 bytes
+ * a pixel for complex64, 4 for float32); AMPCOR_HIP_EINVAL past the raster's end.
+ * Replaces the SLC element read of ext/ampcor/slc.cc:105-140 (slc_getitem) and
+ * lib/libampcor/dom/SLC.icc (operator[]). */
+int ampcor_hip_raster_read(const ampcor_hip_raster *raster, size_t offset, size_t count,
+                           void *host);
 int ampcor_hip_raster_destroy(ampcor_hip_raster *raster);
 
 /* -- plans ----------------------------------------------------------------- */

@@ -27,9 +27,15 @@ def test_library_exports_every_declared_symbol():
 
 def test_module_mirrors_the_cuda_extension_entry_points():
     lib = ext.require()
-    for name in ("sequential", "addReference", "addTarget", "adjust", "slc_map", "slc_pixelSize"):
+    for name in ("sequential", "addReference", "addTarget", "adjust", "refine", "slc_map",
+                 "slc_pixelSize", "slc_getitem"):
         assert callable(getattr(lib, name))
     assert lib.slc_pixelSize() == 8
+    # invalid capsules raise TypeError, as in ext/ampcor/sequential.cc:277-283 and slc.cc:124-129
+    with pytest.raises(TypeError):
+        lib.refine(object())
+    with pytest.raises(TypeError):
+        lib.slc_getitem(object(), 0)
     assert issubclass(lib.Error, Exception)
 
 
@@ -58,6 +64,7 @@ def test_c_abi_status_codes_without_a_device():
     assert L.ampcor_hip_adjust(None, None) == 1
     assert L.ampcor_hip_plan_destroy(None) == 0
     assert L.ampcor_hip_raster_destroy(None) == 0
+    assert L.ampcor_hip_raster_read(None, ctypes.c_size_t(0), ctypes.c_size_t(1), None) == 1
     n = ctypes.c_int(-1)
     assert L.ampcor_hip_device_count(ctypes.byref(n)) == 0 and n.value >= 0
 

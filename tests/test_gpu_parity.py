@@ -133,3 +133,30 @@ def test_wide_refined_surface_unstaged_zoom(lib):
                   s.refine_factor, s.refine_margin, s.zoom_factor)
     twin, exact = oracles(s)
     check_against_oracles(hip, twin, exact)
+
+
+def test_slc_getitem_and_refine(lib):
+    # slc_getitem reads one pixel at a row-major offset (ext/ampcor/slc.cc:105-140);
+    # refine is the CPU module's separate call, a no-op after adjust
+    rng = np.random.default_rng(12)
+    z = (rng.standard_normal((7, 9)) + 1j * rng.standard_normal((7, 9))).astype(np.complex64)
+    a = rng.standard_normal((5, 6)).astype(np.float32)
+    rz = lib.raster(z, 7, 9)
+    ra = lib.raster(a, 5, 6, dtype="float32")
+    for off in (0, 1, 8, 9, 40, 62):
+        assert lib.slc_getitem(rz, off) == complex(z.reshape(-1)[off])
+    for off in (0, 7, 29):
+        assert lib.slc_getitem(ra, off) == complex(float(a.reshape(-1)[off]), 0.0)
+    with pytest.raises(IndexError):
+        lib.slc_getitem(rz, 63)
+    s = synth.shifted_scene(seed=5, size=128, grid=(2, 2))
+    hip = run_hip(lib, s.reference, s.target, s.ref_begin, s.tgt_begin, s.chip, s.window,
+                  s.refine_factor, s.refine_margin, s.zoom_factor)
+    worker = lib.sequential(s.pairs, (s.chip, s.chip), (s.window, s.window), s.refine_margin,
+                            s.refine_factor, s.zoom_factor)
+    r = lib.raster(s.reference, *s.reference.shape)
+    t = lib.raster(s.target, *s.target.shape)
+    lib.addPairs(worker, r, t, 0, s.ref_begin.tobytes(), s.tgt_begin.tobytes())
+    offsets = np.array(lib.adjust(worker), np.float32)
+    assert lib.refine(worker) is None
+    assert np.array_equal(offsets, hip["offsets"])

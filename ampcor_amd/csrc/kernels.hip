@@ -2386,7 +2386,7 @@ hipError_t launch_upsample(hipStream_t s, const PairDesc *desc, int pairs, int d
     hipLaunchKernelGGL((k_upsample2h<DT, QM, MN>), grid, block, lds, s, desc, role, window, N, Np32, \
                        hf, amp, q, cs, stats)
         if (dtype == 0) {
-            if (Np32 <= 64) {
+            if (Np32 <= 64) {   // 76 VGPRs: three workgroups per CU instead of two
                 if (qmode == kUpFP16) AMPCOR_UP2H(0, kUpFP16, 64);
                 else AMPCOR_UP2H(0, kUpBF16, 64);
             } else {
@@ -2394,8 +2394,13 @@ hipError_t launch_upsample(hipStream_t s, const PairDesc *desc, int pairs, int d
                 else AMPCOR_UP2H(0, kUpBF16, kUp2hMaxN);
             }
         } else {
-            if (qmode == kUpFP16) AMPCOR_UP2H(1, kUpFP16, kUp2hMaxNReal);
-            else AMPCOR_UP2H(1, kUpBF16, kUp2hMaxNReal);
+            if (Np32 <= 128) {   // 102 VGPRs: two workgroups per CU instead of one
+                if (qmode == kUpFP16) AMPCOR_UP2H(1, kUpFP16, 128);
+                else AMPCOR_UP2H(1, kUpBF16, 128);
+            } else {
+                if (qmode == kUpFP16) AMPCOR_UP2H(1, kUpFP16, kUp2hMaxNReal);
+                else AMPCOR_UP2H(1, kUpBF16, kUp2hMaxNReal);
+            }
         }
 #undef AMPCOR_UP2H
         return hipGetLastError();

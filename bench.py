@@ -73,21 +73,32 @@ def cpu_baseline(scene, seconds):
     args = (scene.reference, scene.target)
     geo = (scene.chip, scene.window, scene.refine_factor, scene.refine_margin, scene.zoom_factor)
 
-    def run(n):
+    def run(n, threads, form="fp64"):
         idx = np.unique(np.linspace(0, scene.pairs - 1, n).astype(int))
         t0 = time.perf_counter()
         out = pyoracle.adjust(*args, scene.ref_begin[idx], scene.tgt_begin[idx], *geo,
-                              form="fp64", threads=cores)
-        return time.perf_counter() - t0, idx, out["offsets"]
+                              form=form, threads=threads)
+        return time.perf_counter() - t0, idx, out
 
     n = 2 * cores
-    dt, _, _ = run(n)
+    dt, _, _ = run(n, cores)
     n2 = int(min(scene.pairs, max(n, n * seconds / max(dt, 1e-3))))
     n2 = max(cores, (n2 // cores) * cores)
-    dt, idx, offsets = run(n2)
+    dt, idx, exact = run(n2, cores)
+    # the same oracle on one core (the reference's CPU path is single-threaded),
+    # on a sample of about a third of the multi-core run's duration
+    per_pair_1 = dt * cores / len(idx)
+    n1 = int(max(4, min(len(idx), seconds / 3 / max(per_pair_1, 1e-6))))
+    dt1, idx1, _ = run(n1, 1)
+    # the FP32 twin (the GPU coarse stage's rounding) on a quarter of the sample
+    _, idx_t, twin = run(max(cores, len(idx) // 4), cores, form="fp32")
     return ({"value": len(idx) / dt, "unit": "tile pairs/sec", "cores": cores, "kind": "port",
              "sample": f"FP64 oracle (oracle/ampcor_oracle.c, OpenMP over pairs) on {len(idx)} "
-                       f"of the {scene.pairs} pairs, evenly spaced, {dt:.1f} s"}, idx, offsets)
+                       f"of the {scene.pairs} pairs, evenly spaced, {dt:.1f} s",
+             "single_core": {"value": len(idx1) / dt1, "cores": 1,
+                             "sample": f"the same oracle on one core, {len(idx1)} pairs, "
+                                       f"{dt1:.1f} s"}},
+            idx, exact, idx_t, twin)
 
 
 def stage_work(precision, scene, dtype):
@@ -200,17 +211,21 @@ def main():
             for k, v in lib.stage_times(worker).items():
                 stage_sum[k] = stage_sum.get(k, 0.0) + v
         stages = {k: v / a.steps for k, v in stage_sum.items()}
-        off = np.frombuffer(lib.products(worker)["offsets"], np.float32).reshape(N, 2).copy()
+        prods = {k: np.frombuffer(v, np.float32).copy() for k, v in lib.products(worker).items()}
+        prods["offsets"] = prods["offsets"].reshape(N, 2)
+        prods["coarse_loc"] = np.frombuffer(lib.stage_locations(worker)["coarse"],
+                                            np.int32).reshape(N, 2).copy()
         print(f"[bench] {precision} stage ms/step: " +
               ", ".join(f"{k} {v:.3f}" for k, v in stages.items()), file=sys.stderr, flush=True)
-        return elapsed / a.steps, stages, off
+        return elapsed / a.steps, stages, prods
 
     forms = ["fp32"] if a.precision == "fp32" else ["fp32", a.precision]
     for f in (x.strip() for x in a.extra_forms.split(",")):
         if f and f not in forms:
             forms.append(f)
     runs = {f: timed(f) for f in forms}
-    step_s, stages, off = runs[a.precision]
+    step_s, stages, prods = runs[a.precision]
+    off = prods["offsets"]
     ms_per_step = 1e3 * step_s
     value = N / step_s
     fp32_value = N / runs["fp32"][0]
@@ -242,11 +257,26 @@ def main():
     cpu = None
     accuracy = None
     if not a.no_cpu:
-        cpu, idx, exact = cpu_baseline(scene, a.cpu_seconds)
+        cpu, idx, exact, idx_t, twin = cpu_baseline(scene, a.cpu_seconds)
         print(f"[bench] cpu baseline: {cpu}", file=sys.stderr, flush=True)
-        accuracy = {f"{f}_max_offset_err_px_vs_fp64": float(np.max(np.abs(runs[f][2][idx] - exact)))
+        accuracy = {f"{f}_max_offset_err_px_vs_fp64":
+                    float(np.max(np.abs(runs[f][2]["offsets"][idx] - exact["offsets"])))
                     for f in forms}
+        rel = lambda x, y: float(np.max(np.abs(x.astype(np.float64) - y) / np.abs(y)))
+        for f in forms:
+            p = runs[f][2]
+            accuracy[f"{f}_coarse_loc_equal_fp64"] = int(
+                np.all(p["coarse_loc"][idx] == exact["coarse_loc"], axis=1).sum())
+            accuracy[f"{f}_coarse_peak_max_rel_err_vs_fp64"] = rel(p["coarse_peak"][idx],
+                                                                   exact["coarse_peak"])
+            accuracy[f"{f}_snr_max_rel_err_vs_fp64"] = rel(p["snr"][idx], exact["snr"])
+        accuracy["fp32_coarse_loc_equal_fp32_twin"] = int(
+            np.all(runs["fp32"][2]["coarse_loc"][idx_t] == twin["coarse_loc"], axis=1).sum())
+        accuracy["fp32_coarse_peak_bitwise_equal_fp32_twin"] = int(
+            (runs["fp32"][2]["coarse_peak"][idx_t].view(np.uint32)
+             == twin["coarse_peak"].view(np.uint32)).sum())
         accuracy["pairs_compared"] = int(len(idx))
+        accuracy["pairs_compared_fp32_twin"] = int(len(idx_t))
         print(f"[bench] accuracy vs FP64 oracle: {accuracy}", file=sys.stderr, flush=True)
 
     serving = None

@@ -160,3 +160,37 @@ def test_slc_getitem_and_refine(lib):
     offsets = np.array(lib.adjust(worker), np.float32)
     assert lib.refine(worker) is None
     assert np.array_equal(offsets, hip["offsets"])
+
+
+def test_tile_registration_errors(lib):
+    # the host checks that keep every gather inside its raster (no kernel runs
+    # on a rejected pair): slices past the raster, wrong shapes, pair indices
+    # out of range, mixed pixel types, and an adjust with a pair missing
+    rng = np.random.default_rng(21)
+    z = (rng.standard_normal((96, 80)) + 1j * rng.standard_normal((96, 80))).astype(np.complex64)
+    a = np.zeros((96, 80), np.float32)
+    rz = lib.raster(z, 96, 80)
+    ra = lib.raster(a, 96, 80, dtype="float32")
+    worker = lib.sequential(2, (16, 16), (32, 32), 4, 2, 4)
+    with pytest.raises(ValueError, match="beyond the raster"):
+        lib.addReference(worker, rz, 0, (90, 0), (106, 16))
+    with pytest.raises(ValueError, match="beyond the raster"):
+        lib.addTarget(worker, rz, 0, (0, 60), (32, 92))
+    with pytest.raises(ValueError, match="shape"):
+        lib.addReference(worker, rz, 0, (0, 0), (16, 17))
+    with pytest.raises(ValueError, match="out of range"):
+        lib.addTarget(worker, rz, 2, (0, 0), (32, 32))
+    with pytest.raises(ValueError, match="negative"):
+        lib.addPairs(worker, rz, rz, 0, np.array([[0, -1]], np.int64).tobytes(),
+                     np.array([[0, 0]], np.int64).tobytes())
+    lib.addReference(worker, rz, 0, (0, 0), (16, 16))
+    with pytest.raises(ValueError, match="pixel type"):
+        lib.addTarget(worker, ra, 0, (0, 0), (32, 32))
+    lib.addTarget(worker, rz, 0, (0, 0), (32, 32))
+    with pytest.raises(lib.Error, match="pair 1"):
+        lib.adjust(worker)
+    # once every pair is registered the same worker runs
+    lib.addReference(worker, rz, 1, (40, 40), (56, 56))
+    lib.addTarget(worker, rz, 1, (32, 32), (64, 64))
+    offsets = np.array(lib.adjust(worker), np.float32)
+    assert offsets.shape == (2, 2) and np.all(np.isfinite(offsets))

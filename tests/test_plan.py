@@ -116,3 +116,28 @@ def test_plan_records_each_tiles_initial_shift():
     assert plan.shift.tolist() == [[1, 2], [3, 4], [5, 6], [7, 8]]
     r, t = plan.corners()
     assert np.array_equal(t - r, plan.shift - 16)
+
+
+def test_next_pass_field_from_an_estimate_without_a_device():
+    # MGA.offset_field / next_pass on products set by hand (no strategy runs)
+    from ampcor_amd import MGA
+
+    ref, tgt = SLC(shape=(400, 400)), SLC(shape=(400, 400))
+    mga = MGA(chip=(32, 32), padding=(16, 16),
+              coarse=Grid(domain=UniformGrid(shape=(2, 2)), functor=Constant(shift=(5, -3))))
+    mga.plan = mga.makePlan(regmap=mga.coarse.map(reference=ref), rasters=(ref, tgt))
+    n = len(mga.plan)
+    assert n == 4
+    mga.products = {"offsets": np.array([[0.25, -0.5], [0.5, 0.25], [-0.25, 0.0], [1.0, 1.0]],
+                                        np.float32),
+                    "coarse_peak": np.full(n, 0.9, np.float32), "snr": np.full(n, 30.0, np.float32)}
+    rel = mga.offset_field()
+    tot = mga.offset_field(total=True)
+    assert rel.shape == (2, 2, 4)
+    assert np.allclose(tot[..., 0:2] - rel[..., 0:2], [5, -3])
+    coarse = mga.next_pass(shape=(2, 2))
+    offmap = coarse.map(reference=ref)
+    # the next pass's guesses on the same grid: the rounded total offsets
+    got = np.array(offmap.codomain) - np.array(offmap.domain)
+    want = np.floor(tot[..., 0:2].reshape(-1, 2) + 0.5).astype(int)
+    assert np.array_equal(got, want)

@@ -822,13 +822,21 @@ __global__ __launch_bounds__(256) void k_coarse_peak(const float *__restrict__ g
         const int col = none ? 0 : bidx - row * P;
         double sum = 0.0;
         int count = 0;
+        // (u, v) of cell lane + 64 j, stepped by 64 cells without divisions
+        const int du = kWave / P, dv = kWave - du * P;
+        int u = lane / P, v = lane - u * P;
 #pragma unroll
         for (int j = 0; j < KEEP; ++j) {
-            const int k = lane + j * kWave;
-            const int u = k / P, v = k - u * P;
-            if (x[j] != x[j] || (abs(u - row) <= exclude && abs(v - col) <= exclude)) continue;
-            sum += (double)x[j] * (double)x[j];
-            ++count;
+            if (x[j] == x[j] && !(abs(u - row) <= exclude && abs(v - col) <= exclude)) {
+                sum += (double)x[j] * (double)x[j];
+                ++count;
+            }
+            u += du;
+            v += dv;
+            if (v >= P) {
+                v -= P;
+                ++u;
+            }
         }
         coarse_peak_finish(pair, lane, P, W, E, rm, none, row, col, none ? __builtin_nanf("") : best,
                            sum, count, loc, window, peak, snr);

@@ -14,7 +14,7 @@ the field adds the guess back.
 import numpy as np
 
 from .hip import HIP
-from .plan import Field, Grid, Plan, UniformGrid
+from .plan import Field, Grid, OffsetMap, Plan, UniformGrid
 
 
 class MGA:
@@ -62,6 +62,17 @@ class MGA:
         field[valid, 2] = p["coarse_peak"]
         field[valid, 3] = p["snr"]
         return field.reshape(tuple(self.plan.tile) + (4,))
+
+    def offset_map(self):
+        """The registration map this estimate found: an ``OffsetMap`` on the tile
+        grid whose codomain is each reference point plus its total offset (float
+        target positions; NaN for invalid pairs)."""
+        total = self.offset_field(total=True)[..., 0:2].astype(np.float64).reshape(-1, 2)
+        offmap = OffsetMap(shape=self.plan.tile)
+        offmap.domain = list(self.plan.domain)
+        offmap.codomain = [tuple(float(p) + d for p, d in zip(point, off))
+                           for point, off in zip(self.plan.domain, total)]
+        return offmap
 
     def next_pass(self, shape, fallback=(0, 0)):
         """The coarse map of a further pass: a ``shape`` grid of tiles whose

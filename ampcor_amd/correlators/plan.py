@@ -111,13 +111,47 @@ class Field:
 
 
 class OffsetMap:
-    """A coarse offset map: reference points (domain) and their target images."""
+    """A logically Cartesian map from reference points (domain) to their target
+    images (codomain), as pkg/ampcor/dom/OffsetMap.py:14-95: ``map[index]`` is
+    the pair of points at a grid index, stored row-major (the last index
+    varies fastest)."""
 
     def __init__(self, shape):
-        self.shape = tuple(shape)
+        self.shape = tuple(int(s) for s in shape)
         self.tile = self.shape
-        self.domain = ()
-        self.codomain = ()
+        self.layout = tuple(reversed(range(len(self.shape))))
+        n = self.size
+        self.domain = [None] * n
+        self.codomain = [None] * n
+
+    @property
+    def size(self):
+        n = 1
+        for s in self.shape:
+            n *= s
+        return n
+
+    def offset(self, index):
+        """Row-major offset of a grid index; IndexError outside the grid."""
+        index = tuple(index) if isinstance(index, (tuple, list)) else (index,)
+        if len(index) != len(self.shape):
+            raise IndexError(f"index {index} has the wrong rank for shape {self.shape}")
+        off = 0
+        for i, s in zip(index, self.shape):
+            if not 0 <= i < s:
+                raise IndexError(f"index {index} outside the grid {self.shape}")
+            off = off * s + i
+        return off
+
+    def __getitem__(self, index):
+        off = self.offset(index)
+        return self.domain[off], self.codomain[off]
+
+    def __setitem__(self, index, points):
+        off = self.offset(index)
+        ref, tgt = points
+        self.domain[off] = ref
+        self.codomain[off] = tgt
 
 
 class Grid:
@@ -129,8 +163,8 @@ class Grid:
 
     def map(self, reference):
         offmap = OffsetMap(shape=self.domain.shape)
-        offmap.domain = tuple(self.domain.points(bounds=reference.shape))
-        offmap.codomain = tuple(self.functor.codomain(domain=offmap.domain))
+        offmap.domain = list(self.domain.points(bounds=reference.shape))
+        offmap.codomain = list(self.functor.codomain(domain=offmap.domain))
         return offmap
 
 

@@ -141,3 +141,38 @@ def test_next_pass_field_from_an_estimate_without_a_device():
     got = np.array(offmap.codomain) - np.array(offmap.domain)
     want = np.floor(tot[..., 0:2].reshape(-1, 2) + 0.5).astype(int)
     assert np.array_equal(got, want)
+
+
+def test_offset_map_is_indexed_row_major():
+    # pkg/ampcor/dom/OffsetMap.py:14-95: map[index] is the (reference, target) pair at a grid index
+    from ampcor_amd.correlators import OffsetMap
+
+    m = Grid(domain=UniformGrid(shape=(2, 3)), functor=Constant(shift=(1, -1))).map(
+        reference=SLC(shape=(100, 90)))
+    assert m.shape == (2, 3) and m.size == 6 and m.layout == (1, 0)
+    assert m[0, 0] == ((25, 15), (26, 14))
+    assert m[1, 2] == ((75, 75), (76, 74))
+    m[1, 2] = ((75, 75), (80, 70))
+    assert m.codomain[5] == (80, 70)
+    with pytest.raises(IndexError):
+        m[2, 0]
+    with pytest.raises(IndexError):
+        m[0]
+    assert OffsetMap(shape=(4, 5)).size == 20
+
+
+def test_registration_map_of_an_estimate_without_a_device():
+    from ampcor_amd import MGA
+
+    ref, tgt = SLC(shape=(400, 400)), SLC(shape=(400, 400))
+    mga = MGA(chip=(32, 32), padding=(16, 16),
+              coarse=Grid(domain=UniformGrid(shape=(2, 2)), functor=Constant(shift=(5, -3))))
+    mga.plan = mga.makePlan(regmap=mga.coarse.map(reference=ref), rasters=(ref, tgt))
+    mga.products = {"offsets": np.array([[0.25, -0.5], [0.5, 0.25], [-0.25, 0.0], [1.0, 1.0]],
+                                        np.float32),
+                    "coarse_peak": np.full(4, 0.9, np.float32), "snr": np.full(4, 30.0, np.float32)}
+    m = mga.offset_map()
+    assert m.shape == (2, 2)
+    r, t = m[0, 1]
+    # pair (0, 1): offsets (0.5, 0.25) on the guess (5, -3)
+    assert r == (100, 300) and t == (100 + 5.5, 300 - 3 + 0.25)

@@ -192,6 +192,18 @@ class Plan:
     def __len__(self):
         return len(tuple(filter(None, self.reference)))
 
+    def __getitem__(self, index):
+        """The (reference, target) slices at a tile-grid index (None, None when invalid)."""
+        off = OffsetMap(shape=self.tile).offset(index)
+        return self.reference[off], self.target[off]
+
+    @property
+    def footprint(self):
+        """Bytes of the valid reference and target tiles (Plan.py:48-59)."""
+        ref = sum(t.footprint for t in filter(None, self.reference))
+        tgt = sum(t.footprint for t in filter(None, self.target))
+        return ref, tgt
+
     @property
     def valid(self):
         """Boolean mask over the tile grid (row-major) of the valid pairs."""

@@ -29,6 +29,10 @@ class Slice:
     def __repr__(self):
         return f"Slice(begin={self.begin}, shape={self.shape})"
 
+    def __getitem__(self, index):
+        """The raster's pixel at ``index`` relative to the slice (Slice.py:70-77)."""
+        return self.raster[tuple(b + i for b, i in zip(self.begin, index))]
+
 
 class SLC:
     """A complex64 (or float32 amplitude) raster of shape (lines, samples)."""
@@ -54,6 +58,21 @@ class SLC:
     def size(self):
         lines, samples = self.shape
         return lines * samples * self.pixelSize
+
+    def __getitem__(self, index):
+        """The pixel at a (line, sample) index, as a Python complex (SLC.py:102-110).
+
+        Read from the device copy once the raster is open (``slc_getitem`` of
+        the extension, at the row-major offset), else from the host data."""
+        line, sample = index
+        lines, samples = self.shape
+        if not (0 <= line < lines and 0 <= sample < samples):
+            raise IndexError(f"index {index} outside the raster {self.shape}")
+        if self.raster is not None:
+            return ext.require().slc_getitem(self.raster, line * samples + sample)
+        if self.data is None:
+            raise ValueError("the raster has no data: open it or give it data")
+        return complex(self.data[line, sample])
 
     def slice(self, begin, shape):
         """A slice, or ``None`` when it starts before or ends past the raster."""

@@ -24,7 +24,13 @@ def test_estimate_equals_the_worker_path_and_fills_the_tile_grid(lib):
     mga = MGA(chip=(s.chip, s.chip), padding=(s.margin, s.margin),
               refineMargin=s.refine_margin, refineFactor=s.refine_factor,
               zoomFactor=s.zoom_factor, coarse=Grid(domain=UniformGrid(shape=s.grid)))
-    offsets = np.array(mga.estimate(SLC(data=s.reference), SLC(data=s.target)), np.float32)
+    ref, tgt = SLC(data=s.reference), SLC(data=s.target)
+    offsets = np.array(mga.estimate(ref, tgt), np.float32)
+    # the opened rasters answer pixel reads from the device copy (SLC.py:102-110)
+    assert ref.raster is not None
+    assert ref[5, 7] == complex(s.reference[5, 7]) and tgt[300, 2] == complex(s.target[300, 2])
+    assert mga.plan[0, 0][0][3, 4] == complex(s.reference[s.ref_begin[0][0] + 3,
+                                                          s.ref_begin[0][1] + 4])
     direct = run_hip(lib, s.reference, s.target, s.ref_begin, s.tgt_begin, s.chip, s.window,
                      s.refine_factor, s.refine_margin, s.zoom_factor)
     assert np.array_equal(offsets.view(np.uint32), direct["offsets"].view(np.uint32))

@@ -176,3 +176,24 @@ def test_registration_map_of_an_estimate_without_a_device():
     r, t = m[0, 1]
     # pair (0, 1): offsets (0.5, 0.25) on the guess (5, -3)
     assert r == (100, 300) and t == (100 + 5.5, 300 - 3 + 0.25)
+
+
+def test_plan_indexing_and_footprint():
+    # Plan.py:48-59 (footprint) and :129-141 (grid indexing)
+    plan = plan_for((512, 512), (8, 8), 32, 16, shift=(0, 0))
+    r, t = plan[0, 0]
+    assert r.begin == (16, 16) and t.begin == (0, 0)
+    assert plan.footprint == (64 * 32 * 32 * 8, 64 * 64 * 64 * 8)
+    edge = plan_for((100, 100), (2, 2), 32, 16)
+    assert edge[0, 0] == (None, None)   # the window would start before the raster
+    with pytest.raises(IndexError):
+        plan[8, 0]
+
+
+def test_slc_and_slice_pixels_from_host_data():
+    z = (np.arange(12) + 1j * np.arange(12)[::-1]).astype(np.complex64).reshape(3, 4)
+    slc = SLC(data=z)
+    assert slc[1, 2] == complex(z[1, 2])
+    assert slc.slice((1, 1), (2, 2))[1, 0] == complex(z[2, 1])
+    with pytest.raises(IndexError):
+        slc[3, 0]

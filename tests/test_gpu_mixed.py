@@ -96,3 +96,24 @@ def test_mixed_window_not_a_multiple_of_8(lib, precision):
     print(f"{precision} W=52: max |offset - fp64| = {err.max():.2e} px", flush=True)
     assert err.max() <= MIXED_OFFSET_TOL
     assert np.mean(np.all(hip["coarse_loc"] == exact["coarse_loc"], axis=1)) >= 0.98
+
+
+@pytest.fixture(scope="module")
+def wide_complex():
+    # complex tiles above 96 samples (chip 128, refined window 144): the banded
+    # two-pass upsampler in its quantized modes, as the serving stream's chip-128 jobs use it
+    s = synth.shifted_scene(seed=17, size=640, grid=(2, 2), chip=128, margin=32,
+                            refine_margin=8, zoom_factor=4, shift=(1.3, -2.6))
+    return s, fp64(s)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16", "fp16", "bf16x3"])
+def test_complex_chip_128_banded_upsampler(lib, wide_complex, precision):
+    s, exact = wide_complex
+    print(f"wide complex tiles: {s.pairs} pairs, chip {s.chip}, window {s.window}, {precision}",
+          flush=True)
+    hip = run(lib, s, precision)
+    err = np.abs(hip["offsets"] - exact["offsets"])
+    assert err.max() <= (1e-3 if precision in ("fp32", "bf16x3") else MIXED_OFFSET_TOL)
+    assert np.array_equal(hip["coarse_loc"], exact["coarse_loc"])
+    assert np.max(np.abs(hip["offsets"] - s.truth)) < 0.05

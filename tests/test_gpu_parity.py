@@ -194,3 +194,20 @@ def test_tile_registration_errors(lib):
     lib.addTarget(worker, rz, 1, (32, 32), (64, 64))
     offsets = np.array(lib.adjust(worker), np.float32)
     assert offsets.shape == (2, 2) and np.all(np.isfinite(offsets))
+
+
+@pytest.mark.parametrize("rf,rm,zf", [(3, 4, 4), (4, 3, 2)])
+def test_refine_factors_other_than_two(lib, rf, rm, zf):
+    # the dense U x U^T upsampler (k_upsample) for refine factors other than 2
+    s = synth.shifted_scene(seed=23, size=256, grid=(3, 3), chip=32, margin=16, refine_factor=rf,
+                            refine_margin=rm, zoom_factor=zf, shift=(2.4, -1.1))
+    print(f"refine factor {rf}, margin {rm}, zoom {zf}: {s.pairs} pairs", flush=True)
+    hip = run_hip(lib, s.reference, s.target, s.ref_begin, s.tgt_begin, s.chip, s.window, rf, rm,
+                  zf)
+    twin, exact = oracles(s)
+    check_against_oracles(hip, twin, exact)
+    for precision in ("bf16", "fp16"):
+        mixed = run_hip(lib, s.reference, s.target, s.ref_begin, s.tgt_begin, s.chip, s.window,
+                        rf, rm, zf, precision=precision)
+        assert np.max(np.abs(mixed["offsets"] - exact["offsets"])) <= 1e-2
+        assert np.array_equal(mixed["coarse_loc"], exact["coarse_loc"])

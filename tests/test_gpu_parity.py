@@ -211,3 +211,20 @@ def test_refine_factors_other_than_two(lib, rf, rm, zf):
                         rf, rm, zf, precision=precision)
         assert np.max(np.abs(mixed["offsets"] - exact["offsets"])) <= 1e-2
         assert np.array_equal(mixed["coarse_loc"], exact["coarse_loc"])
+
+
+def test_refine_factor_one_and_a_chip_of_160(lib):
+    # no upsampling (refine factor 1) and a chip above 128 samples: the mixed
+    # forms' coarse stage then gathers FP32 amplitudes and quantizes them in a
+    # separate pass (k_gather_detect + k_prep_mixed)
+    s = synth.shifted_scene(seed=29, size=512, grid=(2, 2), chip=160, margin=16, refine_factor=1,
+                            refine_margin=4, zoom_factor=8, shift=(-3.6, 2.2))
+    print(f"chip 160, refine factor 1: {s.pairs} pairs", flush=True)
+    hip = run_hip(lib, s.reference, s.target, s.ref_begin, s.tgt_begin, s.chip, s.window, 1, 4, 8)
+    twin, exact = oracles(s)
+    check_against_oracles(hip, twin, exact)
+    for precision in ("bf16", "fp16"):
+        mixed = run_hip(lib, s.reference, s.target, s.ref_begin, s.tgt_begin, s.chip, s.window,
+                        1, 4, 8, precision=precision)
+        assert np.max(np.abs(mixed["offsets"] - exact["offsets"])) <= 1e-2
+        assert np.array_equal(mixed["coarse_loc"], exact["coarse_loc"])

@@ -117,3 +117,18 @@ def test_complex_chip_128_banded_upsampler(lib, wide_complex, precision):
     assert err.max() <= (1e-3 if precision in ("fp32", "bf16x3") else MIXED_OFFSET_TOL)
     assert np.array_equal(hip["coarse_loc"], exact["coarse_loc"])
     assert np.max(np.abs(hip["offsets"] - s.truth)) < 0.05
+
+
+@pytest.mark.parametrize("precision", ["bf16", "fp16"])
+def test_surface_above_144_placements(lib, precision):
+    # 145 x 145 placements (margin 72): past the fused normalizer's reach, the
+    # mixed coarse stage sums the window in FP64 separately (k_window_sums +
+    # k_normalize_mixed), and the coarse peak reads the surface twice
+    s = synth.shifted_scene(seed=31, size=512, grid=(2, 2), chip=64, margin=72, refine_margin=8,
+                            zoom_factor=4, shift=(5.7, -4.4))
+    print(f"145 x 145 placements: {s.pairs} pairs, {precision}", flush=True)
+    exact = fp64(s)
+    hip = run(lib, s, precision)
+    assert np.max(np.abs(hip["offsets"] - exact["offsets"])) <= MIXED_OFFSET_TOL
+    assert np.array_equal(hip["coarse_loc"], exact["coarse_loc"])
+    np.testing.assert_allclose(hip["coarse_peak"], exact["coarse_peak"], rtol=PEAK_REL_TOL)

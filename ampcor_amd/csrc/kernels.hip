@@ -460,8 +460,12 @@ __device__ inline void lds_barrier()
     __builtin_amdgcn_s_barrier();
 }
 
-template <int FORM, int COLS, int JO>
-__global__ __launch_bounds__(kFuseThreads, COLS == 1 && JO < 9 ? 4 : 2) void k_normalize_fused(
+// LAZY: the outputs add each chunk's offset to the chunk prefixes they read,
+// instead of a second pass over every entry (the same FP64 sums); it needs a
+// few more registers, so its workgroups are held to three per CU -- the LDS of
+// the wide windows it is chosen for allows no more anyway
+template <int FORM, int COLS, int JO, bool LAZY>
+__global__ __launch_bounds__(kFuseThreads, COLS == 1 && JO < 9 ? (LAZY ? 3 : 4) : 2) void k_normalize_fused(
     const void *__restrict__ target, int W, int C, const float *__restrict__ partial, int nsplit,
     const double2 *__restrict__ stats, const MixedStats *__restrict__ mstats,
     float *__restrict__ gamma)
@@ -611,6 +615,9 @@ __global__ __launch_bounds__(kFuseThreads, COLS == 1 && JO < 9 ? 4 : 2) void k_n
     // prefix chunk per slot, odd: the 16 slots of a row start in distinct LDS banks
     const int cw = ((W + 15) / 16) | 1;
     const int x0 = min(W, sl * cw), x1 = min(W, x0 + cw);
+    // (lazy form) chunk of column x = (x * inv_cw) >> 20, exact for x < 512 and cw <= 33
+    const int inv_cw = (1 << 20) / cw + 1;
+    __shared__ double off1[LAZY ? kFuseRows : 1][16], off2[LAZY ? kFuseRows : 1][16];
 
     for (int u0 = 0; u0 < P; u0 += kFuseRows) {
         const int rows = min(kFuseRows, P - u0);
@@ -690,7 +697,11 @@ __global__ __launch_bounds__(kFuseThreads, COLS == 1 && JO < 9 ? 4 : 2) void k_n
         }
         o1 -= run1;   // exclusive: the totals of the chunks to the left
         o2 -= run2;
-        if (r < rows && sl > 0)
+        if (LAZY && r < rows) {
+            off1[r][sl] = o1;
+            off2[r][sl] = o2;
+        }
+        if (!LAZY && r < rows && sl > 0)
             for (int xb = x0; xb < x1; xb += kFuseScan) {
                 double e1[kFuseScan], e2[kFuseScan];
 #pragma unroll
@@ -711,8 +722,24 @@ __global__ __launch_bounds__(kFuseThreads, COLS == 1 && JO < 9 ? 4 : 2) void k_n
             for (int j = 0; j < JO; ++j) {
                 const int v = sl + 16 * j;
                 if (v >= P) break;
-                const double S1 = p1[r * ws + v + C - 1] - (v > 0 ? p1[r * ws + v - 1] : 0.0);
-                const double S2 = p2[r * ws + v + C - 1] - (v > 0 ? p2[r * ws + v - 1] : 0.0);
+                double S1, S2;
+                if constexpr (LAZY) {
+                    // the row's prefix sums: chunk prefix + chunk offset
+                    const int xa = v + C - 1, ca = (xa * inv_cw) >> 20;
+                    S1 = p1[r * ws + xa] + off1[r][ca];
+                    S2 = p2[r * ws + xa] + off2[r][ca];
+                    if (v > 0) {
+                        const int cb = ((v - 1) * inv_cw) >> 20;
+                        S1 = S1 - (p1[r * ws + v - 1] + off1[r][cb]);
+                        S2 = S2 - (p2[r * ws + v - 1] + off2[r][cb]);
+                    } else {
+                        S1 = S1 - 0.0;
+                        S2 = S2 - 0.0;
+                    }
+                } else {
+                    S1 = p1[r * ws + v + C - 1] - (v > 0 ? p1[r * ws + v - 1] : 0.0);
+                    S2 = p2[r * ws + v + C - 1] - (v > 0 ? p2[r * ws + v - 1] : 0.0);
+                }
                 // FP64 throughout: the centred sums cancel, and a tiny variance
                 // must not flush to zero
                 // (the BF16 / FP16 / mixed forms multiply by 1 / n: one FP64
@@ -2326,9 +2353,17 @@ hipError_t launch_normalize_fused(hipStream_t s, int form, const void *target, i
     const bool one = W <= kFuseThreads;   // one target column per thread
     // outputs per thread and block: 3, 5 or 9 (P <= 48, 80, 144)
     const int P = W - C + 1, jo = P <= 48 ? 3 : P <= 80 ? 5 : 9;
-#define AMPCOR_FUSE_J(F, CO, J)                                                                     \
-    hipLaunchKernelGGL((k_normalize_fused<F, CO, J>), dim3(pairs), dim3(kFuseThreads), lds, s,      \
+#define AMPCOR_FUSE_K(F, CO, J, LZ)                                                                 \
+    hipLaunchKernelGGL((k_normalize_fused<F, CO, J, LZ>), dim3(pairs), dim3(kFuseThreads), lds, s,  \
                        target, W, C, partial, nsplit, stats, mstats, gamma)
+    // wide windows with few outputs per thread: the lazy offsets (measured faster
+    // for the 160-sample refined window; slower where four workgroups fit a CU)
+    const bool lazy = one && jo == 3 && W >= 144;
+#define AMPCOR_FUSE_J(F, CO, J)                                                                     \
+    do {                                                                                            \
+        if (CO == 1 && J == 3 && lazy) AMPCOR_FUSE_K(F, CO, J, (CO == 1 && J == 3));                \
+        else AMPCOR_FUSE_K(F, CO, J, false);                                                        \
+    } while (0)
 #define AMPCOR_FUSE_C(F, CO)                                                                        \
     do {                                                                                            \
         if (jo == 3) AMPCOR_FUSE_J(F, CO, 3);                                                       \
@@ -2351,6 +2386,7 @@ hipError_t launch_normalize_fused(hipStream_t s, int form, const void *target, i
 #undef AMPCOR_FUSE
 #undef AMPCOR_FUSE_C
 #undef AMPCOR_FUSE_J
+#undef AMPCOR_FUSE_K
     return hipGetLastError();
 }
 

@@ -1322,7 +1322,7 @@ __host__ __device__ inline size_t up2h_lds_bytes(int Np, int dtype)
 }
 
 // MAXN: the largest padded tile edge of the instantiation; the register arrays
-// are sized for it, so a smaller MAXN (64 for the chip of the scene) leaves
+// are sized for it, so a smaller MAXN (64, for 64-sample chips) leaves
 // room for more workgroups per CU
 template <int DTYPE, int QMODE, int MAXN>
 __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2h(

@@ -1516,15 +1516,24 @@ __global__ __launch_bounds__(kUp2Waves * kWave) void k_upsample2h(
         }
     };
     // stage 2: odd columns, Y_s = T_s H^T for T_0 = x and T_1 = T, then the
-    // four phases of each output 2 x 2 block
+    // four phases of each output 2 x 2 block; each tile's circulant fragments
+    // are loaded during the tile before it
+    uint4 fnext[kMaxN / 32];
+    auto load_frag = [&](int tile) {
+        const int bt = tile % NT;
+#pragma unroll
+        for (int kc = 0; kc < kMaxN / 32; ++kc)
+            if (kc < KC) fnext[kc] = hfrag[(bt * KC + kc) * kWave + lane];
+    };
+    if (wave < ntile) load_frag(wave);
     for (int tile = wave; tile < ntile; tile += kUp2Waves) {
         const int at = tile / NT, bt = tile - at * NT;
         const int arow = (16 * at + lm) * Ls + 8 * lk;
         floatx4 xR = {0.0f, 0.0f, 0.0f, 0.0f}, xI = xR, yR = xR, yI = xR;
         uint4 fb[kMaxN / 32];
 #pragma unroll
-        for (int kc = 0; kc < kMaxN / 32; ++kc)
-            if (kc < KC) fb[kc] = hfrag[(bt * KC + kc) * kWave + lane];
+        for (int kc = 0; kc < kMaxN / 32; ++kc) fb[kc] = fnext[kc];
+        if (tile + kUp2Waves < ntile) load_frag(tile + kUp2Waves);
 #pragma unroll
         for (int kc = 0; kc < kMaxN / 32; ++kc) {
             if (kc >= KC) break;

@@ -253,8 +253,9 @@ def main():
     # HBM bytes per launch of that kernel, from the committed rocprofv3 --pmc
     # FETCH_SIZE / WRITE_SIZE passes (copy-calibrated) on this same workload
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic_scene8192x4096.json")
-    if os.path.exists(tfile) and a.workload == "scene":
+    tfile = os.path.join(ROOT, "profiles", "traffic_scene8192x4096.json" if a.workload == "scene"
+                         else "traffic_large8192x8192.json")
+    if os.path.exists(tfile):
         with open(tfile) as fh:
             traffic = json.load(fh).get("bytes_per_launch", {}).get(a.precision, {}).get(dom)
     peak = FP32_PEAK_TFLOPS if a.precision == "fp32" else MIXED_PEAK_TFLOPS

@@ -60,7 +60,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_correlate_fp8(
     constexpr int CS = G::CS, CS4 = G::CS4, NF = G::NF, ncols = G::ncols, TS = G::TS;
     extern __shared__ uint4 lds_u4[];
     uint8_t *lds = reinterpret_cast<uint8_t *>(lds_u4);
-    int bidx = blockIdx.x;
+    int bidx = xcd_block(blockIdx.x, gridDim.x);
     const int part = bidx % nsplit;
     bidx /= nsplit;
     const int tv = bidx % tiles_v;

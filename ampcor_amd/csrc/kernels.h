@@ -163,4 +163,14 @@ hipError_t launch_zoom(hipStream_t s, const float *gamma_refined, int pairs, int
                        const double *dirichlet, const int2 *window, const float *coarse_peak,
                        int margin, int rf, float *offsets, float *refined_peak, int2 *zoom_loc);
 
+// the logical index of workgroup b of n: the hardware hands consecutive
+// workgroups to the eight XCDs in turn, and this gives each XCD a contiguous
+// run of logical indices, so the workgroups that read one pair's operands (or
+// neighbouring pairs of the scene) meet in the same L2
+__device__ inline int xcd_block(int b, int n)
+{
+    const int x = b & 7, q = n >> 3, r = n & 7;
+    return x * q + min(x, r) + (b >> 3);
+}
+
 }  // namespace ampcor_hip

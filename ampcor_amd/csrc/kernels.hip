@@ -255,7 +255,7 @@ __global__ __launch_bounds__(1024) void k_correlate(const float *__restrict__ rT
                                                     float *__restrict__ partial)
 {
     extern __shared__ float tile[];
-    int b = blockIdx.x;
+    int b = xcd_block(blockIdx.x, gridDim.x);
     const int part = b % nsplit;
     b /= nsplit;
     const int tv = b % tiles_v;
@@ -1862,15 +1862,6 @@ __global__ void k_up2_finish(int pairs, int N, const double2 *__restrict__ band,
 constexpr int kGatherPrepThreads = 256;
 constexpr size_t kGatherPrepLds = 64 * 1024;
 
-// the pair a workgroup takes: consecutive workgroups go to the eight XCDs in
-// turn, so each XCD is given a contiguous run of pairs -- neighbours in the
-// scene, whose windows overlap, then meet in the same L2
-__device__ inline int xcd_pair(int b, int pairs)
-{
-    const int x = b & 7, q = pairs >> 3, r = pairs & 7;
-    return x * q + min(x, r) + (b >> 3);
-}
-
 // a thread's walk over the column pairs (i, 2 m), (i, 2 m + 1) of an N-row
 // tile with hw column pairs per row, kGatherPrepThreads pairs apart
 struct PairWalk {
@@ -1957,7 +1948,7 @@ __global__ __launch_bounds__(kGatherPrepThreads, 4) void k_gather_prep_mixed(
     Elem *chip_q = static_cast<Elem *>(chip_qv), *tgt_q = static_cast<Elem *>(tgt_qv);
     extern __shared__ float ra[];   // the chip's amplitudes, [C][C]
     __shared__ double red[2][kGatherPrepThreads / kWave];
-    const int pair = xcd_pair(blockIdx.x, pairs);
+    const int pair = xcd_block(blockIdx.x, pairs);
     const int t = threadIdx.x;
     const PairDesc d = desc[pair];
     const int hc = (C + 1) / 2, hwin = (W + 1) / 2;

@@ -89,7 +89,7 @@ __global__ __launch_bounds__(64 * max_waves(NK)) void k_correlate_mfma(
     constexpr int planes = SPLIT ? 2 : 1;
     extern __shared__ uint4 lds_u4[];
     uint16_t *lds = reinterpret_cast<uint16_t *>(lds_u4);
-    int bidx = blockIdx.x;
+    int bidx = xcd_block(blockIdx.x, gridDim.x);
     const int part = bidx % nsplit;
     bidx /= nsplit;
     const int tv = bidx % tiles_v;

@@ -699,11 +699,18 @@ int ampcor_hip_adjust(ampcor_hip_plan *plan, float *offsets)
     const int N = (int)plan->pairs;
     if (int rc = plan_enqueue(plan, N)) return rc;
     const int stages = plan->stages_run;
-    if (offsets)
-        HIP_TRY(hipMemcpyAsync(offsets, plan->offsets.ptr, 2 * (size_t)N * sizeof(float),
-                               hipMemcpyDeviceToHost, s),
+    const size_t bytes = 2 * (size_t)N * sizeof(float);
+    if (offsets) {
+        if (!plan->offsets_host)
+            HIP_TRY(hipHostMalloc((void **)&plan->offsets_host, 2 * plan->pairs * sizeof(float),
+                                  hipHostMallocDefault),
+                    "allocating the pinned offsets buffer");
+        HIP_TRY(hipMemcpyAsync(plan->offsets_host, plan->offsets.ptr, bytes, hipMemcpyDeviceToHost,
+                               s),
                 "copying offsets to the host");
+    }
     HIP_TRY(hipStreamSynchronize(s), "adjust");
+    if (offsets) std::memcpy(offsets, plan->offsets_host, bytes);
     if (plan->timing) {
         for (int k = 0; k < stages; ++k)
             HIP_TRY(hipEventElapsedTime(&plan->stage_ms[k], plan->ev[k], plan->ev[k + 1]),

@@ -99,6 +99,10 @@ struct ampcor_hip_plan {
     ampcor_hip::DeviceBuffer<double2> up_band, up_rows;
     ampcor_hip::UpsampleScratch up_scratch{};
     ampcor_hip::DeviceBuffer<ampcor_hip::MixedStats> mstats, mstats_r;
+    // pinned host staging of the offsets for adjust's host copy (allocated on
+    // its first use): a copy from pageable memory goes through the runtime's
+    // own staging, with a synchronization per chunk
+    float *offsets_host = nullptr;
 
     bool timing = false;
     hipEvent_t ev[ampcor_hip::kStages + 1] = {};
@@ -113,6 +117,8 @@ struct ampcor_hip_plan {
                                      &peak_r, &offsets, &amp_ref_r, &amp_tgt_r, &rT_r,
                                      &partial_r, &gamma_r, &U_ref, &U_tgt, &g_ref, &g_tgt};
         for (auto *b : fb) b->release();
+        if (offsets_host) (void)hipHostFree(offsets_host);
+        offsets_host = nullptr;
         sums.release();
         sums_r.release();
         chip_q.release();

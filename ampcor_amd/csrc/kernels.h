@@ -166,7 +166,8 @@ hipError_t launch_zoom(hipStream_t s, const float *gamma_refined, int pairs, int
 // the logical index of workgroup b of n: the hardware hands consecutive
 // workgroups to the eight XCDs in turn, and this gives each XCD a contiguous
 // run of logical indices, so the workgroups that read one pair's operands (or
-// neighbouring pairs of the scene) meet in the same L2
+// neighbouring pairs of the scene) meet in the same L2.  It is a bijection on
+// [0, n), so results do not depend on where the workgroups actually run.
 __device__ inline int xcd_block(int b, int n)
 {
     const int x = b & 7, q = n >> 3, r = n & 7;

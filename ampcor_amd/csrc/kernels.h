@@ -37,6 +37,7 @@ constexpr int kUpFP8 = 3;
 // tiling of one correlation launch: placements [u0, u0+ub) x [v0, v0+vb) of one
 // pair, chip columns split into `nsplit` ranges, each its own workgroup
 struct CorrelateShape {
+    bool exact;  // one range; FP32 chains per chip column summed in FP64, normalized in place
     int bu;      // placement rows per lane (register block)
     int ub, vb;  // placement tile extent
     int tiles_u, tiles_v;
@@ -58,8 +59,12 @@ hipError_t launch_chip_stats(hipStream_t s, const float *amp, int pairs, int C, 
 // FP64 sums of t and t^2 over every C x C placement of each W x W window: [pairs][P][P]
 hipError_t launch_window_sums(hipStream_t s, const float *amp, int pairs, int W, int C,
                               double2 *sums);
+// exact shapes write gamma (from the window sums and chip statistics, which
+// must be given); split shapes write the FP32 partial numerators
 hipError_t launch_correlate(hipStream_t s, const CorrelateShape &shape, const float *rT,
-                            const float *amp, int pairs, int C, int W, float *partial);
+                            const float *amp, int pairs, int C, int W, float *partial,
+                            const double2 *sums = nullptr, const double2 *stats = nullptr,
+                            float *gamma = nullptr);
 hipError_t launch_normalize(hipStream_t s, const float *partial, int nsplit, const double2 *sums,
                             const double2 *stats, int pairs, int C, int W, float *gamma);
 // window sums and normalization in one pass (W <= 512, P <= 144; tolerance-checked

@@ -18,6 +18,7 @@
 
 #include <math.h>
 
+#include <algorithm>
 #include <type_traits>
 
 namespace ampcor_hip {
@@ -244,15 +245,39 @@ __global__ __launch_bounds__(kWave) void k_window_sums(const float *__restrict__
 // the chip width (one block covers the whole range when it fits).  The
 // accumulation order (j ascending, i ascending) is the same whatever the
 // tiling.
+//
+// EXACT (the FP32 drop-in's coarse stage, one chip-column range): each chip
+// column is its own FP32 fused multiply-add chain of C terms, started at
+// zero, and the column sums are added in FP64 in column order.  A single FP32
+// chain over all C^2 terms loses up to ~1e-5 of the background correlation
+// on 128 x 128 chips, where the target amplitudes sit on a large mean.  The
+// epilogue then normalizes in FP64 (as k_normalize) and writes gamma, so the
+// FP64 numerator never passes through a float buffer.  Otherwise (split
+// chip-column ranges) each lane writes its range's FP32 partial numerator.
 // ---------------------------------------------------------------------------
 constexpr int kJB = 16;  // column block when the whole chip range does not fit
 
-template <int BU>
+// gamma from the FP64 numerator and the FP64 window sums / chip statistics:
+// (num - mu_t sum r') / (|r'| sqrt(sum t^2 - mu_t sum t)); NaN where the
+// window or the chip has no variance
+__device__ inline float ncc_from_sums(double num, double2 ws, double2 st, double n)
+{
+    const double mu = ws.x / n;
+    const double var = ws.y - mu * ws.x;
+    const double A = mu * st.x;
+    const double B = (var > 0.0 && st.y > 0.0) ? st.y * sqrt(var) : 0.0;
+    return (B > 0.0) ? (float)((num - A) / B) : __builtin_nanf("");
+}
+
+template <int BU, bool EXACT>
 __global__ __launch_bounds__(1024) void k_correlate(const float *__restrict__ rT,
                                                     const float *__restrict__ amp, int C, int W,
                                                     int P, int ub_extent, int vb_extent,
                                                     int tiles_u, int tiles_v, int nsplit, int jblock,
-                                                    float *__restrict__ partial)
+                                                    float *__restrict__ partial,
+                                                    const double2 *__restrict__ sums,
+                                                    const double2 *__restrict__ stats,
+                                                    float *__restrict__ gamma)
 {
     extern __shared__ float tile[];
     int b = xcd_block(blockIdx.x, gridDim.x);
@@ -279,8 +304,11 @@ __global__ __launch_bounds__(1024) void k_correlate(const float *__restrict__ rT
     const float *src = amp + (size_t)pair * W * W;
     const float *rp = rT + (size_t)pair * C * C;
     float acc[BU];
+    double total[EXACT ? BU : 1];
 #pragma unroll
     for (int a = 0; a < BU; ++a) acc[a] = 0.0f;
+#pragma unroll
+    for (int a = 0; a < (EXACT ? BU : 1); ++a) total[a] = 0.0;
 
     for (int jb = 0; jb < jlen; jb += jblock) {
         const int jn = min(jblock, jlen - jb);
@@ -309,14 +337,35 @@ __global__ __launch_bounds__(1024) void k_correlate(const float *__restrict__ rT
                     for (int a = 0; a < BU; ++a) acc[a] = fmaf(r, w[(s + a) % BU], acc[a]);
                 }
             }
+            if constexpr (EXACT) {
+#pragma unroll
+                for (int a = 0; a < BU; ++a) {
+                    total[a] = total[a] + (double)acc[a];
+                    acc[a] = 0.0f;
+                }
+            }
         }
     }
     if (!active) return;
-    float *dst = partial + ((size_t)pair * nsplit + part) * P * P;
+    if constexpr (EXACT) {
+        const double2 st = stats[pair];
+        const double n = (double)C * C;
+        const size_t base = (size_t)pair * P * P;
 #pragma unroll
-    for (int a = 0; a < BU; ++a) {
-        const int uu = ubk * BU + a;
-        if (uu < ucount) dst[(size_t)(u0 + uu) * P + v0 + v] = acc[a];
+        for (int a = 0; a < BU; ++a) {
+            const int uu = ubk * BU + a;
+            if (uu < ucount) {
+                const size_t idx = base + (size_t)(u0 + uu) * P + v0 + v;
+                gamma[idx] = ncc_from_sums(total[a], sums[idx], st, n);
+            }
+        }
+    } else {
+        float *dst = partial + ((size_t)pair * nsplit + part) * P * P;
+#pragma unroll
+        for (int a = 0; a < BU; ++a) {
+            const int uu = ubk * BU + a;
+            if (uu < ucount) dst[(size_t)(u0 + uu) * P + v0 + v] = acc[a];
+        }
     }
 }
 
@@ -343,14 +392,7 @@ __global__ __launch_bounds__(256) void k_normalize(const float *__restrict__ par
         for (int k = 0; k < w / 2; ++k) part[k] = part[2 * k] + part[2 * k + 1];
     const float num = part[0];
 
-    const double2 ws = sums[idx];
-    const double n = (double)C * C;
-    const double mu = ws.x / n;
-    const double var = ws.y - mu * ws.x;
-    const double2 st = stats[pair];
-    const double A = mu * st.x;
-    const double B = (var > 0.0 && st.y > 0.0) ? st.y * sqrt(var) : 0.0;
-    gamma[idx] = (B > 0.0) ? (float)(((double)num - A) / B) : __builtin_nanf("");
+    gamma[idx] = ncc_from_sums((double)num, sums[idx], stats[pair], (double)C * C);
 }
 
 // ---------------------------------------------------------------------------
@@ -2227,6 +2269,7 @@ static size_t correlate_lds(int C, int bu, int ucount, int vcount, int jblock)
 CorrelateShape choose_correlate_shape(int C, int P, bool exact, int bu, int nsplit)
 {
     CorrelateShape s{};
+    s.exact = exact;
     if (bu > 0 && C % bu == 0 && (bu == 1 || bu == 2 || bu == 4 || bu == 8 || bu == 16))
         s.bu = bu;
     else if (!exact && C % 4 == 0)
@@ -2308,23 +2351,47 @@ hipError_t launch_window_sums(hipStream_t s, const float *amp, int pairs, int W,
 }
 
 hipError_t launch_correlate(hipStream_t s, const CorrelateShape &sh, const float *rT,
-                            const float *amp, int pairs, int C, int W, float *partial)
+                            const float *amp, int pairs, int C, int W, float *partial,
+                            const double2 *sums, const double2 *stats, float *gamma)
 {
     const int P = W - C + 1;
-    const dim3 grid((unsigned)pairs * sh.tiles_u * sh.tiles_v * sh.nsplit), block(sh.threads);
-#define AMPCOR_CORRELATE(BUV)                                                                    \
-    hipLaunchKernelGGL(k_correlate<BUV>, grid, block, sh.lds_bytes, s, rT, amp, C, W, P, sh.ub,  \
-                       sh.vb, sh.tiles_u, sh.tiles_v, sh.nsplit, sh.jblock, partial)
-    switch (sh.bu) {
-    case 16: AMPCOR_CORRELATE(16); break;
-    case 8: AMPCOR_CORRELATE(8); break;
-    case 4: AMPCOR_CORRELATE(4); break;
-    case 2: AMPCOR_CORRELATE(2); break;
-    case 1: AMPCOR_CORRELATE(1); break;
-    default: return hipErrorInvalidValue;
+    const size_t blocks = (size_t)sh.tiles_u * sh.tiles_v * sh.nsplit;
+    if (sh.exact && (sh.nsplit != 1 || !sums || !stats || !gamma)) return hipErrorInvalidValue;
+    if (!sh.exact && !partial) return hipErrorInvalidValue;
+    // at most 2^31 - 1 workgroups per launch: pairs go in chunks
+    const int chunk = (int)std::min<size_t>((size_t)pairs, (size_t)0x7fffffff / blocks);
+    for (int first = 0; first < pairs; first += chunk) {
+        const int n = std::min(chunk, pairs - first);
+        const dim3 grid((unsigned)(n * blocks)), block(sh.threads);
+        const float *rTf = rT + (size_t)first * C * C;
+        const float *ampf = amp + (size_t)first * W * W;
+        float *pf = sh.exact ? nullptr : partial + (size_t)first * sh.nsplit * P * P;
+        const double2 *sf = sh.exact ? sums + (size_t)first * P * P : nullptr;
+        const double2 *stf = sh.exact ? stats + first : nullptr;
+        float *gf = sh.exact ? gamma + (size_t)first * P * P : nullptr;
+#define AMPCOR_CORRELATE(BUV, EX)                                                                \
+    hipLaunchKernelGGL((k_correlate<BUV, EX>), grid, block, sh.lds_bytes, s, rTf, ampf, C, W, P, \
+                       sh.ub, sh.vb, sh.tiles_u, sh.tiles_v, sh.nsplit, sh.jblock, pf, sf, stf, gf)
+#define AMPCOR_CORRELATE_BU(EX)                      \
+    switch (sh.bu) {                                 \
+    case 16: AMPCOR_CORRELATE(16, EX); break;        \
+    case 8: AMPCOR_CORRELATE(8, EX); break;          \
+    case 4: AMPCOR_CORRELATE(4, EX); break;          \
+    case 2: AMPCOR_CORRELATE(2, EX); break;          \
+    case 1: AMPCOR_CORRELATE(1, EX); break;          \
+    default: return hipErrorInvalidValue;            \
     }
+        if (sh.exact) {
+            AMPCOR_CORRELATE_BU(true)
+        } else {
+            AMPCOR_CORRELATE_BU(false)
+        }
+#undef AMPCOR_CORRELATE_BU
 #undef AMPCOR_CORRELATE
-    return hipGetLastError();
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_normalize(hipStream_t s, const float *partial, int nsplit, const double2 *sums,

@@ -597,12 +597,10 @@ int plan_enqueue(ampcor_hip_plan *plan, int N)
                                         p->mstats.ptr, N, p->C, p->W, p->gamma.ptr),
                  "normalize");
     } else {
+        // the exact-order contraction normalizes in its epilogue
         STEP(launch_correlate(s, p->coarse_shape, p->rT.ptr, p->amp_tgt.ptr, N, p->C, p->W,
-                              p->partial.ptr),
+                              nullptr, p->sums.ptr, p->stats.ptr, p->gamma.ptr),
              "correlate");
-        STEP(launch_normalize(s, p->partial.ptr, p->coarse_shape.nsplit, p->sums.ptr, p->stats.ptr, N,
-                              p->C, p->W, p->gamma.ptr),
-             "normalize");
     }
     STEP(launch_coarse_peak(s, p->gamma.ptr, N, p->P, p->W, p->E, p->rm, p->rf, p->loc.ptr,
                             p->window.ptr, p->peak.ptr, p->snr.ptr),
@@ -851,7 +849,8 @@ int ampcor_hip_plan_autotune(ampcor_hip_plan *plan, int reps)
             CorrelateShape sh = choose_correlate_shape(plan->C, plan->P, true, bu, 0);
             float ms;
             if ((rc = timeit([&] { return launch_correlate(s, sh, plan->rT.ptr, plan->amp_tgt.ptr, N,
-                                                           plan->C, plan->W, plan->partial.ptr); },
+                                                           plan->C, plan->W, nullptr, plan->sums.ptr,
+                                                           plan->stats.ptr, plan->gamma.ptr); },
                              &ms)))
                 break;
             if (ms < best) best = ms, pick = sh;

@@ -282,6 +282,8 @@ def main():
             accuracy[f"{f}_coarse_peak_max_rel_err_vs_fp64"] = rel(p["coarse_peak"][idx],
                                                                    exact["coarse_peak"])
             accuracy[f"{f}_snr_max_rel_err_vs_fp64"] = rel(p["snr"][idx], exact["snr"])
+            accuracy[f"{f}_refined_peak_max_rel_err_vs_fp64"] = rel(p["refined_peak"][idx],
+                                                                    exact["refined_peak"])
         accuracy["fp32_coarse_loc_equal_fp32_twin"] = int(
             np.all(runs["fp32"][2]["coarse_loc"][idx_t] == twin["coarse_loc"], axis=1).sum())
         accuracy["fp32_coarse_peak_bitwise_equal_fp32_twin"] = int(

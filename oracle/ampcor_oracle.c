@@ -420,10 +420,14 @@ static void twin_coarse(const float *amp_ref /* C*C, becomes r' */, const float 
     const double n = (double)C * C;
     for (int u = 0; u < P; ++u)
         for (int v = 0; v < P; ++v) {
-            float acc = 0.0f;
+            /* one FP32 fused-multiply-add chain per chip column (rows in
+             * order, from zero); the column sums added in FP64, in order */
+            double num = 0.0;
             for (int j = 0; j < C; ++j) {
                 const float *rc = rT + (size_t)j * C;
-                for (int i = 0; i < C; ++i) acc = fmaf(rc[i], amp_tgt[(size_t)(u + i) * W + v + j], acc);
+                float col = 0.0f;
+                for (int i = 0; i < C; ++i) col = fmaf(rc[i], amp_tgt[(size_t)(u + i) * W + v + j], col);
+                num = num + (double)col;
             }
             const double s1 = S1[(size_t)u * P + v];
             const double s2 = S2[(size_t)u * P + v];
@@ -431,7 +435,7 @@ static void twin_coarse(const float *amp_ref /* C*C, becomes r' */, const float 
             const double var = s2 - mu * s1;
             const double A = mu * sum_rp;
             const double B = (var > 0.0 && norm > 0.0) ? norm * sqrt(var) : 0.0;
-            gamma[(size_t)u * P + v] = (B > 0.0) ? (float)(((double)acc - A) / B) : NAN;
+            gamma[(size_t)u * P + v] = (B > 0.0) ? (float)((num - A) / B) : NAN;
         }
 }
 

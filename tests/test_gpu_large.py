@@ -1,5 +1,14 @@
 """Large-chip amplitude case (chip 128, margin 64, zoom 16, float32 rasters) on
-a 1024 x 1024 cut of the scene geometry, FP32 and FP16 forms vs the oracles."""
+a 2048 x 2048 cut of the scene geometry (256 pairs), FP32 and FP16 forms vs
+the oracles.
+
+The FP32 bars are the drop-in's: coarse surfaces bitwise equal to the FP32
+twin, offsets within 1e-3 px of the FP64 oracle, coarse peak, refined peak and
+SNR within 1e-5 relative.  The background of a 129 x 129 surface over
+textured amplitudes (a large mean under a small variance) is where a single
+FP32 accumulation chain over all 16384 products loses precision; a cut with
+only a handful of pairs would not see it.
+"""
 
 import numpy as np
 import pytest
@@ -13,9 +22,10 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(scope="module")
 def scene():
-    # the field keeps the scene gradient: 12 px over 8192 lines is 1.5 px over this 1024 cut
-    s = synth.amplitude_scene(seed=14, size=1024, grid=(6, 6), max_shift=1.5)
+    # the field keeps the scene gradient: 12 px over 8192 lines is 3 px over this 2048 cut
+    s = synth.amplitude_scene(seed=14, size=2048, grid=(18, 18), max_shift=3.0)
     print(f"amplitude scene: {s.pairs} pairs", flush=True)
+    assert s.pairs >= 200
     return s
 
 
@@ -33,14 +43,23 @@ def run(lib, s, precision):
                    s.refine_factor, s.refine_margin, s.zoom_factor, precision=precision)
 
 
+def rel_err(x, y):
+    return float(np.max(np.abs(x.astype(np.float64) - y) / np.abs(y)))
+
+
 def test_large_chip_fp32(lib, scene, oracles):
     twin, exact = oracles
     hip = run(lib, scene, "fp32")
     assert np.array_equal(hip["gamma"].view(np.uint32), twin["gamma"].view(np.uint32))
     assert np.array_equal(hip["coarse_loc"], twin["coarse_loc"])
-    np.testing.assert_allclose(hip["offsets"], exact["offsets"], rtol=0, atol=1e-3)
-    np.testing.assert_allclose(hip["coarse_peak"], exact["coarse_peak"], rtol=1e-5)
-    np.testing.assert_allclose(hip["snr"], exact["snr"], rtol=1e-5)
+    assert np.array_equal(hip["coarse_loc"], exact["coarse_loc"])
+    errs = {k: rel_err(hip[k], exact[k]) for k in ("coarse_peak", "refined_peak", "snr")}
+    off = float(np.max(np.abs(hip["offsets"] - exact["offsets"])))
+    print(f"fp32 large chip, {scene.pairs} pairs: max |offset - fp64| = {off:.2e} px, "
+          f"max relative error vs fp64: {errs}", flush=True)
+    assert off <= 1e-3
+    for k, e in errs.items():
+        assert e <= 1e-5, (k, e)
     assert np.max(np.abs(hip["offsets"] - scene.truth)) < 0.1
 
 

@@ -12,7 +12,8 @@
 // raster(buffer, lines, samples[, dtype, device]), slc_pixelSize(),
 // slc_getitem(raster, offset); the CPU module's refine(worker); and the
 // batched and product accessors: addPairs, products, dims, stage_gamma,
-// stage_gamma_refined, stage_locations, set_timing, stage_times.
+// stage_gamma_refined, stage_locations, stage_zoom, zoom_stats, fp8_guard,
+// set_timing, stage_times.
 //
 // Unlike the reference, adjust releases the GIL, and failures raise
 // ampcor_hip.Error instead of escaping as C++ exceptions.
@@ -424,6 +425,61 @@ PyObject *stage_locations(PyObject *, PyObject *args)
     return Py_BuildValue("{sNsNsN}", "coarse", c, "window", wi, "zoomed", z);
 }
 
+// stage_zoom(worker, surfaces): the zoom stage alone on float32 refined
+// surfaces [pairs][R][R]; products() and stage_locations() then read its results
+PyObject *stage_zoom(PyObject *, PyObject *args)
+{
+    PyObject *pyWorker;
+    Py_buffer view;
+    if (!PyArg_ParseTuple(args, "O!y*:stage_zoom", &PyCapsule_Type, &pyWorker, &view)) return nullptr;
+    Worker *w = get_worker(pyWorker);
+    int P, R, Z;
+    int rc = w ? ampcor_hip_plan_dims(w->plan, &P, &R, &Z) : 0;
+    if (!w || rc) {
+        PyBuffer_Release(&view);
+        return w ? raise(rc) : nullptr;
+    }
+    if ((size_t)view.len != 4 * w->pairs * (size_t)R * R) {
+        PyBuffer_Release(&view);
+        PyErr_Format(PyExc_ValueError, "surfaces hold %zd bytes, the plan needs %zu", view.len,
+                     4 * w->pairs * (size_t)R * R);
+        return nullptr;
+    }
+    Py_BEGIN_ALLOW_THREADS
+    rc = ampcor_hip_stage_zoom(w->plan, static_cast<const float *>(view.buf));
+    Py_END_ALLOW_THREADS
+    PyBuffer_Release(&view);
+    if (rc) return raise(rc);
+    Py_RETURN_NONE;
+}
+
+// zoom_stats(worker) -> {"cells", "pairs", "searched"}: the zoom's whole-grid searches
+PyObject *zoom_stats(PyObject *, PyObject *args)
+{
+    PyObject *pyWorker;
+    if (!PyArg_ParseTuple(args, "O!:zoom_stats", &PyCapsule_Type, &pyWorker)) return nullptr;
+    Worker *w = get_worker(pyWorker);
+    if (!w) return nullptr;
+    int64_t cells = 0, pairs = 0, searched = 0;
+    if (int rc = ampcor_hip_zoom_stats(w->plan, &cells, &pairs, &searched)) return raise(rc);
+    return Py_BuildValue("{sLsLsL}", "cells", (long long)cells, "pairs", (long long)pairs,
+                         "searched", (long long)searched);
+}
+
+// fp8_guard(worker) -> {"rescored", "moved", "overflow"}: the FP8 form's peak
+// guard in the last adjust (zeros for the other forms)
+PyObject *fp8_guard(PyObject *, PyObject *args)
+{
+    PyObject *pyWorker;
+    if (!PyArg_ParseTuple(args, "O!:fp8_guard", &PyCapsule_Type, &pyWorker)) return nullptr;
+    Worker *w = get_worker(pyWorker);
+    if (!w) return nullptr;
+    int64_t rescored = 0, moved = 0, overflow = 0;
+    if (int rc = ampcor_hip_fp8_guard_stats(w->plan, &rescored, &moved, &overflow)) return raise(rc);
+    return Py_BuildValue("{sLsLsL}", "rescored", (long long)rescored, "moved", (long long)moved,
+                         "overflow", (long long)overflow);
+}
+
 PyObject *set_timing(PyObject *, PyObject *args)
 {
     PyObject *pyWorker;
@@ -632,6 +688,9 @@ PyMethodDef methods[] = {
     {"stage_gamma", stage_gamma, METH_VARARGS, "coarse correlation surfaces of the last adjust"},
     {"stage_gamma_refined", stage_gamma_refined, METH_VARARGS, "refined correlation surfaces"},
     {"stage_locations", stage_locations, METH_VARARGS, "coarse, window and zoomed peak indices"},
+    {"fp8_guard", fp8_guard, METH_VARARGS, "FP8 peak guard counts of the last adjust"},
+    {"stage_zoom", stage_zoom, METH_VARARGS, "run the zoom stage on given refined surfaces"},
+    {"zoom_stats", zoom_stats, METH_VARARGS, "the zoom screen's exact re-evaluations"},
     {"set_timing", set_timing, METH_VARARGS, "enable per-stage device timing"},
     {"stage_times", stage_times, METH_VARARGS, "per-stage device milliseconds of the last adjust"},
     {"raster", (PyCFunction)(void (*)(void))raster, METH_VARARGS | METH_KEYWORDS,

@@ -77,9 +77,21 @@ bool normalize_fuses(int W, int C);
 hipError_t launch_normalize_fused(hipStream_t s, int form, const void *target, int pairs, int W,
                                   int C, const float *partial, int nsplit, const double2 *stats,
                                   const MixedStats *mstats, float *gamma);
+// near (optional): per pair, the number of cells within tau of the peak
 hipError_t launch_coarse_peak(hipStream_t s, const float *gamma, int pairs, int P, int W, int E,
                               int refine_margin, int exclude, int2 *loc, int2 *window,
-                              float *peak, float *snr);
+                              float *peak, float *snr, float tau = 0.0f, int *near = nullptr);
+// FP8 coarse peak guard: for the pairs whose peak kernel counted more than
+// one cell within peak_guard_margin(C) of the FP8 peak (near), those cells
+// are re-scored by the FP64 correlation of the raster amplitudes, and the best
+// one becomes the peak location (loc, window, peak and snr rewritten for that
+// pair).  counters (3 unsigned, device):
+// pairs re-scored, pairs whose location moved, pairs with more candidates than
+// the guard re-scores (their FP8 peak is kept); cleared by the launch.
+float peak_guard_margin(int C);
+hipError_t launch_peak_guard(hipStream_t s, const PairDesc *desc, int pairs, int dtype, int C,
+                             int W, int E, int rm, int exclude, const float *gamma, const int *near,
+                             int2 *loc, int2 *window, float *peak, float *snr, unsigned *counters);
 // band-limited upsampling of complex tiles straight from the rasters, detected:
 // role 0 = reference chips (N = C, origin = chip corner), role 1 = target
 // windows (N = E, origin = window corner + nudged (top, left)); U is the
@@ -164,9 +176,34 @@ hipError_t launch_correlate_fp8(hipStream_t s, const Fp8Shape &shape, const uint
 
 // whether the zoom of an R x R refined surface by zf fits one workgroup's LDS
 bool zoom_fits(int R, int zf);
+// the zoom's FP16 Dirichlet fragments (halves): for row tile zt < ceil(zf R / 16),
+// K-chunk kc < ceil(R / 32), lane l, value e: D(16 zt + l % 16 - zf b) with
+// b = 32 kc + 8 (l / 16) + e (dfrag), or, for the transposed screen's A
+// operand, b = 32 kc + 4 (l / 16) + e for e < 4 and 32 kc + 16 + 4 (l / 16) +
+// e - 4 otherwise (pfrag); zero past zf R rows or R columns
+int zoom_fragment_count(int R, int zf);
+// per-pair scratch of the zoom's three kernels: [pairs] each
+struct ZoomScratch {
+    double *thr;
+    int32_t *win;  // [pairs][4]
+    int *flag;
+};
+// the caller's bounds on the zoom screen's FP16 error (g = the refined surface):
+// |f_screen - f| <= ecoef max|g| + eabs over the two products, |H_screen - H|
+// <= hcoef max|g| + habs after the first; lambda = max_p sum_a |D(p - zf a)|
+struct ZoomBounds {
+    double ecoef, eabs;
+    float lambda, hcoef, habs;
+};
+// zoom over the whole zf R x zf R grid: the exact FP64 maximum (first in
+// row-major order), certified by an FP16 screen; stats (optional, 3 unsigned):
+// cells evaluated exactly outside the windows, pairs that had any, pairs given
+// the full search
 hipError_t launch_zoom(hipStream_t s, const float *gamma_refined, int pairs, int R, int zf,
-                       const double *dirichlet, const int2 *window, const float *coarse_peak,
-                       int margin, int rf, float *offsets, float *refined_peak, int2 *zoom_loc);
+                       const double *dirichlet, const uint16_t *dfrag, const uint16_t *pfrag,
+                       const ZoomBounds &bounds, const int2 *window, const float *coarse_peak,
+                       int margin, int rf, float *offsets, float *refined_peak, int2 *zoom_loc,
+                       const ZoomScratch &scratch, unsigned *stats = nullptr);
 
 // the logical index of workgroup b of n: the hardware hands consecutive
 // workgroups to the eight XCDs in turn, and this gives each XCD a contiguous

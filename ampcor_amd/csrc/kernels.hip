@@ -824,15 +824,18 @@ __device__ inline void wave_argmax(float &best, int &bidx)
 // the wave's SNR sums reduced, then lane 0 writes the peak, SNR and nudged window
 __device__ inline void coarse_peak_finish(int pair, int lane, int P, int W, int E, int rm, bool none,
                                           int row, int col, float pk, double sum, int count,
-                                          int2 *__restrict__ loc, int2 *__restrict__ window,
-                                          float *__restrict__ peak, float *__restrict__ snr)
+                                          int ties, int2 *__restrict__ loc,
+                                          int2 *__restrict__ window, float *__restrict__ peak,
+                                          float *__restrict__ snr, int *__restrict__ near)
 {
 #pragma unroll
     for (int d = kWave / 2; d > 0; d >>= 1) {
         sum += __shfl_xor(sum, d, kWave);
         count += __shfl_xor(count, d, kWave);
+        ties += __shfl_xor(ties, d, kWave);
     }
     if (lane == 0) {
+        if (near) near[pair] = none ? 0 : ties;
         int left = col - rm;
         if (left < 0) left = 0;
         if (left + E > W) left = W - E;
@@ -854,14 +857,16 @@ __device__ inline void coarse_peak_finish(int pair, int lane, int P, int W, int 
 // KEEP > 0: the surface has at most KEEP cells per lane and stays in registers
 // between the argmax and the SNR sums (one read of it); KEEP = 0 reads it twice.
 // Both visit each lane's cells in the same increasing order, so they give the
-// same bits.
+// same bits.  near (optional): the number of cells within tau of the peak (the
+// peak included), for the FP8 peak guard.
 template <int KEEP>
 __global__ __launch_bounds__(256) void k_coarse_peak(const float *__restrict__ gamma, int pairs,
                                                      int P, int W, int E, int rm, int exclude,
                                                      int2 *__restrict__ loc,
                                                      int2 *__restrict__ window,
                                                      float *__restrict__ peak,
-                                                     float *__restrict__ snr)
+                                                     float *__restrict__ snr, float tau,
+                                                     int *__restrict__ near)
 {
     const int lane = threadIdx.x & (kWave - 1);
     const int pair = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -890,12 +895,14 @@ __global__ __launch_bounds__(256) void k_coarse_peak(const float *__restrict__ g
         const int row = none ? 0 : bidx / P;
         const int col = none ? 0 : bidx - row * P;
         double sum = 0.0;
-        int count = 0;
+        int count = 0, ties = 0;
+        const float floor_ = best - tau;
         // (u, v) of cell lane + 64 j, stepped by 64 cells without divisions
         const int du = kWave / P, dv = kWave - du * P;
         int u = lane / P, v = lane - u * P;
 #pragma unroll
         for (int j = 0; j < KEEP; ++j) {
+            ties += x[j] >= floor_;
             if (x[j] == x[j] && !(abs(u - row) <= exclude && abs(v - col) <= exclude)) {
                 sum += (double)x[j] * (double)x[j];
                 ++count;
@@ -908,7 +915,7 @@ __global__ __launch_bounds__(256) void k_coarse_peak(const float *__restrict__ g
             }
         }
         coarse_peak_finish(pair, lane, P, W, E, rm, none, row, col, none ? __builtin_nanf("") : best,
-                           sum, count, loc, window, peak, snr);
+                           sum, count, ties, loc, window, peak, snr, near);
         return;
     }
     // kPeakBatch loads in flight per lane; each lane visits its cells in
@@ -937,7 +944,8 @@ __global__ __launch_bounds__(256) void k_coarse_peak(const float *__restrict__ g
     const float pk = none ? __builtin_nanf("") : best;
 
     double sum = 0.0;
-    int count = 0;
+    int count = 0, ties = 0;
+    const float floor_ = best - tau;
     for (int k0 = lane; k0 < cells; k0 += kPeakBatch * kWave) {
         float x[kPeakBatch];
 #pragma unroll
@@ -948,14 +956,170 @@ __global__ __launch_bounds__(256) void k_coarse_peak(const float *__restrict__ g
 #pragma unroll
         for (int j = 0; j < kPeakBatch; ++j) {
             const int k = k0 + j * kWave;
+            ties += x[j] >= floor_;
             const int u = k / P, v = k - u * P;
             if (x[j] != x[j] || (abs(u - row) <= exclude && abs(v - col) <= exclude)) continue;
             sum += (double)x[j] * (double)x[j];
             ++count;
         }
     }
-    coarse_peak_finish(pair, lane, P, W, E, rm, none, row, col, pk, sum, count, loc, window, peak,
-                       snr);
+    coarse_peak_finish(pair, lane, P, W, E, rm, none, row, col, pk, sum, count, ties, loc, window,
+                       peak, snr, near);
+}
+
+// ---------------------------------------------------------------------------
+// FP8 peak guard.  The e4m3 operands move each coarse gamma by up to ~0.2 / C
+// (measured on the 128 x 128 amplitude workload: 1.7e-3), so where the top of
+// the surface is flatter than that the FP8 argmax may not be the FP64 one.
+// Every placement whose FP8 gamma is within tau of the FP8 peak is a
+// candidate; with more than one, each is re-scored by the FP64 normalized
+// correlation of the amplitudes, read straight from the rasters, and the best
+// (ties: the first in row-major order) becomes the peak location.  Values stay
+// those of the FP8 surface: the peak is the surface at the chosen cell and the
+// SNR is recomputed around it.  One workgroup per pair.
+// ---------------------------------------------------------------------------
+constexpr int kGuardThreads = 256;
+constexpr int kGuardMax = 16;  // candidates re-scored per pair
+
+__device__ inline double detect64(int dtype, const void *base, size_t offset)
+{
+    if (dtype == 0) {
+        const float2 z = static_cast<const float2 *>(base)[offset];
+        const double re = z.x, im = z.y;
+        return sqrt(re * re + im * im);
+    }
+    return fabs((double)static_cast<const float *>(base)[offset]);
+}
+
+// sum of x over the workgroup (fixed tree; every thread gets the result)
+template <int N>
+__device__ inline void block_sum(double (&x)[N], double (*red)[kGuardThreads])
+{
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < N; ++q) red[q][t] = x[q];
+    __syncthreads();
+    for (int h = kGuardThreads / 2; h > 0; h >>= 1) {
+        if (t < h)
+#pragma unroll
+            for (int q = 0; q < N; ++q) red[q][t] = red[q][t] + red[q][t + h];
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < N; ++q) x[q] = red[q][0];
+    __syncthreads();
+}
+
+__global__ void k_guard_clear(unsigned *counters)
+{
+    if (threadIdx.x < 3) counters[threadIdx.x] = 0u;
+}
+
+__global__ __launch_bounds__(kGuardThreads) void k_peak_guard(
+    const PairDesc *__restrict__ desc, int dtype, int C, int W, int E, int rm, int exclude,
+    float tau, const float *__restrict__ gamma, const int *__restrict__ near,
+    int2 *__restrict__ loc, int2 *__restrict__ window, float *__restrict__ peak,
+    float *__restrict__ snr, unsigned *__restrict__ counters)
+{
+    __shared__ int cand[kGuardMax];
+    __shared__ int ncand;
+    __shared__ double red[5][kGuardThreads];
+    const int pair = blockIdx.x;
+    // the peak kernel counted the cells within tau of the peak
+    if (near[pair] <= 1) return;
+    const int t = threadIdx.x;
+    const int P = W - C + 1, cells = P * P;
+    const float *g = gamma + (size_t)pair * cells;
+    const float pk = peak[pair];
+    if (pk != pk) return;  // no peak (all NaN): nothing to guard
+    if (t == 0) ncand = 0;
+    __syncthreads();
+    const float floor_ = pk - tau;
+    for (int k = t; k < cells; k += kGuardThreads) {
+        const float x = g[k];
+        if (x >= floor_) {
+            const int at = atomicAdd(&ncand, 1);
+            if (at < kGuardMax) cand[at] = k;
+        }
+    }
+    __syncthreads();
+    const int n = ncand;
+    if (n <= 1) return;
+    if (n > kGuardMax) {
+        if (t == 0) atomicAdd(&counters[2], 1u);
+        return;  // keep the FP8 peak
+    }
+    if (t == 0) {
+        // row-major order, so ties keep the first maximum
+        for (int a = 1; a < n; ++a) {
+            const int key = cand[a];
+            int b = a - 1;
+            while (b >= 0 && cand[b] > key) {
+                cand[b + 1] = cand[b];
+                --b;
+            }
+            cand[b + 1] = key;
+        }
+        atomicAdd(&counters[0], 1u);
+    }
+    __syncthreads();
+    const PairDesc d = desc[pair];
+    const double nn = (double)C * C;
+    double best = -INFINITY;
+    int bidx = cand[0];
+    for (int c = 0; c < n; ++c) {
+        const int k = cand[c];
+        const int u = k / P, v = k - u * P;
+        double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // r, r^2, t, t^2, r t
+        for (int e = t; e < C * C; e += kGuardThreads) {
+            const int i = e / C, j = e - i * C;
+            const double r = detect64(dtype, d.ref, (size_t)(d.ref_line + i) * d.ref_stride +
+                                                        d.ref_sample + j);
+            const double x = detect64(dtype, d.tgt, (size_t)(d.tgt_line + u + i) * d.tgt_stride +
+                                                        d.tgt_sample + v + j);
+            acc[0] += r;
+            acc[1] += r * r;
+            acc[2] += x;
+            acc[3] += x * x;
+            acc[4] += r * x;
+        }
+        block_sum(acc, red);
+        const double vr = acc[1] - acc[0] * acc[0] / nn, vt = acc[3] - acc[2] * acc[2] / nn;
+        const double gc = (vr > 0.0 && vt > 0.0) ? (acc[4] - acc[0] * acc[2] / nn) / sqrt(vr * vt)
+                                                 : -INFINITY;
+        if (gc > best) {
+            best = gc;
+            bidx = k;
+        }
+    }
+    const int2 old = loc[pair];
+    const int row = bidx / P, col = bidx - row * P;
+    if (row == old.x && col == old.y) return;
+    // the new peak: nudged window, the surface's value, SNR around it
+    double acc[2] = {0.0, 0.0};
+    for (int k = t; k < cells; k += kGuardThreads) {
+        const float x = g[k];
+        const int u = k / P, v = k - u * P;
+        if (x != x || (abs(u - row) <= exclude && abs(v - col) <= exclude)) continue;
+        acc[0] += (double)x * (double)x;
+        acc[1] += 1.0;
+    }
+    block_sum(acc, red);
+    if (t == 0) {
+        int left = col - rm;
+        if (left < 0) left = 0;
+        if (left + E > W) left = W - E;
+        int top = row - rm;
+        if (top < 0) top = 0;
+        if (top + E > W) top = W - E;
+        const float npk = g[bidx];
+        const double p2 = (double)npk * (double)npk;
+        loc[pair] = make_int2(row, col);
+        window[pair] = make_int2(top, left);
+        peak[pair] = npk;
+        snr[pair] = (acc[1] == 0.0 || acc[0] <= 0.0) ? INFINITY : (float)(p2 / (acc[0] / acc[1]));
+        atomicAdd(&counters[1], 1u);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -2115,11 +2279,28 @@ __global__ __launch_bounds__(kGatherPrepThreads, 4) void k_gather_prep_mixed(
 }
 
 // ---------------------------------------------------------------------------
-// zoom + sub-pixel peak + offsets: band-limited (Dirichlet) interpolation of
-// the refined surface on the (2 zf + 1)^2 zoom cells around its peak, first
-// maximum, three-point parabola per axis; one workgroup per pair
+// zoom + sub-pixel peak + offsets.  The zoom is the band-limited (periodic
+// Dirichlet) interpolation of the refined surface g (R x R, NaN read as 0) on
+// the Z x Z grid, Z = zf R: H[p][b] = sum_a D(p - zf a) g[a][b], f[p][q] =
+// sum_b H[p][b] D(q - zf b).  The offset comes from the first maximum of f
+// over the WHOLE grid (the reference zooms the whole surface and takes its
+// maximum, Sequential.icc:122-124) and a three-point parabola per axis
+// through its neighbours (none at the grid's edge).  One workgroup per pair:
+//
+//   1. exact: f in FP64 on the (2 zf + 1)^2 cells around zf times the
+//      surface's argmax, in the same order as the oracle; f_w = their maximum;
+//   2. screen: f on every cell with two FP16 matrix-core products
+//      (v_mfma_f32_16x16x32_f16, FP32 accumulate; the Dirichlet matrix comes
+//      as precomputed fragments that serve as the A operand of H = D g and the
+//      B operand of f = H D^T), H kept in FP16 in LDS one chunk of rows at a
+//      time.  The host bounds its error: |f_screen - f| <= ecoef max|g| + eabs;
+//   3. every cell outside the window whose screened value reaches f_w minus
+//      that bound is evaluated exactly (same order), so no cell the screen
+//      passes over can beat the exact maximum: the result is the first
+//      maximum of the exact f over the whole grid.
 // ---------------------------------------------------------------------------
 constexpr int kZoomThreads = 256;
+constexpr int kZoomCand = 512;   // candidates listed per pair (more are evaluated in place)
 
 // the first maximum over a workgroup of kZoomThreads: every thread gets it
 __device__ inline void block_argmax(float *rb, int *ri, float &best, int &bidx)
@@ -2134,43 +2315,107 @@ __device__ inline void block_argmax(float *rb, int *ri, float &best, int &bidx)
     best = rb[0];
     bidx = ri[0];
     for (int w = 1; w < kZoomThreads / kWave; ++w) better(best, bidx, rb[w], ri[w]);
+    __syncthreads();
 }
 
-// LDS of k_zoom (bytes); the Dirichlet table (2 zf R + 1 weights) is staged
-// when `stage_d`
-__host__ __device__ inline size_t zoom_lds_bytes(int R, int zf, bool stage_d)
+__device__ inline void better_d(double &best, int &bidx, double x, int k)
 {
-    const size_t span = 2 * zf + 1;
-    return sizeof(double) * (span * R + span * span + (stage_d ? 2 * (size_t)zf * R + 1 : 0)) +
-           sizeof(float) * (size_t)R * R;
+    if (x > best || (x == best && k < bidx)) {
+        best = x;
+        bidx = k;
+    }
 }
 
-template <bool STAGE>
-__global__ __launch_bounds__(kZoomThreads) void k_zoom(
+// the first maximum (value, then lowest index) over the workgroup, FP64
+__device__ inline void block_argmax_d(double *rb, int *ri, double &best, int &bidx)
+{
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) {
+        const double ob = __shfl_xor(best, d, kWave);
+        const int oi = __shfl_xor(bidx, d, kWave);
+        better_d(best, bidx, ob, oi);
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        rb[wave] = best;
+        ri[wave] = bidx;
+    }
+    __syncthreads();
+    best = rb[0];
+    bidx = ri[0];
+    for (int w = 1; w < kZoomThreads / kWave; ++w) better_d(best, bidx, rb[w], ri[w]);
+    __syncthreads();
+}
+
+__device__ inline double parabola_vertex(double fm, double f0, double fp)
+{
+    const double den = fm - 2.0 * f0 + fp;
+    double d = 0.0;
+    if (den < 0.0) d = 0.5 * (fm - fp) / den;
+    return d < -0.5 ? -0.5 : (d > 0.5 ? 0.5 : d);
+}
+
+// LDS layout of k_zoom (byte offsets, 16-byte aligned): the exact window's
+// H [span][R] and f [span][span] (FP64), the Dirichlet table [2 Z + 1] when
+// staged, g [R][R] (FP32), g transposed [BT16][KP + 8] and a chunk of H
+// [rt 16][KP + 8] (FP16); KP = R padded to 32, BT16 = R padded to 16
+struct ZoomLayout {
+    size_t f, dt, g, gt, h16, total;
+};
+__host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
+__host__ __device__ inline ZoomLayout zoom_layout(int R, int zf, bool stage_d, int rt)
+{
+    const size_t span = 2 * zf + 1, Z = (size_t)zf * R;
+    const size_t KP8 = (R + 31) / 32 * 32 + 8, BT16 = (R + 15) / 16 * 16;
+    ZoomLayout L;
+    L.f = align16(sizeof(double) * span * R);
+    L.dt = align16(L.f + sizeof(double) * span * span);
+    L.g = align16(L.dt + (stage_d ? sizeof(double) * (2 * Z + 1) : 0));
+    L.gt = align16(L.g + sizeof(float) * R * R);
+    L.h16 = align16(L.gt + 2 * BT16 * KP8);
+    L.total = L.h16 + 2 * (size_t)rt * 16 * KP8;
+    return L;
+}
+
+// the whole search of one pair (window, screen, exact cells): for the pairs
+// the window kernel or the batched screen flagged (flag[pair] != 0)
+template <bool STAGE, int KC>
+__global__ __launch_bounds__(kZoomThreads) void k_zoom_full(
     const float *__restrict__ gamma_refined, int R, int zf, const double *__restrict__ D,
-    const int2 *__restrict__ window, const float *__restrict__ coarse_peak,
-    int margin, int rf, float *__restrict__ offsets, float *__restrict__ refined_peak,
-    int2 *__restrict__ zoom_loc)
+    const uint4 *__restrict__ dfrag, double ecoef, double eabs, int rt,
+    const int2 *__restrict__ window,
+    const float *__restrict__ coarse_peak, int margin, int rf, float *__restrict__ offsets,
+    float *__restrict__ refined_peak, int2 *__restrict__ zoom_loc, unsigned *__restrict__ stats,
+    const int *__restrict__ flag)
 {
-    extern __shared__ double zs[];
-    const int pair = blockIdx.x;
-    const int t = threadIdx.x;
-    const int Z = zf * R;
-    const int span = 2 * zf + 1;
-    double *H = zs;                   // [span][R]
-    double *f = H + span * R;         // [span][span]
-    double *Dt = f + span * span;     // the Dirichlet table, [2 Z + 1], when staged
-    float *g = reinterpret_cast<float *>(STAGE ? Dt + 2 * Z + 1 : Dt);   // [R][R]
+    if (flag[blockIdx.x] == 0) return;
+    if (stats && threadIdx.x == 0) atomicAdd(&stats[2], 1u);
+    constexpr int NQ = KC <= 2 ? 5 : 2;   // column tiles whose fragments a wave holds
+    extern __shared__ __align__(16) unsigned char zlds[];
     __shared__ float rb[kZoomThreads / kWave];
     __shared__ int ri[kZoomThreads / kWave];
+    __shared__ double rbd[kZoomThreads / kWave];
+    __shared__ int cand[kZoomCand];
+    __shared__ int ncand;
+    const int pair = blockIdx.x;
+    const int t = threadIdx.x, lane = t & (kWave - 1), wave = t >> 6;
+    const int lm = lane & 15, lk = lane >> 4;
+    const int Z = zf * R;
+    const int span = 2 * zf + 1;
+    const int KP8 = KC * 32 + 8, BT = (R + 15) / 16, ZT = (Z + 15) / 16;
+    const ZoomLayout L = zoom_layout(R, zf, STAGE, rt);
+    double *H = reinterpret_cast<double *>(zlds);               // [span][R]
+    double *f = reinterpret_cast<double *>(zlds + L.f);         // [span][span]
+    double *Dt = reinterpret_cast<double *>(zlds + L.dt);       // [2 Z + 1], when staged
+    float *g = reinterpret_cast<float *>(zlds + L.g);           // [R][R]
+    _Float16 *gT = reinterpret_cast<_Float16 *>(zlds + L.gt);   // [BT 16][KP8]
+    _Float16 *H16 = reinterpret_cast<_Float16 *>(zlds + L.h16); // [rt 16][KP8]
 
-    // the Dirichlet table (loads in flight with the surface's)
     if (STAGE)
         for (int k = t; k < 2 * Z + 1; k += kZoomThreads) Dt[k] = D[k];
     const double *Dw = STAGE ? Dt : D;   // weights, LDS or global
-    // the refined surface (NaN read as 0 by the zoom, never a maximum)
     const float *gr = gamma_refined + (size_t)pair * R * R;
-    float best = -INFINITY;
+    float best = -INFINITY, gmax = 0.0f;
     int bidx = 0x7fffffff;
     for (int k = t; k < R * R; k += kZoomThreads) {
         const float x = gr[k];
@@ -2179,12 +2424,36 @@ __global__ __launch_bounds__(kZoomThreads) void k_zoom(
             best = x;
             bidx = k;
         }
+        gmax = fmaxf(gmax, fabsf(g[k]));
+    }
+    // g transposed in FP16 (b-major), zero outside R x R
+    for (int k = t; k < BT * 16 * KC * 32; k += kZoomThreads) {
+        const int b = k / (KC * 32), a = k - b * (KC * 32);
+        float x = (a < R && b < R) ? gr[a * R + b] : 0.0f;
+        gT[b * KP8 + a] = (_Float16)((x != x) ? 0.0f : x);
+    }
+    // H chunk columns past the product's 16-column tiles stay zero
+    for (int k = t; k < rt * 16 * (KC * 32 - BT * 16); k += kZoomThreads) {
+        const int w = KC * 32 - BT * 16, r = k / w, c = BT * 16 + (k - r * w);
+        H16[r * KP8 + c] = (_Float16)0.0f;
+    }
+    if (t == 0) ncand = 0;
+    // max |g| (for the screen's error bound), then the surface's argmax
+    {
+        float m = gmax;
+#pragma unroll
+        for (int d = kWave / 2; d > 0; d >>= 1) m = fmaxf(m, __shfl_xor(m, d, kWave));
+        if (lane == 0) rb[wave] = m;
+        __syncthreads();
+        gmax = fmaxf(fmaxf(rb[0], rb[1]), fmaxf(rb[2], rb[3]));
+        __syncthreads();
     }
     block_argmax(rb, ri, best, bidx);
     const bool none = (bidx == 0x7fffffff);
     const int a = none ? 0 : bidx / R;
     const int bcol = none ? 0 : bidx - a * R;
 
+    // 1. the exact window
     const int p0 = max(zf * a - zf, 0), p1 = min(zf * a + zf, Z - 1);
     const int q0 = max(zf * bcol - zf, 0), q1 = min(zf * bcol + zf, Z - 1);
     const int np = p1 - p0 + 1, nq = q1 - q0 + 1;
@@ -2196,54 +2465,119 @@ __global__ __launch_bounds__(kZoomThreads) void k_zoom(
         H[k] = acc;
     }
     __syncthreads();
+    double fb = -INFINITY;
+    int fi = 0x7fffffff;
     for (int k = t; k < np * nq; k += kZoomThreads) {
         const int ip = k / nq, iq = k - ip * nq;
         const double *w = Dw + (q0 + iq) + Z;
         double acc = 0.0;
         for (int bb = 0; bb < R; ++bb) acc += H[ip * R + bb] * w[-zf * bb];
         f[k] = acc;
+        better_d(fb, fi, acc, (p0 + ip) * Z + q0 + iq);
     }
-    __syncthreads();
-    // the first maximum of the zoomed cells: wave 0
-    if (t >= kWave) return;
-    int bi = 0x7fffffff;
-    double fb = -INFINITY;
-    for (int k = t; k < np * nq; k += kWave)
-        if (f[k] > fb) {
-            fb = f[k];
-            bi = k;
+    block_argmax_d(rbd, ri, fb, fi);   // (also orders the writes of f before the reads below)
+
+    // the exact zoomed value of any cell, in the window's order
+    auto exact = [&](int p, int q) -> double {
+        const double *wp = Dw + p + Z, *wq = Dw + q + Z;
+        double acc = 0.0;
+        for (int bb = 0; bb < R; ++bb) {
+            double h = 0.0;
+            for (int aa = 0; aa < R; ++aa) h += wp[-zf * aa] * (double)g[aa * R + bb];
+            acc += h * wq[-zf * bb];
         }
+        return acc;
+    };
+
+    // 2. the screen over the whole grid, 3. exact values where it reaches
+    const double thr = fb - (ecoef * (double)gmax + eabs);
+    double cb = -INFINITY;
+    int ci = 0x7fffffff;
+    if (!none) {
+        for (int pt0 = 0; pt0 < ZT; pt0 += rt) {
+            const int nrt = min(rt, ZT - pt0);
+            // H = D g for this chunk's rows
+            for (int item = wave; item < nrt * BT; item += kZoomThreads / kWave) {
+                const int i = item / BT, bt = item - i * BT;
+                floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int d = kWave / 2; d > 0; d >>= 1) {
-        const double ob = __shfl_xor(fb, d, kWave);
-        const int oi = __shfl_xor(bi, d, kWave);
-        if (ob > fb || (ob == fb && oi < bi)) {
-            fb = ob;
-            bi = oi;
+                for (int kc = 0; kc < KC; ++kc)
+                    acc = mfma_f16(dfrag[((pt0 + i) * KC + kc) * kWave + lane],
+                                   *reinterpret_cast<const uint4 *>(gT + (16 * bt + lm) * KP8 +
+                                                                    32 * kc + 8 * lk),
+                                   acc);
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    H16[(16 * i + 4 * lk + r) * KP8 + 16 * bt + lm] = (_Float16)acc[r];
+            }
+            __syncthreads();
+            // f = H D^T: a wave holds the fragments of NQ column tiles per pass
+            for (int qg = wave; qg < ZT; qg += NQ * (kZoomThreads / kWave)) {
+                uint4 bq[NQ][KC];
+#pragma unroll
+                for (int j = 0; j < NQ; ++j)
+#pragma unroll
+                    for (int kc = 0; kc < KC; ++kc) {
+                        const int qt = qg + j * (kZoomThreads / kWave);
+                        bq[j][kc] = qt < ZT ? dfrag[(qt * KC + kc) * kWave + lane]
+                                            : make_uint4(0, 0, 0, 0);
+                    }
+                for (int i = 0; i < nrt; ++i) {
+                    uint4 af[KC];
+#pragma unroll
+                    for (int kc = 0; kc < KC; ++kc)
+                        af[kc] = *reinterpret_cast<const uint4 *>(H16 + (16 * i + lm) * KP8 +
+                                                                  32 * kc + 8 * lk);
+#pragma unroll
+                    for (int j = 0; j < NQ; ++j) {
+                        const int qt = qg + j * (kZoomThreads / kWave);
+                        if (qt >= ZT) break;
+                        floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                        for (int kc = 0; kc < KC; ++kc) acc = mfma_f16(af[kc], bq[j][kc], acc);
+                        const int q = 16 * qt + lm;
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int p = 16 * (pt0 + i) + 4 * lk + r;
+                            if (p >= Z || q >= Z || (double)acc[r] < thr) continue;
+                            if (p >= p0 && p <= p1 && q >= q0 && q <= q1) continue;
+                            const int slot = atomicAdd(&ncand, 1);
+                            if (slot < kZoomCand) cand[slot] = p * Z + q;
+                            else better_d(cb, ci, exact(p, q), p * Z + q);
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        const int n = min(ncand, kZoomCand);
+        for (int k = t; k < n; k += kZoomThreads) {
+            const int idx = cand[k];
+            better_d(cb, ci, exact(idx / Z, idx - (idx / Z) * Z), idx);
+        }
+        if (stats && t == 0 && ncand > 0) {
+            atomicAdd(&stats[0], (unsigned)ncand);
+            atomicAdd(&stats[1], 1u);
         }
     }
+    block_argmax_d(rbd, ri, cb, ci);
+    better_d(fb, fi, cb, ci);
     if (t != 0) return;
-    if (bi == 0x7fffffff) bi = 0;   // every zoomed cell NaN: the serial scan's pick
-    const int ip = bi / nq, iq = bi - ip * nq;
-    double dp = 0.0, dq = 0.0;
-    if (ip > 0 && ip < np - 1) {
-        const double fm = f[(ip - 1) * nq + iq], f0 = f[bi], fp = f[(ip + 1) * nq + iq];
-        const double den = fm - 2.0 * f0 + fp;
-        if (den < 0.0) dp = 0.5 * (fm - fp) / den;
-    }
-    if (iq > 0 && iq < nq - 1) {
-        const double fm = f[ip * nq + iq - 1], f0 = f[bi], fp = f[ip * nq + iq + 1];
-        const double den = fm - 2.0 * f0 + fp;
-        if (den < 0.0) dq = 0.5 * (fm - fp) / den;
-    }
-    dp = dp < -0.5 ? -0.5 : (dp > 0.5 ? 0.5 : dp);
-    dq = dq < -0.5 ? -0.5 : (dq > 0.5 ? 0.5 : dq);
-    const int pp = p0 + ip, pq = q0 + iq;
+    if (fi == 0x7fffffff) fi = 0;
+    const int pp = fi / Z, pq = fi - pp * Z;
+    auto value = [&](int p, int q) -> double {
+        if (p >= p0 && p <= p1 && q >= q0 && q <= q1) return f[(p - p0) * nq + (q - q0)];
+        return exact(p, q);
+    };
+    const double dp = (pp > 0 && pp < Z - 1)
+                          ? parabola_vertex(value(pp - 1, pq), fb, value(pp + 1, pq)) : 0.0;
+    const double dq = (pq > 0 && pq < Z - 1)
+                          ? parabola_vertex(value(pp, pq - 1), fb, value(pp, pq + 1)) : 0.0;
     const int2 tl = window[pair];
     const double zoom = (double)rf * zf;
     float dl = (float)((double)(tl.x - margin) + ((double)pp + dp) / zoom);
     float ds = (float)((double)(tl.y - margin) + ((double)pq + dq) / zoom);
-    float fpk = (float)f[bi];
+    float fpk = (float)fb;
     const float cp = coarse_peak[pair];
     if (none || cp != cp) {
         dl = ds = fpk = __builtin_nanf("");
@@ -2252,6 +2586,436 @@ __global__ __launch_bounds__(kZoomThreads) void k_zoom(
     offsets[2 * pair + 1] = ds;
     refined_peak[pair] = fpk;
     zoom_loc[pair] = make_int2(pp, pq);
+}
+
+// per-pair state the zoom's kernels hand on: the threshold the screen must
+// reach (exact window maximum minus the screen's error bound), the window
+// (p0, p1, q0, q1) and whether the pair needs the full search
+struct ZoomHandoff {
+    double *thr;
+    int4 *win;
+    int *flag;
+};
+
+// the exact window alone, with provisional results; flags the pair for the
+// full search when its maximum lies on a window edge inside the grid (a
+// parabola neighbour is then outside the window) or when `force`
+template <bool STAGE>
+__global__ __launch_bounds__(kZoomThreads) void k_zoom_window(
+    const float *__restrict__ gamma_refined, int R, int zf, const double *__restrict__ D,
+    double ecoef, double eabs, int force, const int2 *__restrict__ window,
+    const float *__restrict__ coarse_peak, int margin, int rf, float *__restrict__ offsets,
+    float *__restrict__ refined_peak, int2 *__restrict__ zoom_loc, ZoomHandoff ho)
+{
+    extern __shared__ __align__(16) unsigned char zlds[];
+    __shared__ float rb[kZoomThreads / kWave];
+    __shared__ int ri[kZoomThreads / kWave];
+    __shared__ double rbd[kZoomThreads / kWave];
+    const int pair = blockIdx.x;
+    const int t = threadIdx.x, lane = t & (kWave - 1), wave = t >> 6;
+    const int Z = zf * R;
+    const ZoomLayout L = zoom_layout(R, zf, STAGE, 0);
+    double *H = reinterpret_cast<double *>(zlds);          // [span][R]
+    double *f = reinterpret_cast<double *>(zlds + L.f);    // [span][span]
+    double *Dt = reinterpret_cast<double *>(zlds + L.dt);  // [2 Z + 1], when staged
+    float *g = reinterpret_cast<float *>(zlds + L.g);      // [R][R]
+
+    if (STAGE)
+        for (int k = t; k < 2 * Z + 1; k += kZoomThreads) Dt[k] = D[k];
+    const double *Dw = STAGE ? Dt : D;
+    const float *gr = gamma_refined + (size_t)pair * R * R;
+    float best = -INFINITY, gmax = 0.0f;
+    int bidx = 0x7fffffff;
+    for (int k = t; k < R * R; k += kZoomThreads) {
+        const float x = gr[k];
+        g[k] = (x != x) ? 0.0f : x;
+        if (x > best) {
+            best = x;
+            bidx = k;
+        }
+        gmax = fmaxf(gmax, fabsf(g[k]));
+    }
+    {
+        float m = gmax;
+#pragma unroll
+        for (int d = kWave / 2; d > 0; d >>= 1) m = fmaxf(m, __shfl_xor(m, d, kWave));
+        if (lane == 0) rb[wave] = m;
+        __syncthreads();
+        gmax = fmaxf(fmaxf(rb[0], rb[1]), fmaxf(rb[2], rb[3]));
+        __syncthreads();
+    }
+    block_argmax(rb, ri, best, bidx);
+    const bool none = (bidx == 0x7fffffff);
+    const int a = none ? 0 : bidx / R;
+    const int bcol = none ? 0 : bidx - a * R;
+    const int p0 = max(zf * a - zf, 0), p1 = min(zf * a + zf, Z - 1);
+    const int q0 = max(zf * bcol - zf, 0), q1 = min(zf * bcol + zf, Z - 1);
+    const int np = p1 - p0 + 1, nq = q1 - q0 + 1;
+    for (int k = t; k < np * R; k += kZoomThreads) {
+        const int ip = k / R, bb = k - ip * R;
+        const double *w = Dw + (p0 + ip) + Z;   // weight of surface row aa: w[-zf aa]
+        double acc = 0.0;
+        for (int aa = 0; aa < R; ++aa) acc += w[-zf * aa] * (double)g[aa * R + bb];
+        H[k] = acc;
+    }
+    __syncthreads();
+    double fb = -INFINITY;
+    int fi = 0x7fffffff;
+    for (int k = t; k < np * nq; k += kZoomThreads) {
+        const int ip = k / nq, iq = k - ip * nq;
+        const double *w = Dw + (q0 + iq) + Z;
+        double acc = 0.0;
+        for (int bb = 0; bb < R; ++bb) acc += H[ip * R + bb] * w[-zf * bb];
+        f[k] = acc;
+        better_d(fb, fi, acc, (p0 + ip) * Z + q0 + iq);
+    }
+    block_argmax_d(rbd, ri, fb, fi);
+    if (t != 0) return;
+    if (fi == 0x7fffffff) fi = 0;
+    const int pp = fi / Z, pq = fi - pp * Z;
+    const int ip = pp - p0, iq = pq - q0;
+    const bool edge = (pp == p0 && p0 > 0) || (pp == p1 && p1 < Z - 1) ||
+                      (pq == q0 && q0 > 0) || (pq == q1 && q1 < Z - 1);
+    const double dp = (ip > 0 && ip < np - 1)
+                          ? parabola_vertex(f[(ip - 1) * nq + iq], fb, f[(ip + 1) * nq + iq]) : 0.0;
+    const double dq = (iq > 0 && iq < nq - 1)
+                          ? parabola_vertex(f[ip * nq + iq - 1], fb, f[ip * nq + iq + 1]) : 0.0;
+    const int2 tl = window[pair];
+    const double zoom = (double)rf * zf;
+    float dl = (float)((double)(tl.x - margin) + ((double)pp + dp) / zoom);
+    float ds = (float)((double)(tl.y - margin) + ((double)pq + dq) / zoom);
+    float fpk = (float)fb;
+    const float cp = coarse_peak[pair];
+    if (none || cp != cp) {
+        dl = ds = fpk = __builtin_nanf("");
+    }
+    offsets[2 * pair] = dl;
+    offsets[2 * pair + 1] = ds;
+    refined_peak[pair] = fpk;
+    zoom_loc[pair] = make_int2(pp, pq);
+    ho.thr[pair] = none ? __builtin_nan("") : fb - (ecoef * (double)gmax + eabs);
+    ho.win[pair] = make_int4(p0, p1, q0, q1);
+    ho.flag[pair] = !none && (force || edge);
+}
+
+// the FP16 screen of the whole zoomed grid, kZoomScreenPairs pairs per
+// workgroup: a wave owns the 16-row (and 16-column) tiles w, w + 4, ... and
+// keeps their Dirichlet fragments in registers for every pair, since they are
+// the A operand of H = D g for its row tiles and the B operand of f = H D^T
+// for its column tiles; H goes through LDS in FP16.  A pair any of whose
+// cells outside the window reaches the threshold is flagged for the full search.
+constexpr int kZoomScreenPairs = 8;
+
+__host__ __device__ inline size_t zoom_screen_lds(int R, int zf)
+{
+    const size_t KP8 = (R + 31) / 32 * 32 + 8;
+    return 2 * KP8 * ((size_t)(R + 15) / 16 * 16 + (size_t)(zf * R + 15) / 16 * 16);
+}
+
+constexpr int kZoomScreenLoads = 5;   // surface values per thread: R^2 <= 5 x 256
+
+template <int KC, int NT>
+__global__ __launch_bounds__(kZoomThreads, 1) void k_zoom_screen(
+    const float *__restrict__ gamma_refined, int pairs, int R, int zf,
+    const uint4 *__restrict__ dfrag, ZoomHandoff ho)
+{
+    extern __shared__ __align__(16) unsigned char zlds[];
+    __shared__ int hit;
+    __shared__ double sthr[kZoomScreenPairs];
+    __shared__ int4 swin[kZoomScreenPairs];
+    __shared__ int sgo[kZoomScreenPairs];
+    const int t = threadIdx.x, lane = t & (kWave - 1);
+    // wave-uniform, so the tile tests below are scalar branches
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int lm = lane & 15, lk = lane >> 4;
+    const int Z = zf * R, ZT = (Z + 15) / 16, BT = (R + 15) / 16;
+    const int KP8 = KC * 32 + 8;
+    const int first = blockIdx.x * kZoomScreenPairs;
+    _Float16 *gT = reinterpret_cast<_Float16 *>(zlds);    // [BT 16][KP8]
+    _Float16 *H16 = gT + BT * 16 * KP8;                    // [ZT 16][KP8]
+    if (t < kZoomScreenPairs) {
+        const int pair = first + t;
+        const bool in = pair < pairs;
+        const double thr = in ? ho.thr[pair] : 0.0;
+        sgo[t] = in && ho.flag[pair] == 0 && thr == thr;
+        sthr[t] = thr;
+        swin[t] = in ? ho.win[pair] : make_int4(0, 0, 0, 0);
+    }
+    // zero: g's padding (never written below) and H's columns past the tiles
+    for (int k = t; k < BT * 16 * KP8; k += kZoomThreads) gT[k] = (_Float16)0.0f;
+    for (int k = t; k < ZT * 16 * (KC * 32 - BT * 16); k += kZoomThreads) {
+        const int w = KC * 32 - BT * 16, r = k / w, c = BT * 16 + (k - r * w);
+        H16[r * KP8 + c] = (_Float16)0.0f;
+    }
+    uint4 fr[NT][KC];
+#pragma unroll
+    for (int j = 0; j < NT; ++j)
+#pragma unroll
+        for (int kc = 0; kc < KC; ++kc) {
+            const int tile = wave + 4 * j;
+            fr[j][kc] = tile < ZT ? dfrag[(tile * KC + kc) * kWave + lane] : make_uint4(0, 0, 0, 0);
+        }
+    __syncthreads();
+    // the surfaces go through registers, the next pair's loading during this one's products
+    float x[kZoomScreenLoads];
+    auto fetch = [&](int k) {
+        const float *gr = gamma_refined + (size_t)(first + k) * R * R;
+#pragma unroll
+        for (int j = 0; j < kZoomScreenLoads; ++j) {
+            const int i = t + j * kZoomThreads;
+            x[j] = i < R * R ? gr[i] : 0.0f;
+        }
+    };
+    int k = 0;
+    while (k < kZoomScreenPairs && !sgo[k]) ++k;
+    if (k < kZoomScreenPairs) fetch(k);
+    while (k < kZoomScreenPairs) {
+        const double thr = sthr[k];
+        const int4 w = swin[k];
+        const int pair = first + k;
+#pragma unroll
+        for (int j = 0; j < kZoomScreenLoads; ++j) {
+            const int i = t + j * kZoomThreads;
+            if (i < R * R) {
+                const int a = i / R, b = i - a * R;
+                gT[b * KP8 + a] = (_Float16)((x[j] != x[j]) ? 0.0f : x[j]);
+            }
+        }
+        int next = k + 1;
+        while (next < kZoomScreenPairs && !sgo[next]) ++next;
+        if (t == 0) hit = 0;
+        __syncthreads();
+        // H = D g, this wave's row tiles
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+            const int pt = wave + 4 * j;
+            if (pt < ZT)
+                for (int bt = 0; bt < BT; ++bt) {
+                    floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                    for (int kc = 0; kc < KC; ++kc)
+                        acc = mfma_f16(fr[j][kc],
+                                       *reinterpret_cast<const uint4 *>(gT + (16 * bt + lm) * KP8 +
+                                                                        32 * kc + 8 * lk),
+                                       acc);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        H16[(16 * pt + 4 * lk + r) * KP8 + 16 * bt + lm] = (_Float16)acc[r];
+                }
+        }
+        __syncthreads();
+        // f = H D^T: every row tile, this wave's column tiles.  The threshold is
+        // rounded down to FP32 (a superset of the cells it passes); tiles inside
+        // the grid and clear of the window only raise a running maximum, the
+        // others are checked cell by cell
+        const float thr_f = (float)(thr - fabs(thr) * 0x1p-22 - 1e-30);
+        float mx = -INFINITY;
+        bool any = false;
+        for (int pt = 0; pt < ZT; ++pt) {
+            uint4 af[KC];
+#pragma unroll
+            for (int kc = 0; kc < KC; ++kc)
+                af[kc] = *reinterpret_cast<const uint4 *>(H16 + (16 * pt + lm) * KP8 + 32 * kc +
+                                                          8 * lk);
+            // every product of this row tile first (tiles past the grid have zero
+            // fragments), then their results
+            floatx4 acc[NT];
+#pragma unroll
+            for (int j = 0; j < NT; ++j) {
+                acc[j] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int kc = 0; kc < KC; ++kc) acc[j] = mfma_f16(af[kc], fr[j][kc], acc[j]);
+            }
+            const bool row_edge = 16 * pt + 15 >= Z;
+            const bool row_win = 16 * pt <= w.y && 16 * pt + 15 >= w.x;
+#pragma unroll
+            for (int j = 0; j < NT; ++j) {
+                const int qt = wave + 4 * j;
+                if (qt >= ZT) continue;
+                const bool slow = row_edge || 16 * qt + 15 >= Z ||
+                                  (row_win && 16 * qt <= w.w && 16 * qt + 15 >= w.z);
+                if (!slow) {
+                    mx = fmaxf(mx, fmaxf(fmaxf(acc[j][0], acc[j][1]), fmaxf(acc[j][2], acc[j][3])));
+                    continue;
+                }
+                const int q = 16 * qt + lm;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int p = 16 * pt + 4 * lk + r;
+                    const bool inside = p >= w.x && p <= w.y && q >= w.z && q <= w.w;
+                    any |= p < Z && q < Z && !inside && acc[j][r] >= thr_f;
+                }
+            }
+        }
+        // the next pair's surface loads while the block settles this one
+        if (next < kZoomScreenPairs) fetch(next);
+        if (any || mx >= thr_f) hit = 1;
+        __syncthreads();
+        if (t == 0 && hit) ho.flag[pair] = 1;
+        k = next;
+    }
+}
+
+// The screen with one wave per pair: f^T = D_q (H^T), H^T = g^T D_p^T.  The
+// output of the first product lands in the lanes as the B operand of the
+// second when the second product's K slots of lane group lk hold
+// b = 32 kc + 4 lk + (0..3) and 32 kc + 16 + 4 lk + (0..3); its A operand
+// (D_q) comes as fragments in that order (pfrag).  Both Dirichlet fragment
+// tables sit in LDS, shared by the workgroup's waves; a wave builds its pair's
+// g^T fragments from global memory and needs no barrier.  Rows whose bound
+// (below) stays under the threshold skip the second product.
+constexpr int kScreenWaves = 8;
+constexpr int kScreenMaxTiles = 40;   // row tiles: Z <= 640
+
+__host__ __device__ inline size_t zoom_screen1_lds(int R, int zf)
+{
+    const size_t KC = (R + 31) / 32, ZT = ((size_t)zf * R + 15) / 16;
+    return 2 * ZT * KC * kWave * 16;
+}
+
+template <int KC>
+__global__ __launch_bounds__(kScreenWaves * kWave) void k_zoom_screen1(
+    const float *__restrict__ gamma_refined, int pairs, int R, int zf,
+    const uint4 *__restrict__ dfrag, const uint4 *__restrict__ pfrag, float lambda, float hcoef,
+    float habs, ZoomHandoff ho)
+{
+    extern __shared__ __align__(16) unsigned char zlds[];
+    const int t = threadIdx.x, lane = t & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int lm = lane & 15, lk = lane >> 4;
+    const int Z = zf * R, ZT = (Z + 15) / 16;
+    uint4 *Dp = reinterpret_cast<uint4 *>(zlds);   // [ZT][KC][64]
+    uint4 *Dq = Dp + ZT * KC * kWave;               // [ZT][KC][64]
+    for (int k = t; k < ZT * KC * kWave; k += kScreenWaves * kWave) {
+        Dp[k] = dfrag[k];
+        Dq[k] = pfrag[k];
+    }
+    __syncthreads();
+    const int waves = gridDim.x * kScreenWaves;
+    for (int pair = blockIdx.x * kScreenWaves + wave; pair < pairs; pair += waves) {
+        if (ho.flag[pair] != 0) continue;
+        const double thr = ho.thr[pair];
+        if (thr != thr) continue;
+        const int4 w = ho.win[pair];
+        const float thr_f = (float)(thr - fabs(thr) * 0x1p-22 - 1e-30);
+        // column tiles: past the grid (or its last, partial one), or meeting the window
+        const uint64_t valid_cols = ZT >= 64 ? ~0ull : (1ull << ZT) - 1;
+        uint64_t edge_cols = ~valid_cols, win_cols = 0;
+        if (Z % 16) edge_cols |= 1ull << (ZT - 1);
+        for (int j = w.z / 16; j <= w.w / 16; ++j) win_cols |= 1ull << j;
+        // the first product's A operand: g^T rows b = 16 bt + lm, K = a
+        const float *gr = gamma_refined + (size_t)pair * R * R;
+        float gv[2 * KC][KC][8];
+#pragma unroll
+        for (int bt = 0; bt < 2 * KC; ++bt)
+#pragma unroll
+            for (int kc = 0; kc < KC; ++kc)
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int a = 32 * kc + 8 * lk + e, b = 16 * bt + lm;
+                    gv[bt][kc][e] = (a < R && b < R) ? gr[a * R + b] : 0.0f;
+                }
+        float gmax = 0.0f;
+        uint4 ga[2 * KC][KC];
+#pragma unroll
+        for (int bt = 0; bt < 2 * KC; ++bt)
+#pragma unroll
+            for (int kc = 0; kc < KC; ++kc) {
+                halfx8_t v;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const float x = gv[bt][kc][e] != gv[bt][kc][e] ? 0.0f : gv[bt][kc][e];
+                    gmax = fmaxf(gmax, fabsf(x));
+                    v[e] = (_Float16)x;
+                }
+                ga[bt][kc] = __builtin_bit_cast(uint4, v);
+            }
+#pragma unroll
+        for (int d = kWave / 2; d > 0; d >>= 1) gmax = fmaxf(gmax, __shfl_xor(gmax, d, kWave));
+        // bound on |H - H_screen| (the first product's error)
+        const float eh = hcoef * gmax + habs;
+        float mx = -INFINITY;
+        bool any = false;
+        for (int pt = 0; pt < ZT; ++pt) {
+            // H^T tiles (b rows, p columns) for this row tile of the grid
+            floatx4 h[2 * KC];
+#pragma unroll
+            for (int bt = 0; bt < 2 * KC; ++bt) {
+                h[bt] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int kc = 0; kc < KC; ++kc)
+                    h[bt] = mfma_f16(ga[bt][kc], Dp[(pt * KC + kc) * kWave + lane], h[bt]);
+            }
+            // row bound: for row p, f(p, q) - c = sum_b (H(p, b) - c) D(q - zf b)
+            // (a row's weights sum to 1), so f <= c + lambda r with c, r the
+            // midrange and half-range of H(p, .); with H's error eh added, rows
+            // whose bound stays under the threshold need no second product
+            float hmax = -INFINITY, hmin = INFINITY;
+#pragma unroll
+            for (int bt = 0; bt < 2 * KC; ++bt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    if (16 * bt + 4 * lk + r < R) {
+                        hmax = fmaxf(hmax, h[bt][r]);
+                        hmin = fminf(hmin, h[bt][r]);
+                    }
+            hmax = fmaxf(hmax, __shfl_xor(hmax, 16, kWave));
+            hmin = fminf(hmin, __shfl_xor(hmin, 16, kWave));
+            hmax = fmaxf(hmax, __shfl_xor(hmax, 32, kWave));
+            hmin = fminf(hmin, __shfl_xor(hmin, 32, kWave));
+            const float c = 0.5f * (hmax + hmin), rad = 0.5f * (hmax - hmin);
+            // (rounded up past the few FP32 roundings that computed it)
+            const float b0 = c + eh + lambda * (rad + eh);
+            const float bound = b0 + fabsf(b0) * 0x1p-20f + 0x1p-60f;
+            if (!__any(16 * pt + lm < Z && bound >= thr_f)) continue;
+            // as the second product's B operand, in the K order of pfrag
+            uint4 hb[KC];
+#pragma unroll
+            for (int kc = 0; kc < KC; ++kc) {
+                const halfx8_t v = {(_Float16)h[2 * kc][0], (_Float16)h[2 * kc][1],
+                                    (_Float16)h[2 * kc][2], (_Float16)h[2 * kc][3],
+                                    (_Float16)h[2 * kc + 1][0], (_Float16)h[2 * kc + 1][1],
+                                    (_Float16)h[2 * kc + 1][2], (_Float16)h[2 * kc + 1][3]};
+                hb[kc] = __builtin_bit_cast(uint4, v);
+            }
+            const bool row_edge = 16 * pt + 15 >= Z;
+            const bool row_win = 16 * pt <= w.y && 16 * pt + 15 >= w.x;
+            const uint64_t slow = row_edge ? ~0ull : (edge_cols | (row_win ? win_cols : 0ull));
+            // column tiles in groups of 8: their products first, then their results
+            for (int j0 = 0; j0 < ZT; j0 += 8) {
+                floatx4 acc[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    acc[u] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+                    const int j = min(j0 + u, ZT - 1);
+#pragma unroll
+                    for (int kc = 0; kc < KC; ++kc)
+                        acc[u] = mfma_f16(Dq[(j * KC + kc) * kWave + lane], hb[kc], acc[u]);
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int j = j0 + u;
+                    if (j >= ZT) break;
+                    // lane: f[16 pt + lm][16 j + 4 lk + r]
+                    const float m4 =
+                        fmaxf(fmaxf(acc[u][0], acc[u][1]), fmaxf(acc[u][2], acc[u][3]));
+                    if (!((slow >> j) & 1)) {
+                        mx = fmaxf(mx, m4);
+                        continue;
+                    }
+                    const int p = 16 * pt + lm;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int q = 16 * j + 4 * lk + r;
+                        const bool inside = p >= w.x && p <= w.y && q >= w.z && q <= w.w;
+                        any |= p < Z && q < Z && !inside && acc[u][r] >= thr_f;
+                    }
+                }
+            }
+        }
+        any |= mx >= thr_f;
+        if (__any(any) && lane == 0) ho.flag[pair] = 1;
+    }
 }
 
 }  // namespace
@@ -2459,15 +3223,31 @@ hipError_t launch_normalize_fused(hipStream_t s, int form, const void *target, i
 
 hipError_t launch_coarse_peak(hipStream_t s, const float *gamma, int pairs, int P, int W, int E,
                               int refine_margin, int exclude, int2 *loc, int2 *window,
-                              float *peak, float *snr)
+                              float *peak, float *snr, float tau, int *near)
 {
     constexpr int kKeep = 72;   // surfaces up to 67 x 67 stay in registers
     if (P * P <= kKeep * kWave)
         hipLaunchKernelGGL(k_coarse_peak<kKeep>, dim3((pairs + 3) / 4), dim3(256), 0, s, gamma,
-                           pairs, P, W, E, refine_margin, exclude, loc, window, peak, snr);
+                           pairs, P, W, E, refine_margin, exclude, loc, window, peak, snr, tau,
+                           near);
     else
         hipLaunchKernelGGL(k_coarse_peak<0>, dim3((pairs + 3) / 4), dim3(256), 0, s, gamma, pairs,
-                           P, W, E, refine_margin, exclude, loc, window, peak, snr);
+                           P, W, E, refine_margin, exclude, loc, window, peak, snr, tau, near);
+    return hipGetLastError();
+}
+
+float peak_guard_margin(int C) { return 0.75f / (float)C; }
+
+hipError_t launch_peak_guard(hipStream_t s, const PairDesc *desc, int pairs, int dtype, int C,
+                             int W, int E, int rm, int exclude, const float *gamma, const int *near,
+                             int2 *loc, int2 *window, float *peak, float *snr, unsigned *counters)
+{
+    hipLaunchKernelGGL(k_guard_clear, dim3(1), dim3(64), 0, s, counters);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || pairs == 0) return e;
+    hipLaunchKernelGGL(k_peak_guard, dim3(pairs), dim3(kGuardThreads), 0, s, desc, dtype, C, W, E,
+                       rm, exclude, peak_guard_margin(C), gamma, near, loc, window, peak, snr,
+                       counters);
     return hipGetLastError();
 }
 
@@ -2607,27 +3387,134 @@ hipError_t launch_gather_prep_mixed(hipStream_t s, const PairDesc *desc, int pai
     return hipGetLastError();
 }
 
-constexpr size_t kZoomLdsStage = 48 * 1024;   // LDS budget with staged weights
-constexpr size_t kZoomLdsMax = 160 * 1024;
+constexpr size_t kZoomLdsStage = 64 * 1024;   // LDS budget with staged weights
+constexpr size_t kZoomLdsMax = 156 * 1024;    // dynamic LDS; k_zoom's static arrays take the rest
+constexpr int kZoomRowTiles = 8;               // H chunk: at most 8 row tiles (128 rows)
 
-bool zoom_fits(int R, int zf) { return zoom_lds_bytes(R, zf, false) <= kZoomLdsMax; }
+// row tiles of the screen's H chunk: at most kZoomRowTiles and what fits
+// beside the rest, balanced over the chunks (0: the zoom does not fit)
+static int zoom_row_tiles(int R, int zf, bool stage_d)
+{
+    const int ZT = (zf * R + 15) / 16;
+    const ZoomLayout base = zoom_layout(R, zf, stage_d, 0);
+    if (base.total >= kZoomLdsMax) return 0;
+    const size_t per = 2 * 16 * (size_t)((R + 31) / 32 * 32 + 8);
+    const int fit = (int)std::min<size_t>(kZoomRowTiles, (kZoomLdsMax - base.total) / per);
+    if (fit < 1) return 0;
+    const int chunks = (ZT + fit - 1) / fit;
+    return (ZT + chunks - 1) / chunks;
+}
+
+bool zoom_fits(int R, int zf) { return R <= 160 && zoom_row_tiles(R, zf, false) > 0; }
+
+int zoom_fragment_count(int R, int zf) { return (zf * R + 15) / 16 * ((R + 31) / 32) * kWave * 8; }
 
 hipError_t launch_zoom(hipStream_t s, const float *gamma_refined, int pairs, int R, int zf,
-                       const double *dirichlet, const int2 *window, const float *coarse_peak,
-                       int margin, int rf, float *offsets, float *refined_peak, int2 *zoom_loc)
+                       const double *dirichlet, const uint16_t *dfrag, const uint16_t *pfrag,
+                       const ZoomBounds &bounds, const int2 *window, const float *coarse_peak,
+                       int margin, int rf, float *offsets, float *refined_peak, int2 *zoom_loc,
+                       const ZoomScratch &scratch, unsigned *stats)
 {
-    if (!zoom_fits(R, zf)) return hipErrorInvalidValue;
-    // the Dirichlet weights of both passes go to LDS when they fit beside the rest
-    const bool stage_d = zoom_lds_bytes(R, zf, true) <= kZoomLdsStage;
-    const size_t lds = zoom_lds_bytes(R, zf, stage_d);
-    if (stage_d)
-        hipLaunchKernelGGL(k_zoom<true>, dim3(pairs), dim3(kZoomThreads), lds, s, gamma_refined, R,
-                           zf, dirichlet, window, coarse_peak, margin, rf, offsets, refined_peak,
-                           zoom_loc);
-    else
-        hipLaunchKernelGGL(k_zoom<false>, dim3(pairs), dim3(kZoomThreads), lds, s, gamma_refined,
-                           R, zf, dirichlet, window, coarse_peak, margin, rf, offsets,
-                           refined_peak, zoom_loc);
+    const double ecoef = bounds.ecoef, eabs = bounds.eabs;
+    if (!zoom_fits(R, zf) || !scratch.thr || !scratch.win || !scratch.flag)
+        return hipErrorInvalidValue;
+    if (pairs == 0) return hipSuccess;
+    if (stats) {
+        hipLaunchKernelGGL(k_guard_clear, dim3(1), dim3(64), 0, s, stats);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    const ZoomHandoff ho{scratch.thr, reinterpret_cast<int4 *>(scratch.win), scratch.flag};
+    const auto *df = reinterpret_cast<const uint4 *>(dfrag);
+    // the batched screen keeps a wave's fragments (NT tiles x KC chunks) in registers
+    const int ZT = (zf * R + 15) / 16, KC = (R + 31) / 32;
+    const int NT = (ZT + 3) / 4;
+    const int nt = NT <= 2 ? 2 : NT <= 3 ? 3 : NT <= 5 ? 5 : NT <= 9 ? 9 : 0;
+    // one wave per pair where both fragment tables fit in LDS, else the block screen
+    const bool screen1 = ZT <= kScreenMaxTiles && KC <= 2 && zoom_screen1_lds(R, zf) <= kZoomLdsMax;
+    const bool screen = screen1 || (nt > 0 && nt * KC <= 24 &&
+                                    zoom_screen_lds(R, zf) <= kZoomLdsMax &&
+                                    R * R <= kZoomScreenLoads * kZoomThreads);
+    // 1. the exact window
+    {
+        const bool stage_d = zoom_layout(R, zf, true, 0).gt <= kZoomLdsStage;
+        const size_t lds = zoom_layout(R, zf, stage_d, 0).gt;
+        if (stage_d)
+            hipLaunchKernelGGL(k_zoom_window<true>, dim3(pairs), dim3(kZoomThreads), lds, s,
+                               gamma_refined, R, zf, dirichlet, ecoef, eabs, screen ? 0 : 1,
+                               window, coarse_peak, margin, rf, offsets, refined_peak, zoom_loc, ho);
+        else
+            hipLaunchKernelGGL(k_zoom_window<false>, dim3(pairs), dim3(kZoomThreads), lds, s,
+                               gamma_refined, R, zf, dirichlet, ecoef, eabs, screen ? 0 : 1,
+                               window, coarse_peak, margin, rf, offsets, refined_peak, zoom_loc, ho);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    // 2. the batched screen
+    if (screen1) {
+        const int blocks = std::min((pairs + kScreenWaves - 1) / kScreenWaves, 2 * 256);
+        const size_t lds = zoom_screen1_lds(R, zf);
+        const auto *pf = reinterpret_cast<const uint4 *>(pfrag);
+        if (KC == 1)
+            hipLaunchKernelGGL((k_zoom_screen1<1>), dim3(blocks), dim3(kScreenWaves * kWave), lds,
+                               s, gamma_refined, pairs, R, zf, df, pf, bounds.lambda,
+                               bounds.hcoef, bounds.habs, ho);
+        else
+            hipLaunchKernelGGL((k_zoom_screen1<2>), dim3(blocks), dim3(kScreenWaves * kWave), lds,
+                               s, gamma_refined, pairs, R, zf, df, pf, bounds.lambda,
+                               bounds.hcoef, bounds.habs, ho);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    } else if (screen) {
+        const dim3 grid((pairs + kZoomScreenPairs - 1) / kZoomScreenPairs);
+        const size_t lds = zoom_screen_lds(R, zf);
+#define AMPCOR_SCREEN(KCV, NTV)                                                                  \
+    hipLaunchKernelGGL((k_zoom_screen<KCV, NTV>), grid, dim3(kZoomThreads), lds, s, gamma_refined, \
+                       pairs, R, zf, df, ho)
+#define AMPCOR_SCREEN_NT(KCV)                               \
+    switch (nt) {                                           \
+    case 2: AMPCOR_SCREEN(KCV, 2); break;                   \
+    case 3: AMPCOR_SCREEN(KCV, 3); break;                   \
+    case 5: AMPCOR_SCREEN(KCV, 5); break;                   \
+    default: AMPCOR_SCREEN(KCV, 9); break;                  \
+    }
+        switch (KC) {
+        case 1: AMPCOR_SCREEN_NT(1) break;
+        case 2: AMPCOR_SCREEN_NT(2) break;
+        case 3: AMPCOR_SCREEN(3, 5); break;   // nt * KC <= 24: nt <= 5 here
+        case 4: AMPCOR_SCREEN(4, 5); break;
+        default: AMPCOR_SCREEN(5, 3); break;
+        }
+#undef AMPCOR_SCREEN_NT
+#undef AMPCOR_SCREEN
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    // 3. the full search of the flagged pairs
+    const bool stage_d = zoom_layout(R, zf, true, zoom_row_tiles(R, zf, true)).total <=
+                         kZoomLdsStage;
+    const int rt = zoom_row_tiles(R, zf, stage_d);
+    const size_t lds = zoom_layout(R, zf, stage_d, rt).total;
+#define AMPCOR_ZOOM(ST, KCV)                                                                     \
+    hipLaunchKernelGGL((k_zoom_full<ST, KCV>), dim3(pairs), dim3(kZoomThreads), lds, s,          \
+                       gamma_refined, R, zf, dirichlet, df, ecoef, eabs, rt, window, coarse_peak, \
+                       margin, rf, offsets, refined_peak, zoom_loc, stats, scratch.flag)
+#define AMPCOR_ZOOM_KC(ST)                                  \
+    switch (KC) {                                           \
+    case 1: AMPCOR_ZOOM(ST, 1); break;                      \
+    case 2: AMPCOR_ZOOM(ST, 2); break;                      \
+    case 3: AMPCOR_ZOOM(ST, 3); break;                      \
+    case 4: AMPCOR_ZOOM(ST, 4); break;                      \
+    case 5: AMPCOR_ZOOM(ST, 5); break;                      \
+    default: return hipErrorInvalidValue;                   \
+    }
+    if (stage_d) {
+        AMPCOR_ZOOM_KC(true)
+    } else {
+        AMPCOR_ZOOM_KC(false)
+    }
+#undef AMPCOR_ZOOM_KC
+#undef AMPCOR_ZOOM
     return hipGetLastError();
 }
 

@@ -128,6 +128,57 @@ int upload_fragments(DeviceBuffer<uint16_t> &hf, int N)
     return e == hipSuccess ? AMPCOR_HIP_OK : hip_fail(e, "uploading the upsampling fragments");
 }
 
+// the zoom's FP16 screen: the Dirichlet matrix M[p][a] = D(p - zf a) as
+// 16 x 32 MFMA fragments (zoom_fragment_count, in both K orders), and the
+// bounds on the screen's error.  With Lambda = max_p sum_a |D(p - zf a)| (the Lebesgue constant of
+// the zoom), every |H[p][b]| <= Lambda max|g| and sum_b |D(q - zf b)| <=
+// Lambda.  Rounding g, D, D again and H to FP16 (relative 2^-11 each, four
+// times) and FP32 accumulation over KP = R padded to 32 terms in each product
+// give |f_screen - f| <= Lambda^2 max|g| (4 2^-11 + 2 KP 2^-24); FP16
+// subnormals (absolute 2^-25 per rounding) add 2^-25 (Lambda^2 + 3 KP Lambda).
+// After the first product alone: Lambda max|g| (2 2^-11 + KP 2^-24) and
+// 2^-25 (Lambda + KP).  All carry a 1.25 margin for the second-order terms.
+int upload_zoom_screen(ampcor_hip_plan *p, const std::vector<double> &table /* d + Z */)
+{
+    const int R = p->R, zf = p->zf, Z = p->Z;
+    const int ZT = (Z + 15) / 16, KC = (R + 31) / 32, KP = 32 * KC;
+    std::vector<uint16_t> h((size_t)zoom_fragment_count(R, zf), 0), hp(h.size(), 0);
+    size_t at = 0;
+    for (int zt = 0; zt < ZT; ++zt)
+        for (int kc = 0; kc < KC; ++kc)
+            for (int l = 0; l < 64; ++l)
+                for (int e = 0; e < 8; ++e, ++at) {
+                    const int row = 16 * zt + l % 16, lk = l / 16;
+                    const int a = 32 * kc + 8 * lk + e;
+                    const int b = 32 * kc + (e < 4 ? 4 * lk + e : 16 + 4 * lk + e - 4);
+                    if (row < Z && a < R)
+                        h[at] = __builtin_bit_cast(uint16_t, (_Float16)table[row - zf * a + Z]);
+                    if (row < Z && b < R)
+                        hp[at] = __builtin_bit_cast(uint16_t, (_Float16)table[row - zf * b + Z]);
+                }
+    double lambda = 0.0;
+    for (int row = 0; row < Z; ++row) {
+        double sum = 0.0;
+        for (int a = 0; a < R; ++a) sum += std::fabs(table[row - zf * a + Z]);
+        lambda = std::max(lambda, sum);
+    }
+    auto &b = p->zoom_bounds;
+    b.ecoef = 1.25 * lambda * lambda * (4.0 * std::ldexp(1.0, -11) + 2.0 * KP * std::ldexp(1.0, -24));
+    b.eabs = 1.25 * std::ldexp(1.0, -25) * (lambda * lambda + 3.0 * KP * lambda);
+    // after the first product alone (g and D rounded, FP32 accumulation)
+    b.hcoef = (float)(1.25 * lambda * (2.0 * std::ldexp(1.0, -11) + KP * std::ldexp(1.0, -24)));
+    b.habs = (float)(1.25 * std::ldexp(1.0, -25) * (lambda + KP));
+    b.lambda = (float)(lambda * (1.0 + std::ldexp(1.0, -20)));
+    hipError_t e = p->zfrag.alloc(h.size());
+    if (e == hipSuccess)
+        e = hipMemcpy(p->zfrag.ptr, h.data(), h.size() * sizeof(uint16_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = p->zfragp.alloc(hp.size());
+    if (e == hipSuccess)
+        e = hipMemcpy(p->zfragp.ptr, hp.data(), hp.size() * sizeof(uint16_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = p->zstats.alloc(3);
+    return e == hipSuccess ? AMPCOR_HIP_OK : hip_fail(e, "uploading the zoom screen");
+}
+
 int register_tile(ampcor_hip_plan *plan, const ampcor_hip_raster *raster, size_t idx,
                   size_t bl, size_t bs, size_t el, size_t es, bool reference)
 {
@@ -397,10 +448,17 @@ int ampcor_hip_plan_create(int device, size_t pairs, size_t ref_lines, size_t re
         A(p->mstats.alloc(N));
         A(p->mstats_r.alloc(N));
     }
+    if (p->fp8) {
+        A(p->guard.alloc(3));
+        A(p->near.alloc(N));
+    }
     A(p->gamma_r.alloc(N * R * R));
     A(p->offsets.alloc(2 * N));
     A(p->peak_r.alloc(N));
     A(p->zoom_loc.alloc(N));
+    A(p->zthr.alloc(N));
+    A(p->zwin.alloc(4 * N));
+    A(p->zflag.alloc(N));
     if (r.e != hipSuccess) {
         std::string msg = std::string("allocating plan buffers for ") + std::to_string(N) +
                           " pairs: " + hipGetErrorString(r.e);
@@ -438,6 +496,7 @@ int ampcor_hip_plan_create(int device, size_t pairs, size_t ref_lines, size_t re
         if (e == hipSuccess)
             e = hipMemcpy(p->D.ptr, d.data(), d.size() * sizeof(double), hipMemcpyHostToDevice);
         if (e != hipSuccess) return cleanup(hip_fail(e, "uploading the zoom kernel table"));
+        if (int rc = upload_zoom_screen(p, d)) return cleanup(rc);
     }
     for (auto &ev : p->ev) {
         e = hipEventCreate(&ev);
@@ -603,8 +662,16 @@ int plan_enqueue(ampcor_hip_plan *plan, int N)
              "correlate");
     }
     STEP(launch_coarse_peak(s, p->gamma.ptr, N, p->P, p->W, p->E, p->rm, p->rf, p->loc.ptr,
-                            p->window.ptr, p->peak.ptr, p->snr.ptr),
+                            p->window.ptr, p->peak.ptr, p->snr.ptr,
+                            p->fp8 ? peak_guard_margin(p->C) : 0.0f, p->fp8 ? p->near.ptr : nullptr),
          "coarse_peak");
+    // the FP8 surface's near-ties are re-scored in FP64 before the refinement
+    // is centred on the peak
+    if (p->fp8)
+        STEP(launch_peak_guard(s, p->d_desc.ptr, N, p->dtype, p->C, p->W, p->E, p->rm, p->rf,
+                               p->gamma.ptr, p->near.ptr, p->loc.ptr, p->window.ptr, p->peak.ptr,
+                               p->snr.ptr, p->guard.ptr),
+             "peak_guard");
     const bool fused = p->mixed && upsample_fuses_quantization(p->C, p->rf) &&
                        upsample_fuses_quantization(p->E, p->rf);
     // plain mixed forms with the fused upsampling: the refined window sums come
@@ -672,8 +739,9 @@ int plan_enqueue(ampcor_hip_plan *plan, int N)
                                   p->stats_r.ptr, N, p->Cr, p->Wr, p->gamma_r.ptr),
                  "normalize_refined");
     }
-    STEP(launch_zoom(s, p->gamma_r.ptr, N, p->R, p->zf, p->D.ptr, p->window.ptr, p->peak.ptr, p->m,
-                     p->rf, p->offsets.ptr, p->peak_r.ptr, p->zoom_loc.ptr),
+    STEP(launch_zoom(s, p->gamma_r.ptr, N, p->R, p->zf, p->D.ptr, p->zfrag.ptr, p->zfragp.ptr,
+                     p->zoom_bounds, p->window.ptr, p->peak.ptr, p->m, p->rf, p->offsets.ptr,
+                     p->peak_r.ptr, p->zoom_loc.ptr, p->zoom_scratch(), p->zstats.ptr),
          "zoom");
 #undef STEP
     plan->stages_run = stage;
@@ -766,6 +834,58 @@ int ampcor_hip_stage_gamma_refined(ampcor_hip_plan *plan, float *gamma)
     if (!plan->adjusted) return fail(AMPCOR_HIP_ESTATE, "stage requested before adjust");
     DeviceGuard guard(plan->device);
     return copy_out(plan, gamma, plan->gamma_r.ptr, plan->pairs * plan->R * plan->R * sizeof(float));
+}
+
+int ampcor_hip_stage_zoom(ampcor_hip_plan *plan, const float *surfaces)
+{
+    if (!plan || !surfaces) return fail(AMPCOR_HIP_EINVAL, "null argument");
+    DeviceGuard guard(plan->device);
+    hipStream_t s = plan->stream;
+    const size_t N = plan->pairs, RR = (size_t)plan->R * plan->R;
+    HIP_TRY(hipMemcpyAsync(plan->gamma_r.ptr, surfaces, N * RR * sizeof(float),
+                           hipMemcpyHostToDevice, s),
+            "uploading refined surfaces");
+    // window corner (0, 0) and a finite coarse peak: offsets = (p + dp) / (rf zf) - m
+    HIP_TRY(hipMemsetAsync(plan->window.ptr, 0, N * sizeof(int2), s), "clearing windows");
+    HIP_TRY(hipMemsetAsync(plan->peak.ptr, 0, N * sizeof(float), s), "clearing peaks");
+    HIP_TRY(launch_zoom(s, plan->gamma_r.ptr, (int)N, plan->R, plan->zf, plan->D.ptr,
+                        plan->zfrag.ptr, plan->zfragp.ptr, plan->zoom_bounds, plan->window.ptr,
+                        plan->peak.ptr, plan->m, plan->rf, plan->offsets.ptr, plan->peak_r.ptr,
+                        plan->zoom_loc.ptr, plan->zoom_scratch(), plan->zstats.ptr),
+            "zoom");
+    HIP_TRY(hipStreamSynchronize(s), "zoom");
+    plan->adjusted = true;
+    return AMPCOR_HIP_OK;
+}
+
+int ampcor_hip_zoom_stats(ampcor_hip_plan *plan, int64_t *cells, int64_t *pairs,
+                          int64_t *searched)
+{
+    if (!plan) return fail(AMPCOR_HIP_EINVAL, "null plan");
+    if (!plan->adjusted) return fail(AMPCOR_HIP_ESTATE, "stage requested before adjust");
+    unsigned c[3] = {0, 0, 0};
+    DeviceGuard guard(plan->device);
+    if (int rc = copy_out(plan, c, plan->zstats.ptr, sizeof(c))) return rc;
+    if (cells) *cells = c[0];
+    if (pairs) *pairs = c[1];
+    if (searched) *searched = c[2];
+    return AMPCOR_HIP_OK;
+}
+
+int ampcor_hip_fp8_guard_stats(ampcor_hip_plan *plan, int64_t *rescored, int64_t *moved,
+                               int64_t *overflow)
+{
+    if (!plan) return fail(AMPCOR_HIP_EINVAL, "null plan");
+    if (!plan->adjusted) return fail(AMPCOR_HIP_ESTATE, "stage requested before adjust");
+    unsigned c[3] = {0, 0, 0};
+    if (plan->fp8) {
+        DeviceGuard guard(plan->device);
+        if (int rc = copy_out(plan, c, plan->guard.ptr, sizeof(c))) return rc;
+    }
+    if (rescored) *rescored = c[0];
+    if (moved) *moved = c[1];
+    if (overflow) *overflow = c[2];
+    return AMPCOR_HIP_OK;
 }
 
 int ampcor_hip_stage_locations(ampcor_hip_plan *plan, int32_t *coarse, int32_t *window,
@@ -953,6 +1073,8 @@ size_t plan_bytes_per_pair(int C, int W, int rf, int rm, int precision)
              2 * sizeof(MixedStats);
     if (precision == AMPCOR_HIP_BF16X3 || precision == AMPCOR_HIP_FP16X3)
         b += 2 * Cr * (Cr + 48) + 2 * Wr * Wr;
+    if (precision == AMPCOR_HIP_FP8) b += sizeof(int);  // the peak guard's tie counts
+    b += sizeof(double) + 4 * sizeof(int32_t) + sizeof(int);  // the zoom kernels' hand-off
     // banded upsampling scratch for tiles above 96 samples
     size_t nmax = 0;
     if (upsample_needs_scratch(C, rf)) nmax = std::max(nmax, (size_t)C);

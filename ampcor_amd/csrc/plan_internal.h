@@ -92,6 +92,15 @@ struct ampcor_hip_plan {
     ampcor_hip::DeviceBuffer<float> amp_ref_r, amp_tgt_r, rT_r, partial_r, gamma_r;
     ampcor_hip::DeviceBuffer<float> U_ref, U_tgt, g_ref, g_tgt;
     ampcor_hip::DeviceBuffer<double> D;
+    // the zoom's FP16 screen: Dirichlet fragments, its error bound
+    // (ecoef max|g| + eabs) and counts of the cells it sent to exact evaluation
+    ampcor_hip::DeviceBuffer<uint16_t> zfrag, zfragp;
+    ampcor_hip::ZoomBounds zoom_bounds{};
+    ampcor_hip::DeviceBuffer<unsigned> zstats;
+    ampcor_hip::DeviceBuffer<double> zthr;  // the zoom kernels' hand-off, per pair
+    ampcor_hip::DeviceBuffer<int32_t> zwin;
+    ampcor_hip::DeviceBuffer<int> zflag;
+    ampcor_hip::ZoomScratch zoom_scratch() const { return {zthr.ptr, zwin.ptr, zflag.ptr}; }
     ampcor_hip::DeviceBuffer<uint16_t> chip_q, tgt_q, chip_q_r, tgt_q_r, chip_lo_r, tgt_lo_r;
     ampcor_hip::DeviceBuffer<uint16_t> hfrag_ref, hfrag_tgt;  // FP16 circulant fragments
     // scratch of the banded upsampler (tiles above 96 samples)
@@ -99,6 +108,9 @@ struct ampcor_hip_plan {
     ampcor_hip::DeviceBuffer<double2> up_band, up_rows;
     ampcor_hip::UpsampleScratch up_scratch{};
     ampcor_hip::DeviceBuffer<ampcor_hip::MixedStats> mstats, mstats_r;
+    // FP8 peak guard counters: re-scored, moved, over the candidate limit
+    ampcor_hip::DeviceBuffer<unsigned> guard;
+    ampcor_hip::DeviceBuffer<int> near;  // cells within the guard's margin of each FP8 peak
     // pinned host staging of the offsets for adjust's host copy (allocated on
     // its first use): a copy from pageable memory goes through the runtime's
     // own staging, with a synchronization per chunk
@@ -134,7 +146,15 @@ struct ampcor_hip_plan {
         up_rows.release();
         mstats.release();
         mstats_r.release();
+        guard.release();
+        near.release();
         D.release();
+        zfrag.release();
+        zfragp.release();
+        zstats.release();
+        zthr.release();
+        zwin.release();
+        zflag.release();
         stats.release();
         stats_r.release();
         loc.release();

@@ -71,7 +71,11 @@ def cpu_baseline(scene, seconds):
     """FP64 oracle on the host cores over a bounded sample of the same pairs."""
     from oracle import pyoracle
 
-    cores = min(len(os.sched_getaffinity(0)), 16)
+    affinity = len(os.sched_getaffinity(0))
+    # the host's CPU share: OMP_NUM_THREADS where the environment sets it (the
+    # GPU hosts give each GPU a fixed share of cores), else every core this
+    # process may run on
+    cores = int(os.environ.get("OMP_NUM_THREADS") or affinity)
     args = (scene.reference, scene.target)
     geo = (scene.chip, scene.window, scene.refine_factor, scene.refine_margin, scene.zoom_factor)
 
@@ -92,14 +96,20 @@ def cpu_baseline(scene, seconds):
     per_pair_1 = dt * cores / len(idx)
     n1 = int(max(4, min(len(idx), seconds / 3 / max(per_pair_1, 1e-6))))
     dt1, idx1, _ = run(n1, 1)
-    # the FP32 twin (the GPU coarse stage's rounding) on a quarter of the sample
-    _, idx_t, twin = run(max(cores, len(idx) // 4), cores, form="fp32")
+    # the FP32 twin (the GPU coarse stage's rounding, FP64 refinement) on a
+    # quarter of the sample: the CPU-FP32 all-core rate
+    dt_t, idx_t, twin = run(max(cores, len(idx) // 4), cores, form="fp32")
     return ({"value": len(idx) / dt, "unit": "tile pairs/sec", "cores": cores, "kind": "port",
+             "affinity_cores": affinity,
+             "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
              "sample": f"FP64 oracle (oracle/ampcor_oracle.c, OpenMP over pairs) on {len(idx)} "
-                       f"of the {scene.pairs} pairs, evenly spaced, {dt:.1f} s",
+                       f"of the {scene.pairs} pairs, evenly spaced, {dt:.1f} s, {cores} threads",
              "single_core": {"value": len(idx1) / dt1, "cores": 1,
                              "sample": f"the same oracle on one core, {len(idx1)} pairs, "
-                                       f"{dt1:.1f} s"}},
+                                       f"{dt1:.1f} s"},
+             "fp32_all_core": {"value": len(idx_t) / dt_t, "cores": cores,
+                               "sample": f"the FP32 twin (FP32 coarse stage, FP64 refinement) on "
+                                         f"{len(idx_t)} pairs, {dt_t:.1f} s"}},
             idx, exact, idx_t, twin)
 
 
@@ -200,7 +210,10 @@ def main():
         for _ in range(a.warmup):
             lib.adjust(worker)
         # setup, outside the timed steps: the plan autotuner picks the tilings
-        # of both contractions on this workload's data
+        # of both contractions on this workload's data (it times them on the
+        # data of an adjust, so one runs first even without warm-up)
+        if a.warmup == 0:
+            lib.adjust(worker)
         lib.autotune(worker, 3)
         lib.adjust(worker)
         lib.set_timing(worker, True)
@@ -223,6 +236,8 @@ def main():
         host_step = (time.perf_counter() - t) / a.steps
         prods = {k: np.frombuffer(v, np.float32).copy() for k, v in lib.products(worker).items()}
         prods["offsets"] = prods["offsets"].reshape(N, 2)
+        prods["guard"] = lib.fp8_guard(worker)
+        prods["zoom_screen"] = lib.zoom_stats(worker)
         prods["coarse_loc"] = np.frombuffer(lib.stage_locations(worker)["coarse"],
                                             np.int32).reshape(N, 2).copy()
         print(f"[bench] {precision} stage ms/step: " +
@@ -284,6 +299,9 @@ def main():
             accuracy[f"{f}_snr_max_rel_err_vs_fp64"] = rel(p["snr"][idx], exact["snr"])
             accuracy[f"{f}_refined_peak_max_rel_err_vs_fp64"] = rel(p["refined_peak"][idx],
                                                                     exact["refined_peak"])
+        if "fp8" in runs:
+            accuracy["fp8_peak_guard"] = runs["fp8"][2]["guard"]
+        accuracy["zoom_screen_exact_cells"] = {f: runs[f][2]["zoom_screen"] for f in forms}
         accuracy["fp32_coarse_loc_equal_fp32_twin"] = int(
             np.all(runs["fp32"][2]["coarse_loc"][idx_t] == twin["coarse_loc"], axis=1).sum())
         accuracy["fp32_coarse_peak_bitwise_equal_fp32_twin"] = int(

@@ -141,6 +141,27 @@ int ampcor_hip_stage_gamma_refined(ampcor_hip_plan *plan, float *gamma);
  * (top, left), zoomed argmax (p, q) */
 int ampcor_hip_stage_locations(ampcor_hip_plan *plan, int32_t *coarse, int32_t *window,
                                int32_t *zoomed);
+/* run the zoom stage alone on host refined surfaces [pairs][R][R] (the
+ * reference drives _zoomcor the same way in its tests): the products then hold
+ * its results -- refined peak, zoomed location (stage_locations), and offsets
+ * (p + dp) / (rf zf) - m per axis, as for a window at (0, 0).  Overwrites the
+ * products of the last adjust. */
+int ampcor_hip_stage_zoom(ampcor_hip_plan *plan, const float *surfaces);
+/* the zoom's search in the last adjust (or stage_zoom): the zoom takes the
+ * first maximum over every cell of the zoomed grid, evaluating exactly a
+ * window around each refined surface's largest sample and screening the rest
+ * in FP16.  searched: pairs given the whole-grid search (their screen reached
+ * the window's maximum, or it peaked on the window's edge); cells: cells
+ * outside the windows evaluated exactly there; pairs: those that had any */
+int ampcor_hip_zoom_stats(ampcor_hip_plan *plan, int64_t *cells, int64_t *pairs,
+                          int64_t *searched);
+/* the FP8 form's peak guard in the last adjust: pairs whose FP8 surface had
+ * more than one placement within 0.75 / chip of its peak (each re-scored by
+ * the FP64 correlation of the raster amplitudes), pairs whose peak location
+ * moved, and pairs with more candidates than the guard re-scores (16; their
+ * FP8 peak is kept).  All zero for the other forms. */
+int ampcor_hip_fp8_guard_stats(ampcor_hip_plan *plan, int64_t *rescored, int64_t *moved,
+                               int64_t *overflow);
 /* per-stage device time of the last adjust, milliseconds, measured with HIP
  * events on the plan's stream; names[i] point to static strings.  Timing is
  * off until enabled, because events serialize the stream. */

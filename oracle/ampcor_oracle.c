@@ -31,9 +31,11 @@
  *               Nyquist bin split in half for even N, inverse DFT, 1/N^2, so
  *               refined sample rf*n equals coarse sample n;
  *   zoom        band-limited x zf interpolation of the real refined surface
- *               (R = 2 rf rm + 1 is odd: periodic Dirichlet kernel), searched
- *               on the (2 zf + 1)^2 zoom cells around the refined peak;
- *   sub-pixel   three-point parabola per axis through the zoomed peak.
+ *               (R = 2 rf rm + 1 is odd: periodic Dirichlet kernel) on the
+ *               whole Z x Z grid, first maximum over every cell (as
+ *               Sequential.icc:122-124 zooms the whole surface);
+ *   sub-pixel   three-point parabola per axis through the zoomed peak (none
+ *               at the edge of the zoomed grid).
  *
  * Parity pins (tests/test_oracle.py):
  *   oracle_sat / oracle_sat_sum    SumArea closed forms on ones (S = (i+1)(j+1),
@@ -48,6 +50,12 @@
  *                                  on the rf-subgrid;
  *   oracle_dirichlet/zoom_full     NumPy FFT zoom golden vectors; identity on
  *                                  the zf-subgrid; cosine bump peak location;
+ *   oracle_parabola                exact vertex of sampled quadratics;
+ *   oracle_zoom_search             its (p, q) and peak are the argmax of the
+ *                                  NumPy FFT zoom (golden surfaces, random and
+ *                                  known-shift refined surfaces, peaks more
+ *                                  than a refined sample from the surface's
+ *                                  argmax); off-grid known shifts recovered;
  *   oracle_snr                     closed form on a crafted surface;
  *   oracle_adjust (FP64)           Fourier-shifted speckle recovers the known
  *                                  shift; planted chip recovers the placement;
@@ -458,51 +466,67 @@ static inline void load_complex(int dtype, const void *raster, int64_t samples, 
 
 static inline double amp64(double re, double im) { return sqrt(re * re + im * im); }
 
-/* zoom on the (2 zf + 1)^2 cells around zf * (a, b); FP64 */
+/* three-point parabola through (-1, fm), (0, f0), (+1, fp): the abscissa of
+ * its vertex, clamped to [-1/2, 1/2]; 0 unless the parabola opens downward */
+double oracle_parabola(double fm, double f0, double fp)
+{
+    const double den = fm - 2.0 * f0 + fp;
+    double d = 0.0;
+    if (den < 0.0) d = 0.5 * (fm - fp) / den;
+    return d < -0.5 ? -0.5 : (d > 0.5 ? 0.5 : d);
+}
+
+/* full-grid zoom search, as the reference zooms the whole refined surface and
+ * takes the maximum over every zoomed cell (Sequential.icc:122-124): f = the
+ * Z x Z band-limited zoom (H[p][b] = sum_a D(p - zf a) g[a][b], then
+ * f[p][q] = sum_b H[p][b] D(q - zf b), sums in index order), its first
+ * maximum in row-major order, and the parabola through the neighbours of that
+ * cell on each axis (none at the edge of the grid).  FP64. */
 static void zoom_peak(const double *g /* R*R, NaN -> 0 */, int R, int zf, const double *D /* index d + Z */,
-                      int a, int b, int *pp, int *pq, double *fpeak, double *dp, double *dq)
+                      double *H /* Z*R */, double *f /* Z*Z */, int *pp, int *pq, double *fpeak,
+                      double *dp, double *dq)
 {
     const int Z = zf * R;
-    const int p0 = (zf * a - zf < 0) ? 0 : zf * a - zf;
-    const int p1 = (zf * a + zf > Z - 1) ? Z - 1 : zf * a + zf;
-    const int q0 = (zf * b - zf < 0) ? 0 : zf * b - zf;
-    const int q1 = (zf * b + zf > Z - 1) ? Z - 1 : zf * b + zf;
-    const int np = p1 - p0 + 1, nq = q1 - q0 + 1;
-    double H[64 * 128];
-    double f[64 * 64];
-    for (int ip = 0; ip < np; ++ip)
-        for (int bb = 0; bb < R; ++bb) {
+    for (int p = 0; p < Z; ++p)
+        for (int b = 0; b < R; ++b) {
             double acc = 0.0;
-            for (int aa = 0; aa < R; ++aa) acc += D[(p0 + ip) - zf * aa + Z] * g[aa * R + bb];
-            H[ip * R + bb] = acc;
+            for (int a = 0; a < R; ++a) acc += D[p - zf * a + Z] * g[a * R + b];
+            H[(size_t)p * R + b] = acc;
         }
-    for (int ip = 0; ip < np; ++ip)
-        for (int iq = 0; iq < nq; ++iq) {
+    for (int p = 0; p < Z; ++p)
+        for (int q = 0; q < Z; ++q) {
             double acc = 0.0;
-            for (int bb = 0; bb < R; ++bb) acc += H[ip * R + bb] * D[(q0 + iq) - zf * bb + Z];
-            f[ip * nq + iq] = acc;
+            for (int b = 0; b < R; ++b) acc += H[(size_t)p * R + b] * D[q - zf * b + Z];
+            f[(size_t)p * Z + q] = acc;
         }
-    int bi = 0;
-    for (int k = 1; k < np * nq; ++k)
+    size_t bi = 0;
+    for (size_t k = 1; k < (size_t)Z * Z; ++k)
         if (f[k] > f[bi]) bi = k;
-    const int ip = bi / nq, iq = bi % nq;
-    *pp = p0 + ip;
-    *pq = q0 + iq;
+    const int ip = (int)(bi / Z), iq = (int)(bi % Z);
+    *pp = ip;
+    *pq = iq;
     *fpeak = f[bi];
-    double d = 0.0;
-    if (ip > 0 && ip < np - 1) {
-        const double fm = f[(ip - 1) * nq + iq], f0 = f[bi], fp = f[(ip + 1) * nq + iq];
-        const double den = fm - 2.0 * f0 + fp;
-        if (den < 0.0) d = 0.5 * (fm - fp) / den;
-    }
-    *dp = d < -0.5 ? -0.5 : (d > 0.5 ? 0.5 : d);
-    d = 0.0;
-    if (iq > 0 && iq < nq - 1) {
-        const double fm = f[ip * nq + iq - 1], f0 = f[bi], fp = f[ip * nq + iq + 1];
-        const double den = fm - 2.0 * f0 + fp;
-        if (den < 0.0) d = 0.5 * (fm - fp) / den;
-    }
-    *dq = d < -0.5 ? -0.5 : (d > 0.5 ? 0.5 : d);
+    *dp = (ip > 0 && ip < Z - 1) ? oracle_parabola(f[bi - Z], f[bi], f[bi + Z]) : 0.0;
+    *dq = (iq > 0 && iq < Z - 1) ? oracle_parabola(f[bi - 1], f[bi], f[bi + 1]) : 0.0;
+}
+
+int oracle_zoom_search(const double *g, int R, int zf, int *p, int *q, double *peak, double *dp,
+                       double *dq)
+{
+    if (R < 1 || zf < 1) return 1;
+    const int Z = zf * R;
+    double *D = (double *)malloc(sizeof(double) * (2 * Z + 1));
+    double *h = (double *)malloc(sizeof(double) * R * R);
+    double *H = (double *)malloc(sizeof(double) * Z * R);
+    double *f = (double *)malloc(sizeof(double) * Z * Z);
+    for (int d = -Z; d <= Z; ++d) D[d + Z] = oracle_dirichlet(R, zf, d);
+    for (int c = 0; c < R * R; ++c) h[c] = isnan(g[c]) ? 0.0 : g[c];
+    zoom_peak(h, R, zf, D, H, f, p, q, peak, dp, dq);
+    free(f);
+    free(H);
+    free(h);
+    free(D);
+    return 0;
 }
 
 int oracle_adjust(int form, int dtype,
@@ -519,7 +543,7 @@ int oracle_adjust(int form, int dtype,
     if (dtype != ORACLE_COMPLEX64 && dtype != ORACLE_FLOAT32) return 1;
     if (!out.offsets) return 1;
     const int P = W - C + 1, E = C + 2 * rm, Cr = rf * C, Wr = rf * E, R = Wr - Cr + 1, Z = zf * R;
-    if (2 * zf + 1 > 64 || R > 128) return 1;
+    if (R > 129) return 1;
     for (int64_t k = 0; k < n; ++k) {
         if (ref_begin[2 * k] < 0 || ref_begin[2 * k + 1] < 0 || ref_begin[2 * k] + C > ref_lines ||
             ref_begin[2 * k + 1] + C > ref_samples)
@@ -553,6 +577,8 @@ int oracle_adjust(int form, int dtype,
         double *aref = (double *)malloc(sizeof(double) * Cr * Cr);
         double *atgt = (double *)malloc(sizeof(double) * Wr * Wr);
         double *gR = (double *)malloc(sizeof(double) * R * R);
+        double *zH = (double *)malloc(sizeof(double) * Z * R);
+        double *zf_ = (double *)malloc(sizeof(double) * Z * Z);
 
 #ifdef _OPENMP
 #pragma omp for schedule(dynamic, 1)
@@ -623,7 +649,7 @@ int oracle_adjust(int form, int dtype,
                 if (isnan(gR[c])) gR[c] = 0.0;
             int pp, pq;
             double fpeak, dp, dq;
-            zoom_peak(gR, R, zf, D, a, b, &pp, &pq, &fpeak, &dp, &dq);
+            zoom_peak(gR, R, zf, D, zH, zf_, &pp, &pq, &fpeak, &dp, &dq);
             const double zoom = (double)rf * zf;
             float dl = (float)((double)(top - m) + ((double)pp + dp) / zoom);
             float ds = (float)((double)(left - m) + ((double)pq + dq) / zoom);
@@ -640,6 +666,7 @@ int oracle_adjust(int form, int dtype,
             if (out.window_loc) { out.window_loc[2 * k] = top; out.window_loc[2 * k + 1] = left; }
             if (out.zoom_loc) { out.zoom_loc[2 * k] = pp; out.zoom_loc[2 * k + 1] = pq; }
         }
+        free(zf_); free(zH);
         free(gR); free(atgt); free(aref); free(utgt); free(uref); free(ctgt); free(cref);
         free(g32); free(rT); free(tgt32); free(ref32); free(g64); free(S); free(tgt64); free(ref64);
     }

@@ -80,6 +80,15 @@ void oracle_zoom_full(const double *g, int R, int zf, double *out);
 /* periodic Dirichlet kernel of an odd-size R grid at x = d / zf */
 double oracle_dirichlet(int R, int zf, int d);
 
+/* vertex abscissa of the parabola through (-1, fm), (0, f0), (1, fp), clamped
+ * to [-1/2, 1/2]; 0 unless it opens downward (the sub-pixel step) */
+double oracle_parabola(double fm, double f0, double fp);
+/* the pipeline's zoom search on one R x R refined surface (NaN counts as 0):
+ * the first maximum (p, q) of the whole Z x Z zoom, its value and the
+ * sub-pixel steps of the parabolas through its neighbours; 0 on success */
+int oracle_zoom_search(const double *g, int R, int zf, int *p, int *q, double *peak, double *dp,
+                       double *dq);
+
 /* coarse SNR: peak^2 / mean(gamma^2) outside a (2*exclude+1)^2 box */
 double oracle_snr(const double *g, int rows, int cols, int prow, int pcol, int exclude);
 

@@ -54,6 +54,10 @@ def lib():
         L.oracle_zoom_full.argtypes = [_d, c_int, c_int, _d]
         L.oracle_dirichlet.argtypes = [c_int, c_int, c_int]
         L.oracle_dirichlet.restype = ctypes.c_double
+        L.oracle_parabola.argtypes = [ctypes.c_double] * 3
+        L.oracle_parabola.restype = ctypes.c_double
+        L.oracle_zoom_search.argtypes = [_d, c_int, c_int, ip, ip, _d, _d, _d]
+        L.oracle_zoom_search.restype = c_int
         L.oracle_snr.argtypes = [_d, c_int, c_int, c_int, c_int, c_int]
         L.oracle_snr.restype = ctypes.c_double
         L.oracle_adjust.argtypes = [c_int, c_int, vp, c_i64, c_i64, vp, c_i64, c_i64, c_i64, _i, _i,
@@ -141,6 +145,22 @@ def zoom_full(g, zf):
     out = np.empty((zf * R, zf * R))
     lib().oracle_zoom_full(_p(g, _d), R, zf, _p(out, _d))
     return out
+
+
+def parabola(fm, f0, fp):
+    return lib().oracle_parabola(fm, f0, fp)
+
+
+def zoom_search(g, zf):
+    """The pipeline's zoom search on one refined surface: ((p, q), peak, (dp, dq))."""
+    g = np.ascontiguousarray(g, dtype=np.float64)
+    p, q = ctypes.c_int(), ctypes.c_int()
+    pk, dp, dq = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    rc = lib().oracle_zoom_search(_p(g, _d), g.shape[0], zf, ctypes.byref(p), ctypes.byref(q),
+                                  ctypes.byref(pk), ctypes.byref(dp), ctypes.byref(dq))
+    if rc:
+        raise ValueError("zoom search rejected its arguments")
+    return (p.value, q.value), pk.value, (dp.value, dq.value)
 
 
 def snr(g, prow, pcol, exclude):

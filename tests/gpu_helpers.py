@@ -34,5 +34,6 @@ def run_hip(lib, reference, target, ref_begin, tgt_begin, chip, window, rf, rm, 
         "zoom_loc": locs["zoomed"],
         "gamma": np.frombuffer(lib.stage_gamma(worker), np.float32).reshape(n, P, P),
         "gamma_refined": np.frombuffer(lib.stage_gamma_refined(worker), np.float32).reshape(n, R, R),
+        "guard": lib.fp8_guard(worker),
     }
     return out

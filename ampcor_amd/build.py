@@ -23,7 +23,7 @@ INCLUDE = os.path.join(ROOT, "include")
 ARCH = os.environ.get("AMPCOR_HIP_ARCH", "gfx950")
 
 HIP_SOURCES = ["kernels.hip", "mixed.hip", "mixed_nk_a.hip", "mixed_nk_b.hip", "mixed_nk_c.hip",
-               "mixed_nk_d.hip", "fp8.hip", "plan.hip", "server.hip"]
+               "mixed_nk_d.hip", "fp8.hip", "plan.hip", "server.hip", "noalloc.hip"]
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

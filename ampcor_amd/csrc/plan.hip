@@ -77,35 +77,53 @@ int check_device(int device)
     return AMPCOR_HIP_OK;
 }
 
-int upload_matrix(DeviceBuffer<float> &buf, const std::vector<float> &host)
-{
-    HIP_TRY(buf.alloc(host.size()), "allocating an upsampling matrix");
-    HIP_TRY(hipMemcpy(buf.ptr, host.data(), host.size() * sizeof(float), hipMemcpyHostToDevice),
-            "uploading an upsampling matrix");
-    return AMPCOR_HIP_OK;
-}
+// host tables bound for device buffers, copied together once the plan's
+// buffers are laid out
+struct Uploads {
+    std::vector<std::pair<void *, std::vector<unsigned char>>> items;
+    template <typename T>
+    int put(DeviceBuffer<T> &buf, const std::vector<T> &host, Carver &cv, const char *what)
+    {
+        const hipError_t e = buf.carve(cv, host.size());
+        if (e != hipSuccess)
+            return fail(AMPCOR_HIP_ENOMEM, std::string(what) + ": " + hipGetErrorString(e));
+        const auto *b = reinterpret_cast<const unsigned char *>(host.data());
+        items.emplace_back(buf.ptr, std::vector<unsigned char>(b, b + host.size() * sizeof(T)));
+        return AMPCOR_HIP_OK;
+    }
+    int flush(hipStream_t s)
+    {
+        for (auto &it : items)
+            HIP_TRY(hipMemcpyAsync(it.first, it.second.data(), it.second.size(),
+                                   hipMemcpyHostToDevice, s),
+                    "uploading plan tables");
+        HIP_TRY(hipStreamSynchronize(s), "uploading plan tables");
+        return AMPCOR_HIP_OK;
+    }
+};
 
 // the upsampling matrix U[p][n] = kernel(N, rf, p, n), padded with zeros to
 // (rf N rounded up to 16) x (N rounded up to 16) for the matrix-core kernel
-int upload_upsampler(DeviceBuffer<float> &U, DeviceBuffer<float> &g, int N, int rf)
+int upload_upsampler(DeviceBuffer<float> &U, DeviceBuffer<float> &g, int N, int rf, Carver &cv,
+                     Uploads &up)
 {
     const int M = rf * N;
     const int Np = (N + 15) / 16 * 16, Mp = (M + 15) / 16 * 16;
     std::vector<float> u((size_t)Mp * Np, 0.0f);
     for (int p = 0; p < M; ++p)
         for (int n = 0; n < N; ++n) u[(size_t)p * Np + n] = (float)upsample_kernel(N, rf, p, n);
-    int rc = upload_matrix(U, u);
+    int rc = up.put(U, u, cv, "allocating an upsampling matrix");
     if (rc || rf != 2) return rc;
     // the half-sample shift is circulant: U[2a + 1][n] = g((a - n) mod N)
     std::vector<float> row(N);
     for (int d = 0; d < N; ++d) row[d] = (float)upsample_kernel(N, rf, 2 * d + 1, 0);
-    return upload_matrix(g, row);
+    return up.put(g, row, cv, "allocating an upsampling matrix");
 }
 
 // the half-sample circulant H[a][n] = g((a - n) mod N) as FP16 fragments of the
 // 16 x 16 x 32 MFMA: block (mt, kc), lane l, value e holds
 // H[16 mt + l % 16][32 kc + 8 (l / 16) + e], zero outside N x N
-int upload_fragments(DeviceBuffer<uint16_t> &hf, int N)
+int upload_fragments(DeviceBuffer<uint16_t> &hf, int N, Carver &cv, Uploads &up)
 {
     const int Np = (N + 31) / 32 * 32, MT = Np / 16, KC = Np / 32;
     std::vector<double> g(N);
@@ -122,10 +140,7 @@ int upload_fragments(DeviceBuffer<uint16_t> &hf, int N)
                         h[at] = __builtin_bit_cast(uint16_t, v);
                     }
                 }
-    hipError_t e = hf.alloc(h.size());
-    if (e == hipSuccess)
-        e = hipMemcpy(hf.ptr, h.data(), h.size() * sizeof(uint16_t), hipMemcpyHostToDevice);
-    return e == hipSuccess ? AMPCOR_HIP_OK : hip_fail(e, "uploading the upsampling fragments");
+    return up.put(hf, h, cv, "allocating the upsampling fragments");
 }
 
 // the zoom's FP16 screen: the Dirichlet matrix M[p][a] = D(p - zf a) as
@@ -138,7 +153,8 @@ int upload_fragments(DeviceBuffer<uint16_t> &hf, int N)
 // subnormals (absolute 2^-25 per rounding) add 2^-25 (Lambda^2 + 3 KP Lambda).
 // After the first product alone: Lambda max|g| (2 2^-11 + KP 2^-24) and
 // 2^-25 (Lambda + KP).  All carry a 1.25 margin for the second-order terms.
-int upload_zoom_screen(ampcor_hip_plan *p, const std::vector<double> &table /* d + Z */)
+int upload_zoom_screen(ampcor_hip_plan *p, const std::vector<double> &table /* d + Z */,
+                       Carver &cv, Uploads &up)
 {
     const int R = p->R, zf = p->zf, Z = p->Z;
     const int ZT = (Z + 15) / 16, KC = (R + 31) / 32, KP = 32 * KC;
@@ -169,14 +185,12 @@ int upload_zoom_screen(ampcor_hip_plan *p, const std::vector<double> &table /* d
     b.hcoef = (float)(1.25 * lambda * (2.0 * std::ldexp(1.0, -11) + KP * std::ldexp(1.0, -24)));
     b.habs = (float)(1.25 * std::ldexp(1.0, -25) * (lambda + KP));
     b.lambda = (float)(lambda * (1.0 + std::ldexp(1.0, -20)));
-    hipError_t e = p->zfrag.alloc(h.size());
-    if (e == hipSuccess)
-        e = hipMemcpy(p->zfrag.ptr, h.data(), h.size() * sizeof(uint16_t), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = p->zfragp.alloc(hp.size());
-    if (e == hipSuccess)
-        e = hipMemcpy(p->zfragp.ptr, hp.data(), hp.size() * sizeof(uint16_t), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = p->zstats.alloc(3);
-    return e == hipSuccess ? AMPCOR_HIP_OK : hip_fail(e, "uploading the zoom screen");
+    if (int rc = up.put(p->zfrag, h, cv, "allocating the zoom screen")) return rc;
+    if (int rc = up.put(p->zfragp, hp, cv, "allocating the zoom screen")) return rc;
+    const hipError_t e = p->zstats.carve(cv, 3);
+    return e == hipSuccess ? AMPCOR_HIP_OK
+                           : fail(AMPCOR_HIP_ENOMEM, std::string("allocating the zoom screen: ") +
+                                                        hipGetErrorString(e));
 }
 
 int register_tile(ampcor_hip_plan *plan, const ampcor_hip_raster *raster, size_t idx,
@@ -220,6 +234,185 @@ int register_tile(ampcor_hip_plan *plan, const ampcor_hip_raster *raster, size_t
 }
 
 }  // namespace
+
+namespace ampcor_hip {
+
+int plan_configure(ampcor_hip_plan *p, int device, size_t pairs, size_t ref_lines,
+                   size_t ref_samples, size_t tgt_lines, size_t tgt_samples, size_t refine_margin,
+                   size_t refine_factor, size_t zoom_factor, int precision)
+{
+    if (pairs == 0) return fail(AMPCOR_HIP_EINVAL, "a plan needs at least one pair");
+    if (pairs > (size_t)1 << 24) return fail(AMPCOR_HIP_EINVAL, "too many pairs for one plan");
+    // the kernels assume square tiles (Sequential.icc:56-60 makes the same assumption)
+    if (ref_lines != ref_samples || tgt_lines != tgt_samples)
+        return fail(AMPCOR_HIP_EINVAL, "chips and windows must be square");
+    const size_t C = ref_lines, W = tgt_lines;
+    if (C < 2 || C > 512) return fail(AMPCOR_HIP_EINVAL, "chip edge must be in [2, 512]");
+    if (W < C || (W - C) % 2) return fail(AMPCOR_HIP_EINVAL, "window - chip must be even and >= 0");
+    if (W > 512) return fail(AMPCOR_HIP_EINVAL, "window edge must be <= 512");
+    const size_t m = (W - C) / 2;
+    if (refine_margin > m)
+        return fail(AMPCOR_HIP_EINVAL, "refine margin exceeds the search margin");
+    if (refine_factor < 1 || refine_factor > 8) return fail(AMPCOR_HIP_EINVAL, "refine factor must be in [1, 8]");
+    if (zoom_factor < 1 || zoom_factor > 31) return fail(AMPCOR_HIP_EINVAL, "zoom factor must be in [1, 31]");
+    const size_t E = C + 2 * refine_margin, Cr = refine_factor * C, Wr = refine_factor * E;
+    const size_t R = Wr - Cr + 1;
+    if (Wr > 512) return fail(AMPCOR_HIP_EINVAL, "refined window edge must be <= 512");
+    if (R > 129) return fail(AMPCOR_HIP_EINVAL, "refined surface too large (2 rf rm + 1 > 129)");
+    if (!zoom_fits((int)R, (int)zoom_factor))
+        return fail(AMPCOR_HIP_EINVAL, "zoom factor too large for this refined surface (the zoom "
+                                       "of one pair must fit a workgroup's 160 KiB of LDS)");
+    if (precision < AMPCOR_HIP_FP32 || precision > AMPCOR_HIP_FP8)
+        return fail(AMPCOR_HIP_EINVAL, "unknown precision (FP32, BF16, FP16, BF16X3, FP16X3 or FP8)");
+    if (precision != AMPCOR_HIP_FP32 && (C % 16 || C * refine_factor > 256))
+        return fail(AMPCOR_HIP_EINVAL, "the mixed forms need a chip edge that is a multiple of 16 "
+                                       "and a refined chip edge <= 256");
+    // the FP8 coarse stage runs as one gather-quantize pass and the fused
+    // normalizer (its window sums come from the FP8 operand)
+    if (precision == AMPCOR_HIP_FP8 &&
+        (!gather_prep_fits((int)C, (int)W) || !normalize_fuses((int)W, (int)C)))
+        return fail(AMPCOR_HIP_EINVAL, "the FP8 form needs a chip edge <= 128 and (window - chip + 1) <= 144");
+    if (int rc = check_device(device)) return rc;
+
+    p->device = device;
+    p->precision = precision;
+    p->pairs = pairs;
+    p->C = (int)C;
+    p->W = (int)W;
+    p->m = (int)m;
+    p->P = (int)(W - C + 1);
+    p->rf = (int)refine_factor;
+    p->rm = (int)refine_margin;
+    p->E = (int)E;
+    p->Cr = (int)Cr;
+    p->Wr = (int)Wr;
+    p->R = (int)R;
+    p->zf = (int)zoom_factor;
+    p->Z = p->zf * p->R;
+    p->desc.assign(pairs, PairDesc{});
+    p->have_ref.assign(pairs, 0);
+    p->have_tgt.assign(pairs, 0);
+    {
+        // tuning overrides: AMPCOR_HIP_TILE_COARSE="bu", AMPCOR_HIP_TILE_REFINED="bu,nsplit"
+        int cbu = 0, rbu = 0, rns = 0;
+        if (const char *e = std::getenv("AMPCOR_HIP_TILE_COARSE")) std::sscanf(e, "%d", &cbu);
+        if (const char *e = std::getenv("AMPCOR_HIP_TILE_REFINED")) std::sscanf(e, "%d,%d", &rbu, &rns);
+        p->coarse_shape = choose_correlate_shape(p->C, p->P, true, cbu, 0);
+        p->refined_shape = choose_correlate_shape(p->Cr, p->R, false, rbu, rns);
+    }
+    p->mixed = precision != AMPCOR_HIP_FP32;
+    p->split = precision == AMPCOR_HIP_BF16X3 || precision == AMPCOR_HIP_FP16X3;
+    p->fp8 = precision == AMPCOR_HIP_FP8;
+    p->fp16 = precision == AMPCOR_HIP_FP16 || precision == AMPCOR_HIP_FP16X3 || p->fp8;
+    if (p->mixed) {
+        p->mixed_coarse = choose_mixed_shape(p->C, p->P);
+        p->mixed_refined = choose_mixed_shape(p->Cr, p->R, 0, p->split);
+    }
+    if (p->fp8) p->fp8_coarse = choose_fp8_shape(p->C, p->P);
+    return AMPCOR_HIP_OK;
+}
+
+int plan_build(ampcor_hip_plan *p, Carver &cv, hipStream_t s, bool upload)
+{
+    const size_t N = p->pairs;
+    const size_t C = p->C, W = p->W, Cr = p->Cr, Wr = p->Wr, R = p->R;
+    Uploads up;
+    struct {
+        hipError_t e;
+    } r{hipSuccess};
+    auto A = [&](hipError_t x) {
+        if (r.e == hipSuccess) r.e = x;
+    };
+    A(p->d_desc.carve(cv, N));
+    A(p->amp_ref.carve(cv, N * C * C));
+    A(p->amp_tgt.carve(cv, N * W * W));
+    A(p->rT.carve(cv, N * C * C));
+    A(p->stats.carve(cv, N));
+    A(p->sums.carve(cv, N * p->P * p->P));
+    A(p->sums_r.carve(cv, N * R * R));
+    // partial buffers leave room for the autotuner's largest candidate splits
+    const size_t ns_c = std::max({p->coarse_shape.nsplit, p->mixed ? p->mixed_coarse.nsplit : 1,
+                                  p->mixed ? kTuneMaxSplitCoarse : 1,
+                                  p->fp8 ? p->fp8_coarse.nsplit : 1});
+    const size_t ns_r = std::max({p->refined_shape.nsplit, p->mixed ? p->mixed_refined.nsplit : 1,
+                                  kTuneMaxSplitRefined});
+    A(p->partial.carve(cv, N * ns_c * p->P * p->P));
+    A(p->gamma.carve(cv, N * p->P * p->P));
+    A(p->loc.carve(cv, N));
+    A(p->window.carve(cv, N));
+    A(p->peak.carve(cv, N));
+    A(p->snr.carve(cv, N));
+    A(p->amp_ref_r.carve(cv, N * Cr * Cr));
+    A(p->amp_tgt_r.carve(cv, N * Wr * Wr));
+    A(p->rT_r.carve(cv, N * Cr * Cr));
+    A(p->stats_r.carve(cv, N));
+    A(p->partial_r.carve(cv, N * ns_r * R * R));
+    if (p->mixed) {
+        // (the FP8 operands, one byte per value, fit the same buffers)
+        A(p->chip_q.carve(cv, N * C * p->mixed_coarse.cs));
+        A(p->tgt_q.carve(cv, N * W * W));
+        A(p->chip_q_r.carve(cv, N * Cr * p->mixed_refined.cs));
+        A(p->tgt_q_r.carve(cv, N * Wr * Wr));
+        if (p->split) {
+            A(p->chip_lo_r.carve(cv, N * Cr * p->mixed_refined.cs));
+            A(p->tgt_lo_r.carve(cv, N * Wr * Wr));
+        }
+        A(p->mstats.carve(cv, N));
+        A(p->mstats_r.carve(cv, N));
+    }
+    if (p->fp8) {
+        A(p->guard.carve(cv, 3));
+        A(p->near.carve(cv, N));
+    }
+    A(p->gamma_r.carve(cv, N * R * R));
+    A(p->offsets.carve(cv, 2 * N));
+    A(p->peak_r.carve(cv, N));
+    A(p->zoom_loc.carve(cv, N));
+    A(p->zthr.carve(cv, N));
+    A(p->zwin.carve(cv, 4 * N));
+    A(p->zflag.carve(cv, N));
+    if (r.e != hipSuccess) {
+        std::string msg = std::string("allocating plan buffers for ") + std::to_string(N) +
+                          " pairs: " + hipGetErrorString(r.e);
+        return fail(AMPCOR_HIP_ENOMEM, msg);
+    }
+    if (int rc = upload_upsampler(p->U_ref, p->g_ref, p->C, p->rf, cv, up)) return rc;
+    if (int rc = upload_upsampler(p->U_tgt, p->g_tgt, p->E, p->rf, cv, up)) return rc;
+    // the BF16/FP16/FP8 forms upsample tiles up to 96 samples (160 for real
+    // rasters) on FP16 matrix cores (the split forms keep FP32)
+    if (p->mixed && !p->split && p->rf == 2) {
+        if (upsample_uses_fragments(p->C))
+            if (int rc = upload_fragments(p->hfrag_ref, p->C, cv, up)) return rc;
+        if (upsample_uses_fragments(p->E))
+            if (int rc = upload_fragments(p->hfrag_tgt, p->E, cv, up)) return rc;
+    }
+    // larger tiles upsample in two banded passes through scratch, shared by
+    // the two roles (they run one after the other on the plan stream)
+    {
+        int nmax = 0;
+        if (upsample_needs_scratch(p->C, p->rf)) nmax = std::max(nmax, p->C);
+        if (upsample_needs_scratch(p->E, p->rf)) nmax = std::max(nmax, p->E);
+        if (nmax) {
+            const size_t nb = (nmax + 15) / 16;
+            hipError_t ea = p->up_T.carve(cv, N * (size_t)nmax * nmax);
+            if (ea == hipSuccess) ea = p->up_band.carve(cv, N * nb);
+            if (ea == hipSuccess) ea = p->up_rows.carve(cv, N * nb);
+            if (ea != hipSuccess)
+                return fail(AMPCOR_HIP_ENOMEM, std::string("allocating the upsampling scratch: ") +
+                                                   hipGetErrorString(ea));
+            p->up_scratch = UpsampleScratch{p->up_T.ptr, p->up_band.ptr, p->up_rows.ptr};
+        }
+    }
+    {
+        std::vector<double> d(2 * p->Z + 1);
+        for (int k = -p->Z; k <= p->Z; ++k) d[k + p->Z] = dirichlet(p->R, p->zf, k);
+        if (int rc = up.put(p->D, d, cv, "allocating the zoom kernel table")) return rc;
+        if (int rc = upload_zoom_screen(p, d, cv, up)) return rc;
+    }
+    return upload ? up.flush(s) : AMPCOR_HIP_OK;
+}
+
+}  // namespace ampcor_hip
 
 extern "C" {
 
@@ -324,77 +517,13 @@ int ampcor_hip_plan_create(int device, size_t pairs, size_t ref_lines, size_t re
 {
     if (!out) return fail(AMPCOR_HIP_EINVAL, "null output pointer");
     *out = nullptr;
-    if (pairs == 0) return fail(AMPCOR_HIP_EINVAL, "a plan needs at least one pair");
-    if (pairs > (size_t)1 << 24) return fail(AMPCOR_HIP_EINVAL, "too many pairs for one plan");
-    // the kernels assume square tiles (Sequential.icc:56-60 makes the same assumption)
-    if (ref_lines != ref_samples || tgt_lines != tgt_samples)
-        return fail(AMPCOR_HIP_EINVAL, "chips and windows must be square");
-    const size_t C = ref_lines, W = tgt_lines;
-    if (C < 2 || C > 512) return fail(AMPCOR_HIP_EINVAL, "chip edge must be in [2, 512]");
-    if (W < C || (W - C) % 2) return fail(AMPCOR_HIP_EINVAL, "window - chip must be even and >= 0");
-    if (W > 512) return fail(AMPCOR_HIP_EINVAL, "window edge must be <= 512");
-    const size_t m = (W - C) / 2;
-    if (refine_margin > m)
-        return fail(AMPCOR_HIP_EINVAL, "refine margin exceeds the search margin");
-    if (refine_factor < 1 || refine_factor > 8) return fail(AMPCOR_HIP_EINVAL, "refine factor must be in [1, 8]");
-    if (zoom_factor < 1 || zoom_factor > 31) return fail(AMPCOR_HIP_EINVAL, "zoom factor must be in [1, 31]");
-    const size_t E = C + 2 * refine_margin, Cr = refine_factor * C, Wr = refine_factor * E;
-    const size_t R = Wr - Cr + 1;
-    if (Wr > 512) return fail(AMPCOR_HIP_EINVAL, "refined window edge must be <= 512");
-    if (R > 129) return fail(AMPCOR_HIP_EINVAL, "refined surface too large (2 rf rm + 1 > 129)");
-    if (!zoom_fits((int)R, (int)zoom_factor))
-        return fail(AMPCOR_HIP_EINVAL, "zoom factor too large for this refined surface (the zoom "
-                                       "of one pair must fit a workgroup's 160 KiB of LDS)");
-    if (precision < AMPCOR_HIP_FP32 || precision > AMPCOR_HIP_FP8)
-        return fail(AMPCOR_HIP_EINVAL, "unknown precision (FP32, BF16, FP16, BF16X3, FP16X3 or FP8)");
-    if (precision != AMPCOR_HIP_FP32 && (C % 16 || C * refine_factor > 256))
-        return fail(AMPCOR_HIP_EINVAL, "the mixed forms need a chip edge that is a multiple of 16 "
-                                       "and a refined chip edge <= 256");
-    // the FP8 coarse stage runs as one gather-quantize pass and the fused
-    // normalizer (its window sums come from the FP8 operand)
-    if (precision == AMPCOR_HIP_FP8 &&
-        (!gather_prep_fits((int)C, (int)W) || !normalize_fuses((int)W, (int)C)))
-        return fail(AMPCOR_HIP_EINVAL, "the FP8 form needs a chip edge <= 128 and (window - chip + 1) <= 144");
-    if (int rc = check_device(device)) return rc;
-
     auto *p = new (std::nothrow) ampcor_hip_plan();
     if (!p) return fail(AMPCOR_HIP_ENOMEM, "plan");
-    p->device = device;
-    p->precision = precision;
-    p->pairs = pairs;
-    p->C = (int)C;
-    p->W = (int)W;
-    p->m = (int)m;
-    p->P = (int)(W - C + 1);
-    p->rf = (int)refine_factor;
-    p->rm = (int)refine_margin;
-    p->E = (int)E;
-    p->Cr = (int)Cr;
-    p->Wr = (int)Wr;
-    p->R = (int)R;
-    p->zf = (int)zoom_factor;
-    p->Z = p->zf * p->R;
-    p->desc.assign(pairs, PairDesc{});
-    p->have_ref.assign(pairs, 0);
-    p->have_tgt.assign(pairs, 0);
-    {
-        // tuning overrides: AMPCOR_HIP_TILE_COARSE="bu", AMPCOR_HIP_TILE_REFINED="bu,nsplit"
-        int cbu = 0, rbu = 0, rns = 0;
-        if (const char *e = std::getenv("AMPCOR_HIP_TILE_COARSE")) std::sscanf(e, "%d", &cbu);
-        if (const char *e = std::getenv("AMPCOR_HIP_TILE_REFINED")) std::sscanf(e, "%d,%d", &rbu, &rns);
-        p->coarse_shape = choose_correlate_shape(p->C, p->P, true, cbu, 0);
-        p->refined_shape = choose_correlate_shape(p->Cr, p->R, false, rbu, rns);
+    if (int rc = plan_configure(p, device, pairs, ref_lines, ref_samples, tgt_lines, tgt_samples,
+                                refine_margin, refine_factor, zoom_factor, precision)) {
+        delete p;
+        return rc;
     }
-    p->mixed = precision != AMPCOR_HIP_FP32;
-    p->split = precision == AMPCOR_HIP_BF16X3 || precision == AMPCOR_HIP_FP16X3;
-    p->fp8 = precision == AMPCOR_HIP_FP8;
-    p->fp16 = precision == AMPCOR_HIP_FP16 || precision == AMPCOR_HIP_FP16X3 || p->fp8;
-    if (p->mixed) {
-        p->mixed_coarse = choose_mixed_shape(p->C, p->P);
-        p->mixed_refined = choose_mixed_shape(p->Cr, p->R, 0, p->split);
-    }
-    if (p->fp8) p->fp8_coarse = choose_fp8_shape(p->C, p->P);
-
     DeviceGuard guard(device);
     auto cleanup = [&](int rc) {
         p->release();
@@ -403,101 +532,8 @@ int ampcor_hip_plan_create(int device, size_t pairs, size_t ref_lines, size_t re
     };
     hipError_t e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return cleanup(hip_fail(e, "creating the plan stream"));
-
-    const size_t N = pairs;
-    struct {
-        hipError_t e;
-    } r{hipSuccess};
-    auto A = [&](hipError_t x) {
-        if (r.e == hipSuccess) r.e = x;
-    };
-    A(p->d_desc.alloc(N));
-    A(p->amp_ref.alloc(N * C * C));
-    A(p->amp_tgt.alloc(N * W * W));
-    A(p->rT.alloc(N * C * C));
-    A(p->stats.alloc(N));
-    A(p->sums.alloc(N * p->P * p->P));
-    A(p->sums_r.alloc(N * R * R));
-    // partial buffers leave room for the autotuner's largest candidate splits
-    const size_t ns_c = std::max({p->coarse_shape.nsplit, p->mixed ? p->mixed_coarse.nsplit : 1,
-                                  p->mixed ? kTuneMaxSplitCoarse : 1,
-                                  p->fp8 ? p->fp8_coarse.nsplit : 1});
-    const size_t ns_r = std::max({p->refined_shape.nsplit, p->mixed ? p->mixed_refined.nsplit : 1,
-                                  kTuneMaxSplitRefined});
-    A(p->partial.alloc(N * ns_c * p->P * p->P));
-    A(p->gamma.alloc(N * p->P * p->P));
-    A(p->loc.alloc(N));
-    A(p->window.alloc(N));
-    A(p->peak.alloc(N));
-    A(p->snr.alloc(N));
-    A(p->amp_ref_r.alloc(N * Cr * Cr));
-    A(p->amp_tgt_r.alloc(N * Wr * Wr));
-    A(p->rT_r.alloc(N * Cr * Cr));
-    A(p->stats_r.alloc(N));
-    A(p->partial_r.alloc(N * ns_r * R * R));
-    if (p->mixed) {
-        // (the FP8 operands, one byte per value, fit the same buffers)
-        A(p->chip_q.alloc(N * C * p->mixed_coarse.cs));
-        A(p->tgt_q.alloc(N * W * W));
-        A(p->chip_q_r.alloc(N * Cr * p->mixed_refined.cs));
-        A(p->tgt_q_r.alloc(N * Wr * Wr));
-        if (p->split) {
-            A(p->chip_lo_r.alloc(N * Cr * p->mixed_refined.cs));
-            A(p->tgt_lo_r.alloc(N * Wr * Wr));
-        }
-        A(p->mstats.alloc(N));
-        A(p->mstats_r.alloc(N));
-    }
-    if (p->fp8) {
-        A(p->guard.alloc(3));
-        A(p->near.alloc(N));
-    }
-    A(p->gamma_r.alloc(N * R * R));
-    A(p->offsets.alloc(2 * N));
-    A(p->peak_r.alloc(N));
-    A(p->zoom_loc.alloc(N));
-    A(p->zthr.alloc(N));
-    A(p->zwin.alloc(4 * N));
-    A(p->zflag.alloc(N));
-    if (r.e != hipSuccess) {
-        std::string msg = std::string("allocating plan buffers for ") + std::to_string(N) +
-                          " pairs: " + hipGetErrorString(r.e);
-        return cleanup(fail(AMPCOR_HIP_ENOMEM, msg));
-    }
-    if (int rc = upload_upsampler(p->U_ref, p->g_ref, p->C, p->rf)) return cleanup(rc);
-    if (int rc = upload_upsampler(p->U_tgt, p->g_tgt, p->E, p->rf)) return cleanup(rc);
-    // the BF16/FP16/FP8 forms upsample tiles up to 96 samples (160 for real
-    // rasters) on FP16 matrix cores (the split forms keep FP32)
-    if (p->mixed && !p->split && p->rf == 2) {
-        if (upsample_uses_fragments(p->C))
-            if (int rc = upload_fragments(p->hfrag_ref, p->C)) return cleanup(rc);
-        if (upsample_uses_fragments(p->E))
-            if (int rc = upload_fragments(p->hfrag_tgt, p->E)) return cleanup(rc);
-    }
-    // larger tiles upsample in two banded passes through scratch, shared by
-    // the two roles (they run one after the other on the plan stream)
-    {
-        int nmax = 0;
-        if (upsample_needs_scratch(p->C, p->rf)) nmax = std::max(nmax, p->C);
-        if (upsample_needs_scratch(p->E, p->rf)) nmax = std::max(nmax, p->E);
-        if (nmax) {
-            const size_t nb = (nmax + 15) / 16;
-            hipError_t ea = p->up_T.alloc(N * (size_t)nmax * nmax);
-            if (ea == hipSuccess) ea = p->up_band.alloc(N * nb);
-            if (ea == hipSuccess) ea = p->up_rows.alloc(N * nb);
-            if (ea != hipSuccess) return cleanup(hip_fail(ea, "allocating the upsampling scratch"));
-            p->up_scratch = UpsampleScratch{p->up_T.ptr, p->up_band.ptr, p->up_rows.ptr};
-        }
-    }
-    {
-        std::vector<double> d(2 * p->Z + 1);
-        for (int k = -p->Z; k <= p->Z; ++k) d[k + p->Z] = dirichlet(p->R, p->zf, k);
-        e = p->D.alloc(d.size());
-        if (e == hipSuccess)
-            e = hipMemcpy(p->D.ptr, d.data(), d.size() * sizeof(double), hipMemcpyHostToDevice);
-        if (e != hipSuccess) return cleanup(hip_fail(e, "uploading the zoom kernel table"));
-        if (int rc = upload_zoom_screen(p, d)) return cleanup(rc);
-    }
+    MallocCarver cv;
+    if (int rc = plan_build(p, cv, p->stream, true)) return cleanup(rc);
     for (auto &ev : p->ev) {
         e = hipEventCreate(&ev);
         if (e != hipSuccess) return cleanup(hip_fail(e, "creating timing events"));

@@ -27,6 +27,33 @@ int hip_fail(hipError_t e, const char *what);
         if (_e != hipSuccess) return hip_fail(_e, (what));   \
     } while (0)
 
+// where a plan's device buffers come from: hipMalloc (plans), or carved out of
+// a fixed region (the allocation-free entry point)
+struct Carver {
+    virtual ~Carver() = default;
+    virtual hipError_t get(void **ptr, size_t bytes) = 0;
+};
+
+struct MallocCarver : Carver {
+    hipError_t get(void **ptr, size_t bytes) override { return hipMalloc(ptr, bytes); }
+};
+
+// consecutive 256-byte-aligned pieces of [base, base + cap); a null base only
+// counts (used = the bytes a plan needs)
+struct ScratchCarver : Carver {
+    char *base;
+    size_t cap, used = 0;
+    ScratchCarver(void *b, size_t c) : base(static_cast<char *>(b)), cap(c) {}
+    hipError_t get(void **ptr, size_t bytes) override
+    {
+        used = (used + 255) & ~(size_t)255;
+        if (bytes > cap || used > cap - bytes) return hipErrorOutOfMemory;
+        *ptr = base ? base + used : nullptr;
+        used += bytes;
+        return hipSuccess;
+    }
+};
+
 template <typename T>
 struct DeviceBuffer {
     T *ptr = nullptr;
@@ -35,6 +62,11 @@ struct DeviceBuffer {
     {
         count = n;
         return hipMalloc((void **)&ptr, n * sizeof(T) > 0 ? n * sizeof(T) : 1);
+    }
+    hipError_t carve(Carver &c, size_t n)
+    {
+        count = n;
+        return c.get((void **)&ptr, n * sizeof(T) > 0 ? n * sizeof(T) : 1);
     }
     void release()
     {
@@ -176,6 +208,15 @@ namespace ampcor_hip {
 // enqueue every stage for the first n registered pairs on the plan's stream;
 // uploads the descriptors first when they changed; no synchronization
 int plan_enqueue(ampcor_hip_plan *plan, int n);
+// validates a geometry and fills a plan's dimensions and tilings (no device
+// memory); the arguments are those of ampcor_hip_plan_create
+int plan_configure(ampcor_hip_plan *p, int device, size_t pairs, size_t ref_lines,
+                   size_t ref_samples, size_t tgt_lines, size_t tgt_samples, size_t refine_margin,
+                   size_t refine_factor, size_t zoom_factor, int precision);
+// takes every device buffer of a configured plan from `cv` and uploads its
+// tables on stream s (synchronizing it); with upload false only carves (a
+// ScratchCarver with a null base then counts the bytes)
+int plan_build(ampcor_hip_plan *p, Carver &cv, hipStream_t s, bool upload);
 // device bytes a plan of this geometry allocates per pair (for admission)
 size_t plan_bytes_per_pair(int C, int W, int rf, int rm, int precision);
 }  // namespace ampcor_hip

@@ -174,6 +174,33 @@ int ampcor_hip_plan_autotune(ampcor_hip_plan *plan, int reps);
 int ampcor_hip_stage_times(ampcor_hip_plan *plan, int capacity, int *count, float *ms,
                            const char **names);
 
+/* -- allocation-free adjust ----------------------------------------------------
+ *
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): one whole adjust -- the plan of ampcor_hip_plan_create,
+ * the pairs of ampcor_hip_add_pairs and ampcor_hip_adjust -- that calls no
+ * device allocator.  Its intermediate buffers and tables come from a fixed
+ * scratch array of AMPCOR_HIP_NOALLOC_SCRATCH_BYTES in device memory, which
+ * every device the library runs on holds until the process ends; a batch that
+ * needs more is refused with AMPCOR_HIP_EINVAL (the message says how many
+ * pairs of the geometry fit).  Calls on one device share that array: calls
+ * that overlap on one device race.  Rasters (lines x samples, row-major,
+ * dtype as for ampcor_hip_raster_create) and the four outputs (offsets
+ * [pairs][2], coarse peak, refined peak, SNR [pairs]) are device pointers the
+ * caller owns; the corners are host arrays as for ampcor_hip_add_pairs.  The
+ * work runs on `stream` (a hipStream_t; NULL for the default stream), which
+ * the call synchronizes before it returns.  Tilings are the plan's defaults
+ * (no autotuning), so the results equal an untuned plan's bit for bit. */
+#define AMPCOR_HIP_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int ampcor_hip_adjust_noalloc(int device, void *stream, int dtype, const void *reference,
+                              size_t ref_lines, size_t ref_samples, const void *target,
+                              size_t tgt_lines, size_t tgt_samples, size_t pairs,
+                              const int64_t *ref_begin, const int64_t *tgt_begin, size_t chip,
+                              size_t window, size_t refine_margin, size_t refine_factor,
+                              size_t zoom_factor, int precision, float *offsets,
+                              float *coarse_peak, float *refined_peak, float *snr);
+
 /* -- serving: continuous batching of offset-tracking jobs --------------------
  *
  * No counterpart in the reference, which correlates one plan at a time

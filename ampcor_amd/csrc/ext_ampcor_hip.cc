@@ -15,7 +15,8 @@
 // stage_gamma_refined, stage_locations, stage_zoom, zoom_stats, fp8_guard,
 // set_timing, stage_times.
 //
-// Unlike the reference, adjust releases the GIL, and failures raise
+// Unlike the reference, adjust releases the GIL (a worker busy in another
+// thread raises RuntimeError rather than racing it), and failures raise
 // ampcor_hip.Error instead of escaping as C++ exceptions.
 #define PY_SSIZE_T_CLEAN
 #include <Python.h>
@@ -45,6 +46,7 @@ struct Worker {
     ampcor_hip_plan *plan;
     PyObject *rasters;  // list: keeps every raster the plan points into alive
     size_t pairs;
+    bool busy;          // a call released the GIL inside the plan (adjust, autotune)
 };
 
 PyObject *raise(int status)
@@ -69,13 +71,20 @@ void free_raster(PyObject *capsule)
     ampcor_hip_raster_destroy(static_cast<ampcor_hip_raster *>(PyCapsule_GetPointer(capsule, kRasterCapsule)));
 }
 
+// the worker of a capsule; a worker that another thread is using (inside
+// adjust or autotune, with the GIL released) is refused rather than raced
 Worker *get_worker(PyObject *o)
 {
     if (!PyCapsule_IsValid(o, kWorkerCapsule)) {
         PyErr_SetString(PyExc_TypeError, "invalid HIP Sequential worker capsule");
         return nullptr;
     }
-    return static_cast<Worker *>(PyCapsule_GetPointer(o, kWorkerCapsule));
+    auto *w = static_cast<Worker *>(PyCapsule_GetPointer(o, kWorkerCapsule));
+    if (w->busy) {
+        PyErr_SetString(PyExc_RuntimeError, "the worker is in use by another thread");
+        return nullptr;
+    }
+    return w;
 }
 
 ampcor_hip_raster *get_raster(PyObject *o)
@@ -227,7 +236,7 @@ PyObject *sequential(PyObject *, PyObject *args, PyObject *kwds)
     int rc = ampcor_hip_plan_create(device, pairs, refLines, refSamples, tgtLines, tgtSamples,
                                     refineMargin, refineFactor, zoomFactor, precision, &plan);
     if (rc) return raise(rc);
-    auto *w = new Worker{plan, PyList_New(0), pairs};
+    auto *w = new Worker{plan, PyList_New(0), pairs, false};
     return PyCapsule_New(w, kWorkerCapsule, free_worker);
 }
 
@@ -309,9 +318,11 @@ PyObject *adjust(PyObject *, PyObject *args, PyObject *kwds)
     if (!w) return nullptr;
     std::vector<float> offsets(want ? 2 * w->pairs : 0);
     int rc;
+    w->busy = true;
     Py_BEGIN_ALLOW_THREADS
     rc = ampcor_hip_adjust(w->plan, want ? offsets.data() : nullptr);
     Py_END_ALLOW_THREADS
+    w->busy = false;
     if (rc) return raise(rc);
     if (!want) Py_RETURN_NONE;
     PyObject *results = PyTuple_New(w->pairs);
@@ -445,9 +456,11 @@ PyObject *stage_zoom(PyObject *, PyObject *args)
                      4 * w->pairs * (size_t)R * R);
         return nullptr;
     }
+    w->busy = true;
     Py_BEGIN_ALLOW_THREADS
     rc = ampcor_hip_stage_zoom(w->plan, static_cast<const float *>(view.buf));
     Py_END_ALLOW_THREADS
+    w->busy = false;
     PyBuffer_Release(&view);
     if (rc) return raise(rc);
     Py_RETURN_NONE;
@@ -533,6 +546,16 @@ void free_server(PyObject *capsule)
     delete s;
 }
 
+// drop the references a job held on its rasters
+void release_held(Server *s, uint64_t id)
+{
+    auto it = s->held.find(id);
+    if (it == s->held.end()) return;
+    Py_XDECREF(it->second.first);
+    Py_XDECREF(it->second.second);
+    s->held.erase(it);
+}
+
 Server *get_server(PyObject *o)
 {
     if (!PyCapsule_IsValid(o, kServerCapsule)) {
@@ -603,16 +626,17 @@ PyObject *wait(PyObject *, PyObject *args, PyObject *kwds)
 {
     PyObject *pyServer;
     unsigned long long id;
-    unsigned long pairs;
     int timeout_ms = -1;
-    static const char *names[] = {"server", "job", "pairs", "timeout_ms", nullptr};
-    if (!PyArg_ParseTupleAndKeywords(args, kwds, "O!Kk|i:wait", const_cast<char **>(names),
-                                     &PyCapsule_Type, &pyServer, &id, &pairs, &timeout_ms))
+    static const char *names[] = {"server", "job", "timeout_ms", nullptr};
+    if (!PyArg_ParseTupleAndKeywords(args, kwds, "O!K|i:wait", const_cast<char **>(names),
+                                     &PyCapsule_Type, &pyServer, &id, &timeout_ms))
         return nullptr;
     Server *s = get_server(pyServer);
     if (!s) return nullptr;
-    PyObject *off = bytes_of(8 * (size_t)pairs), *cp = bytes_of(4 * (size_t)pairs),
-             *rp = bytes_of(4 * (size_t)pairs), *sn = bytes_of(4 * (size_t)pairs);
+    size_t pairs = 0;
+    if (int rc = ampcor_hip_server_job_pairs(s->server, id, &pairs)) return raise(rc);
+    PyObject *off = bytes_of(8 * pairs), *cp = bytes_of(4 * pairs), *rp = bytes_of(4 * pairs),
+             *sn = bytes_of(4 * pairs);
     if (!off || !cp || !rp || !sn) {
         Py_XDECREF(off); Py_XDECREF(cp); Py_XDECREF(rp); Py_XDECREF(sn);
         return nullptr;
@@ -620,7 +644,7 @@ PyObject *wait(PyObject *, PyObject *args, PyObject *kwds)
     double latency = 0.0;
     int rc;
     Py_BEGIN_ALLOW_THREADS
-    rc = ampcor_hip_server_wait(s->server, id, timeout_ms,
+    rc = ampcor_hip_server_wait(s->server, id, timeout_ms, pairs,
                                 reinterpret_cast<float *>(PyBytes_AS_STRING(off)),
                                 reinterpret_cast<float *>(PyBytes_AS_STRING(cp)),
                                 reinterpret_cast<float *>(PyBytes_AS_STRING(rp)),
@@ -628,14 +652,15 @@ PyObject *wait(PyObject *, PyObject *args, PyObject *kwds)
     Py_END_ALLOW_THREADS
     if (rc) {
         Py_DECREF(off); Py_DECREF(cp); Py_DECREF(rp); Py_DECREF(sn);
-        return raise(rc);
+        // a job that failed alone is forgotten by the server: its rasters may go
+        const std::string text = ampcor_hip_last_error();
+        size_t left = 0;
+        if (ampcor_hip_server_job_pairs(s->server, id, &left) != AMPCOR_HIP_OK) release_held(s, id);
+        PyErr_Format(rc == AMPCOR_HIP_EINVAL ? PyExc_ValueError : g_error, "%s (status %d)",
+                     text.c_str(), rc);
+        return nullptr;
     }
-    auto it = s->held.find(id);
-    if (it != s->held.end()) {
-        Py_XDECREF(it->second.first);
-        Py_XDECREF(it->second.second);
-        s->held.erase(it);
-    }
+    release_held(s, id);
     return Py_BuildValue("{sNsNsNsNsd}", "offsets", off, "coarse_peak", cp, "refined_peak", rp,
                          "snr", sn, "latency_ms", latency);
 }
@@ -661,9 +686,11 @@ PyObject *autotune(PyObject *, PyObject *args)
     Worker *w = get_worker(pyWorker);
     if (!w) return nullptr;
     int rc;
+    w->busy = true;
     Py_BEGIN_ALLOW_THREADS
     rc = ampcor_hip_plan_autotune(w->plan, reps);
     Py_END_ALLOW_THREADS
+    w->busy = false;
     if (rc) return raise(rc);
     Py_RETURN_NONE;
 }

@@ -49,6 +49,9 @@ struct Job {
     std::vector<float> offsets, cpeak, rpeak, snr;
     Clock::time_point submitted, finished;
     bool done = false;
+    bool waited = false;  // a thread waits for it (one at a time)
+    int error = AMPCOR_HIP_OK;  // its lane could not be built: the job failed alone
+    std::string error_text;
 };
 
 struct Segment {
@@ -168,10 +171,15 @@ void admit(ampcor_hip_server *sv)
             }
             if (sv->used + need > sv->budget && !sv->lanes.empty()) return;  // wait for room
             std::unique_ptr<Lane> lane;
-            if (make_lane(sv, k, lane)) {
-                sv->error = AMPCOR_HIP_ENOMEM;
-                sv->error_text = ampcor_hip_last_error();
-                return;
+            if (int rc = make_lane(sv, k, lane)) {
+                // this job (and only it) fails; the server goes on with the others
+                job->error = rc;
+                job->error_text = ampcor_hip_last_error();
+                job->done = true;
+                job->finished = Clock::now();
+                sv->admission.pop_front();
+                sv->finished.notify_all();
+                continue;
             }
             sv->used += lane->bytes;
             it = sv->lanes.emplace(k, std::move(lane)).first;
@@ -301,6 +309,7 @@ int tune(ampcor_hip_server *sv, Lane &lane, Slot &s)
         o.plan->refined_shape = p->refined_shape;
         o.plan->mixed_coarse = p->mixed_coarse;
         o.plan->mixed_refined = p->mixed_refined;
+        o.plan->fp8_coarse = p->fp8_coarse;
         for (auto &g : o.graphs) (void)hipGraphExecDestroy(g.second);
         o.graphs.clear();
     }
@@ -400,15 +409,13 @@ int ampcor_hip_server_submit(ampcor_hip_server *sv, const ampcor_hip_raster *ref
     if (reference->dtype != target->dtype) return fail(AMPCOR_HIP_EINVAL, "rasters differ in pixel type");
     if (reference->device != sv->device || target->device != sv->device)
         return fail(AMPCOR_HIP_EINVAL, "rasters live on another device than the server");
-    // the same geometry checks a plan applies
-    if (window < chip || (window - chip) % 2 || refine_margin > (window - chip) / 2 || chip < 2 ||
-        window > 512 || refine_factor < 1 || refine_factor > 8 || zoom_factor < 1 || zoom_factor > 31 ||
-        refine_factor * (chip + 2 * refine_margin) > 512 || 2 * refine_factor * refine_margin + 1 > 129)
-        return fail(AMPCOR_HIP_EINVAL, "unsupported job geometry");
-    if (precision < AMPCOR_HIP_FP32 || precision > AMPCOR_HIP_FP8)
-        return fail(AMPCOR_HIP_EINVAL, "unknown precision");
-    if (precision != AMPCOR_HIP_FP32 && (chip % 16 || chip * refine_factor > 256))
-        return fail(AMPCOR_HIP_EINVAL, "the mixed forms need chip % 16 == 0 and rf * chip <= 256");
+    // the checks a plan of this geometry applies (its lane's plans are made later)
+    {
+        ampcor_hip_plan probe;
+        if (int rc = plan_configure(&probe, sv->device, 1, chip, chip, window, window, refine_margin,
+                                    refine_factor, zoom_factor, precision))
+            return rc;
+    }
     auto job = std::make_unique<Job>();
     job->key = Key{(int)chip, (int)window, (int)refine_margin, (int)refine_factor, (int)zoom_factor,
                    precision, reference->dtype};
@@ -445,25 +452,48 @@ int ampcor_hip_server_submit(ampcor_hip_server *sv, const ampcor_hip_raster *ref
     return AMPCOR_HIP_OK;
 }
 
-int ampcor_hip_server_wait(ampcor_hip_server *sv, uint64_t job_id, int timeout_ms, float *offsets,
-                           float *coarse_peak, float *refined_peak, float *snr, double *latency_ms)
+int ampcor_hip_server_job_pairs(ampcor_hip_server *sv, uint64_t job_id, size_t *pairs)
+{
+    if (!sv || !pairs) return fail(AMPCOR_HIP_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lock(sv->mu);
+    auto it = sv->jobs.find(job_id);
+    if (it == sv->jobs.end()) return fail(AMPCOR_HIP_EINVAL, "unknown job");
+    *pairs = it->second->desc.size();
+    return AMPCOR_HIP_OK;
+}
+
+int ampcor_hip_server_wait(ampcor_hip_server *sv, uint64_t job_id, int timeout_ms, size_t capacity,
+                           float *offsets, float *coarse_peak, float *refined_peak, float *snr,
+                           double *latency_ms)
 {
     if (!sv) return fail(AMPCOR_HIP_EINVAL, "null server");
     std::unique_lock<std::mutex> lock(sv->mu);
     auto it = sv->jobs.find(job_id);
     if (it == sv->jobs.end()) return fail(AMPCOR_HIP_EINVAL, "unknown job");
     Job *job = it->second.get();
-    auto ready = [&] { return job->done || sv->error != AMPCOR_HIP_OK; };
-    if (timeout_ms < 0) sv->finished.wait(lock, ready);
-    else if (!sv->finished.wait_for(lock, std::chrono::milliseconds(timeout_ms), ready))
-        return fail(AMPCOR_HIP_ESTATE, "job not finished yet");
-    if (!job->done) return fail(sv->error, sv->error_text);
     const size_t n = job->desc.size();
+    if ((offsets || coarse_peak || refined_peak || snr) && capacity < n)
+        return fail(AMPCOR_HIP_EINVAL, "the job has " + std::to_string(n) +
+                                           " pairs, the output buffers hold " +
+                                           std::to_string(capacity));
+    if (job->waited) return fail(AMPCOR_HIP_EINVAL, "another thread waits for this job");
+    job->waited = true;
+    auto ready = [&] { return job->done || sv->error != AMPCOR_HIP_OK; };
+    bool ok = true;
+    if (timeout_ms < 0) sv->finished.wait(lock, ready);
+    else ok = sv->finished.wait_for(lock, std::chrono::milliseconds(timeout_ms), ready);
+    job->waited = false;
+    if (!ok) return fail(AMPCOR_HIP_ESTATE, "job not finished yet");
+    if (!job->done) return fail(sv->error, sv->error_text);
+    if (job->error) {
+        const int rc = fail(job->error, job->error_text);
+        sv->jobs.erase(it);
+        return rc;
+    }
     if (offsets) std::copy(job->offsets.begin(), job->offsets.end(), offsets);
     if (coarse_peak) std::copy(job->cpeak.begin(), job->cpeak.end(), coarse_peak);
     if (refined_peak) std::copy(job->rpeak.begin(), job->rpeak.end(), refined_peak);
     if (snr) std::copy(job->snr.begin(), job->snr.end(), snr);
-    (void)n;
     if (latency_ms)
         *latency_ms = std::chrono::duration<double, std::milli>(job->finished - job->submitted).count();
     sv->jobs.erase(it);

@@ -89,7 +89,7 @@ def warm(lib, server, rasters, precision, pool_jobs):
     """One small job per geometry: creates the lanes, tunes them, captures graphs."""
     ids = [(submit(lib, server, rasters, j, precision), j.pairs) for j in pool_jobs]
     for jid, n in ids:
-        lib.wait(server, jid, n)
+        lib.wait(server, jid)
 
 
 def run_stream(lib, server, rasters, jobs, precision):
@@ -104,7 +104,7 @@ def run_stream(lib, server, rasters, jobs, precision):
         for k, j in enumerate(jobs):
             while ids[k] is None:
                 time.sleep(1e-4)
-            out = lib.wait(server, ids[k], j.pairs)
+            out = lib.wait(server, ids[k])
             lat[k] = out["latency_ms"]
             results[k] = out
 

@@ -222,12 +222,17 @@ int ampcor_hip_server_submit(ampcor_hip_server *server, const ampcor_hip_raster 
                              const int64_t *ref_begin, const int64_t *tgt_begin, size_t chip,
                              size_t window, size_t refine_margin, size_t refine_factor,
                              size_t zoom_factor, int precision, uint64_t *job);
+/* the number of pairs of a submitted job (what its output buffers must hold) */
+int ampcor_hip_server_job_pairs(ampcor_hip_server *server, uint64_t job, size_t *pairs);
 /* wait up to timeout_ms (< 0: forever) for a job; copies its products (any
- * pointer may be NULL) and its submit-to-finish latency, then forgets it.
- * AMPCOR_HIP_ESTATE if it is still running at the timeout. */
+ * pointer may be NULL; each holds `capacity` pairs, at least the job's) and
+ * its submit-to-finish latency, then forgets it.  AMPCOR_HIP_ESTATE if it is
+ * still running at the timeout; AMPCOR_HIP_EINVAL if the buffers are too
+ * small or another thread waits for the same job.  A job whose lane could not
+ * be built fails alone, with that error. */
 int ampcor_hip_server_wait(ampcor_hip_server *server, uint64_t job, int timeout_ms,
-                           float *offsets, float *coarse_peak, float *refined_peak, float *snr,
-                           double *latency_ms);
+                           size_t capacity, float *offsets, float *coarse_peak,
+                           float *refined_peak, float *snr, double *latency_ms);
 int ampcor_hip_server_stats(ampcor_hip_server *server, uint64_t *pairs, uint64_t *batches,
                             uint64_t *graph_launches, uint64_t *lanes, uint64_t *tunes);
 /* stops the scheduler after the batches in flight finish */

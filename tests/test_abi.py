@@ -69,6 +69,18 @@ def test_c_abi_status_codes_without_a_device():
     assert L.ampcor_hip_device_count(ctypes.byref(n)) == 0 and n.value >= 0
 
 
+def test_geometry_the_fp8_form_cannot_run_is_refused_without_a_device():
+    # the predicate a server's submit shares with plan creation: FP8 needs
+    # (window - chip + 1) <= 144, which chip 32 / window 320 breaks
+    L = ctypes.CDLL(build.library_path())
+    L.ampcor_hip_last_error.restype = ctypes.c_char_p
+    sz, out = ctypes.c_size_t, ctypes.c_void_p()
+    rc = L.ampcor_hip_plan_create(0, sz(4), sz(32), sz(32), sz(320), sz(320), sz(8), sz(2), sz(8),
+                                  5, ctypes.byref(out))
+    assert rc == 1 and b"FP8" in L.ampcor_hip_last_error()
+
+
+
 def test_invalid_capsules_raise_type_error():
     lib = ext.require()
     with pytest.raises(TypeError):

@@ -12,7 +12,7 @@
 // raster(buffer, lines, samples[, dtype, device]), slc_pixelSize(),
 // slc_getitem(raster, offset); the CPU module's refine(worker); and the
 // batched and product accessors: addPairs, products, dims, stage_gamma,
-// stage_gamma_refined, stage_locations, stage_zoom, zoom_stats, fp8_guard,
+// stage_gamma_refined, stage_locations, stage_upsample, stage_zoom, zoom_stats, fp8_guard,
 // set_timing, stage_times.
 //
 // Unlike the reference, adjust releases the GIL (a worker busy in another
@@ -436,6 +436,64 @@ PyObject *stage_locations(PyObject *, PyObject *args)
     return Py_BuildValue("{sNsNsN}", "coarse", c, "window", wi, "zoomed", z);
 }
 
+// stage_upsample(worker, tiles, count, dtype="complex64") -> bytes: the target
+// upsampling of the worker's form on count tiles [count][E][E]; returns |y|
+// float32 [count][rf E][rf E]
+PyObject *stage_upsample(PyObject *, PyObject *args, PyObject *kwds)
+{
+    PyObject *pyWorker;
+    Py_buffer view;
+    unsigned long count;
+    const char *dtype_name = "complex64";
+    static const char *names[] = {"worker", "tiles", "count", "dtype", nullptr};
+    if (!PyArg_ParseTupleAndKeywords(args, kwds, "O!y*k|s:stage_upsample",
+                                     const_cast<char **>(names), &PyCapsule_Type, &pyWorker, &view,
+                                     &count, &dtype_name))
+        return nullptr;
+    int dtype = 0;
+    Worker *w = get_worker(pyWorker);
+    if (!w || parse_dtype(dtype_name, &dtype)) {
+        PyBuffer_Release(&view);
+        return nullptr;
+    }
+    int P, R, Z;
+    if (int rc = ampcor_hip_plan_dims(w->plan, &P, &R, &Z)) {
+        PyBuffer_Release(&view);
+        return raise(rc);
+    }
+    // E and rf from the plan's dimensions: R = 2 rf rm + 1, rf E = refined window
+    size_t E = 0, M = 0;
+    if (int rc = ampcor_hip_plan_tile_dims(w->plan, &E, &M)) {
+        PyBuffer_Release(&view);
+        return raise(rc);
+    }
+    const size_t px = dtype == AMPCOR_HIP_COMPLEX64 ? 8 : 4;
+    if ((size_t)view.len != count * E * E * px) {
+        PyBuffer_Release(&view);
+        PyErr_Format(PyExc_ValueError, "tiles hold %zd bytes, %lu tiles of %zu x %zu need %zu",
+                     view.len, count, E, E, count * E * E * px);
+        return nullptr;
+    }
+    PyObject *out = bytes_of(4 * count * M * M);
+    if (!out) {
+        PyBuffer_Release(&view);
+        return nullptr;
+    }
+    int rc;
+    w->busy = true;
+    Py_BEGIN_ALLOW_THREADS
+    rc = ampcor_hip_stage_upsample(w->plan, dtype, view.buf, count,
+                                   reinterpret_cast<float *>(PyBytes_AS_STRING(out)));
+    Py_END_ALLOW_THREADS
+    w->busy = false;
+    PyBuffer_Release(&view);
+    if (rc) {
+        Py_DECREF(out);
+        return raise(rc);
+    }
+    return out;
+}
+
 // stage_zoom(worker, surfaces): the zoom stage alone on float32 refined
 // surfaces [pairs][R][R]; products() and stage_locations() then read its results
 PyObject *stage_zoom(PyObject *, PyObject *args)
@@ -717,6 +775,8 @@ PyMethodDef methods[] = {
     {"stage_locations", stage_locations, METH_VARARGS, "coarse, window and zoomed peak indices"},
     {"fp8_guard", fp8_guard, METH_VARARGS, "FP8 peak guard counts of the last adjust"},
     {"stage_zoom", stage_zoom, METH_VARARGS, "run the zoom stage on given refined surfaces"},
+    {"stage_upsample", (PyCFunction)(void (*)(void))stage_upsample, METH_VARARGS | METH_KEYWORDS,
+     "run the target upsampling on given tiles"},
     {"zoom_stats", zoom_stats, METH_VARARGS, "the zoom screen's exact re-evaluations"},
     {"set_timing", set_timing, METH_VARARGS, "enable per-stage device timing"},
     {"stage_times", stage_times, METH_VARARGS, "per-stage device milliseconds of the last adjust"},

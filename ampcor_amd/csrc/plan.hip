@@ -572,6 +572,14 @@ int ampcor_hip_plan_dims(const ampcor_hip_plan *plan, int *coarse_dim, int *refi
     return AMPCOR_HIP_OK;
 }
 
+int ampcor_hip_plan_tile_dims(const ampcor_hip_plan *plan, size_t *expanded, size_t *refined_window)
+{
+    if (!plan) return fail(AMPCOR_HIP_EINVAL, "null plan");
+    if (expanded) *expanded = plan->E;
+    if (refined_window) *refined_window = plan->Wr;
+    return AMPCOR_HIP_OK;
+}
+
 int ampcor_hip_add_reference(ampcor_hip_plan *plan, const ampcor_hip_raster *raster, size_t idx,
                              size_t begin_line, size_t begin_sample, size_t end_line,
                              size_t end_sample)
@@ -870,6 +878,63 @@ int ampcor_hip_stage_gamma_refined(ampcor_hip_plan *plan, float *gamma)
     if (!plan->adjusted) return fail(AMPCOR_HIP_ESTATE, "stage requested before adjust");
     DeviceGuard guard(plan->device);
     return copy_out(plan, gamma, plan->gamma_r.ptr, plan->pairs * plan->R * plan->R * sizeof(float));
+}
+
+int ampcor_hip_stage_upsample(ampcor_hip_plan *plan, int dtype, const void *tiles, size_t count,
+                              float *amp)
+{
+    if (!plan || !tiles || !amp) return fail(AMPCOR_HIP_EINVAL, "null argument");
+    if (dtype != AMPCOR_HIP_COMPLEX64 && dtype != AMPCOR_HIP_FLOAT32)
+        return fail(AMPCOR_HIP_EINVAL, "unknown raster pixel type");
+    if (count == 0 || count > plan->pairs)
+        return fail(AMPCOR_HIP_EINVAL, "tile count must be in [1, pairs]");
+    DeviceGuard guard(plan->device);
+    hipStream_t s = plan->stream;
+    const int E = plan->E, M = plan->Wr;
+    const size_t px = dtype == AMPCOR_HIP_COMPLEX64 ? 8 : 4, row = count * E * px;
+    // the tiles side by side in one raster of E lines, each read as a target
+    // window at its own corner with the nudged window at (0, 0)
+    std::vector<unsigned char> host((size_t)E * row);
+    for (size_t k = 0; k < count; ++k)
+        for (int i = 0; i < E; ++i)
+            std::memcpy(host.data() + i * row + k * E * px,
+                        static_cast<const unsigned char *>(tiles) + (k * E + i) * E * px, E * px);
+    void *raster = nullptr;
+    HIP_TRY(hipMalloc(&raster, host.size()), "allocating the tile raster");
+    std::vector<PairDesc> desc(count);
+    for (size_t k = 0; k < count; ++k)
+        desc[k] = PairDesc{raster, raster, (int32_t)(count * E), (int32_t)(count * E), 0, 0, 0,
+                           (int32_t)(k * E)};
+    int rc = AMPCOR_HIP_OK;
+    hipError_t e = hipMemcpyAsync(raster, host.data(), host.size(), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(plan->d_desc.ptr, desc.data(), count * sizeof(PairDesc),
+                           hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemsetAsync(plan->window.ptr, 0, count * sizeof(int2), s);
+    // the target upsampling of the plan's adjust: quantized outputs for the plain
+    // mixed forms (their FP16 matrix-core path), FP32 amplitudes otherwise
+    const bool quant = plan->mixed && upsample_fuses_quantization(plan->E, plan->rf);
+    if (e == hipSuccess)
+        e = quant ? launch_upsample(s, plan->d_desc.ptr, (int)count, dtype, 1, plan->window.ptr,
+                                    plan->E, plan->rf, plan->U_tgt.ptr, plan->g_tgt.ptr,
+                                    plan->amp_tgt_r.ptr, plan->fp16 ? kUpFP16 : kUpBF16,
+                                    plan->tgt_q_r.ptr, 0, plan->mstats_r.ptr, plan->tgt_lo_r.ptr,
+                                    plan->hfrag_tgt.ptr, &plan->up_scratch)
+                  : launch_upsample(s, plan->d_desc.ptr, (int)count, dtype, 1, plan->window.ptr,
+                                    plan->E, plan->rf, plan->U_tgt.ptr, plan->g_tgt.ptr,
+                                    plan->amp_tgt_r.ptr, kUpFP32, nullptr, 0, nullptr, nullptr,
+                                    nullptr, &plan->up_scratch);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(amp, plan->amp_tgt_r.ptr, count * (size_t)M * M * sizeof(float),
+                           hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) rc = hip_fail(e, "upsampling tiles");
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(raster);
+    // the plan's own descriptors go up again on the next adjust
+    plan->desc_dirty = true;
+    plan->adjusted = false;
+    return rc;
 }
 
 int ampcor_hip_stage_zoom(ampcor_hip_plan *plan, const float *surfaces)

@@ -133,6 +133,9 @@ int ampcor_hip_products(ampcor_hip_plan *plan, float *offsets, float *coarse_pea
 /* surface dimensions: coarse P = window - chip + 1, refined R = 2 rf rm + 1 */
 int ampcor_hip_plan_dims(const ampcor_hip_plan *plan, int *coarse_dim, int *refined_dim,
                          int *zoomed_dim);
+/* the refined window's source and result edges: E = chip + 2 refine margin
+ * (the expanded target window), rf E */
+int ampcor_hip_plan_tile_dims(const ampcor_hip_plan *plan, size_t *expanded, size_t *refined_window);
 /* coarse correlation surfaces [pairs][P][P] */
 int ampcor_hip_stage_gamma(ampcor_hip_plan *plan, float *gamma);
 /* refined correlation surfaces [pairs][R][R] */
@@ -141,6 +144,16 @@ int ampcor_hip_stage_gamma_refined(ampcor_hip_plan *plan, float *gamma);
  * (top, left), zoomed argmax (p, q) */
 int ampcor_hip_stage_locations(ampcor_hip_plan *plan, int32_t *coarse, int32_t *window,
                                int32_t *zoomed);
+/* run the target upsampling of the plan's adjust alone on host tiles
+ * [count][E][E] (E = chip + 2 refine margin; complex64 or float32, count <=
+ * pairs): amp receives |y| [count][rf E][rf E] of the band-limited x rf
+ * upsampling y of each tile -- for the plain BF16/FP16/FP8 forms through their
+ * FP16 matrix-core upsampler (k_upsample2h, up to 96 complex or 160 real
+ * samples), otherwise the FP32 one (k_upsample2 up to 96, the banded two-pass
+ * form above, the dense form for rf != 2).  Overwrites the last adjust's
+ * intermediate buffers (the next adjust re-registers the pairs). */
+int ampcor_hip_stage_upsample(ampcor_hip_plan *plan, int dtype, const void *tiles, size_t count,
+                              float *amp);
 /* run the zoom stage alone on host refined surfaces [pairs][R][R] (the
  * reference drives _zoomcor the same way in its tests): the products then hold
  * its results -- refined peak, zoomed location (stage_locations), and offsets

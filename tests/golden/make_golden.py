@@ -2,12 +2,19 @@
 
 Vectors computed independently of the oracle, with NumPy's FFT:
   upsample_*.npz   x2 / x3 band-limited upsampling of random complex tiles
-                   (zero-padded spectrum, Nyquist bin split for even sizes)
+                   (zero-padded spectrum, Nyquist bin split for even sizes),
+                   including 48 x 48 and 112 x 112 tiles for the GPU upsamplers
   zoom_*.npz       band-limited zoom of random real odd-size surfaces
-and one end-to-end case whose expected result is fixed by construction:
+and end-to-end cases whose expected result is fixed by construction:
   shift_e2e.npz    a 256 x 256 speckle pair related by a periodic Fourier shift
                    of (+1.375, -2.625) px plus 25 dB noise; the offsets of every
                    pair must come out near that shift.
+  shift_offgrid.npz  the same construction with a shift of (+3.31, -1.69) px,
+                   which is not a multiple of the zoomed cell 1/(rf zf) = 1/8 px:
+                   the sub-pixel parabola moves each offset by about 0.06 px
+                   there, so leaving it out (or flipping it) misses the shift.
+
+    python tests/golden/make_golden.py [name ...]   (default: all)
 """
 
 import os
@@ -54,20 +61,39 @@ def fft_zoom(g, zf):
     return np.real(np.fft.ifft2(Y)) * zf * zf
 
 
-def main():
+def shift_fixture(name, seed, shift):
+    s = synth.shifted_scene(seed=seed, size=256, grid=(4, 4), shift=shift, snr_db=25.0)
+    np.savez(os.path.join(HERE, name), reference=s.reference, target=s.target,
+             ref_begin=s.ref_begin, tgt_begin=s.tgt_begin,
+             geometry=np.array([s.chip, s.window, s.refine_factor, s.refine_margin, s.zoom_factor]),
+             shift=np.array(shift))
+
+
+def main(names=()):
+    want = lambda name: not names or name in names
     rng = np.random.default_rng(20261203)
     for N, rf in [(8, 2), (10, 2), (9, 2), (6, 3)]:
         x = rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))
-        np.savez(os.path.join(HERE, f"upsample_{N}x{rf}.npz"), x=x, rf=rf, y=fft_upsample(x, rf))
+        if want(f"upsample_{N}x{rf}.npz"):
+            np.savez(os.path.join(HERE, f"upsample_{N}x{rf}.npz"), x=x, rf=rf, y=fft_upsample(x, rf))
     for R, zf in [(9, 4), (11, 3)]:
         g = rng.standard_normal((R, R))
-        np.savez(os.path.join(HERE, f"zoom_{R}x{zf}.npz"), g=g, zf=zf, z=fft_zoom(g, zf))
-    s = synth.shifted_scene(seed=21, size=256, grid=(4, 4), shift=(1.375, -2.625), snr_db=25.0)
-    np.savez(os.path.join(HERE, "shift_e2e.npz"), reference=s.reference, target=s.target,
-             ref_begin=s.ref_begin, tgt_begin=s.tgt_begin,
-             geometry=np.array([s.chip, s.window, s.refine_factor, s.refine_margin, s.zoom_factor]),
-             shift=np.array([1.375, -2.625]))
+        if want(f"zoom_{R}x{zf}.npz"):
+            np.savez(os.path.join(HERE, f"zoom_{R}x{zf}.npz"), g=g, zf=zf, z=fft_zoom(g, zf))
+    # tiles the GPU upsamplers take as target windows: 48 (chip 32 + 2 x 8) runs
+    # the FP16 matrix-core upsampler of the mixed forms, 112 (chip 96 + 2 x 8)
+    # the banded two-pass one of the FP32 form
+    big = np.random.default_rng(20261204)
+    for N in (48, 112):
+        x = big.standard_normal((N, N)) + 1j * big.standard_normal((N, N))
+        x = x.astype(np.complex64).astype(np.complex128)   # exactly a complex64 raster's tile
+        if want(f"upsample_{N}x2.npz"):
+            np.savez(os.path.join(HERE, f"upsample_{N}x2.npz"), x=x, rf=2, y=fft_upsample(x, 2))
+    if want("shift_e2e.npz"):
+        shift_fixture("shift_e2e.npz", 21, (1.375, -2.625))
+    if want("shift_offgrid.npz"):
+        shift_fixture("shift_offgrid.npz", 22, (3.31, -1.69))
 
 
 if __name__ == "__main__":
-    main()
+    main(tuple(sys.argv[1:]))

@@ -38,6 +38,10 @@ def check_against_oracles(hip, twin, exact):
     np.testing.assert_allclose(hip["coarse_peak"], exact["coarse_peak"], rtol=REL_TOL)
     np.testing.assert_allclose(hip["refined_peak"], exact["refined_peak"], rtol=REL_TOL)
     np.testing.assert_allclose(hip["snr"], exact["snr"], rtol=REL_TOL)
+    # the refined surfaces themselves, within 1e-5 of each surface's peak
+    gr, er = hip["gamma_refined"].astype(np.float64), exact["gamma_refined"].astype(np.float64)
+    scale = np.nanmax(np.abs(er), axis=(1, 2))[:, None, None]
+    assert np.nanmax(np.abs(gr - er) / scale) <= REL_TOL
     # the tuple adjust returns is the offsets product
     assert np.array_equal(hip["tuple_offsets"], hip["offsets"])
 
@@ -63,6 +67,23 @@ def test_golden_shift_end_to_end(lib):
     exact = pyoracle.adjust(f["reference"], f["target"], f["ref_begin"], f["tgt_begin"], C, W, rf, rm,
                             zf, form="fp64")
     np.testing.assert_allclose(hip["offsets"], exact["offsets"], rtol=0, atol=OFFSET_TOL)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32", OFFSET_TOL), ("bf16", 1e-2)])
+def test_golden_offgrid_shift(lib, precision, tol):
+    """The committed off-grid fixture (+3.31, -1.69 px, not a multiple of the
+    1/8 px zoomed cell): the sub-pixel step carries ~0.06 px of each offset."""
+    import os
+    f = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                             "shift_offgrid.npz"))
+    C, W, rf, rm, zf = (int(v) for v in f["geometry"])
+    hip = run_hip(lib, f["reference"], f["target"], f["ref_begin"], f["tgt_begin"], C, W, rf, rm,
+                  zf, precision=precision)
+    exact = pyoracle.adjust(f["reference"], f["target"], f["ref_begin"], f["tgt_begin"], C, W, rf,
+                            rm, zf, form="fp64")
+    np.testing.assert_allclose(hip["offsets"], exact["offsets"], rtol=0, atol=tol)
+    assert np.max(np.abs(hip["offsets"] - f["shift"])) < 0.03
+    assert np.array_equal(hip["zoom_loc"], exact["zoom_loc"])
 
 
 def test_per_pair_calls_equal_batched(lib):

@@ -8,11 +8,16 @@ sums, NumPy FFT zero-padding), invariances, and planted mistakes that must be
 caught.
 """
 
+import glob
+import os
+
 import numpy as np
 import pytest
 
 from ampcor_amd import synth
 from oracle import pyoracle as po
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 EPS32 = np.finfo(np.float32).eps
 
@@ -276,3 +281,56 @@ def test_invalid_arguments_are_rejected():
     bad[0, 0] = 100
     with pytest.raises(ValueError):
         po.adjust(s.reference, s.target, s.ref_begin, bad, 32, 64, 2, 8, 4)          # off raster
+
+
+def test_parabola_vertex_of_sampled_quadratics():
+    # f(x) = c - a (x - x0)^2 sampled at -1, 0, 1: the vertex is x0 exactly
+    for a in (0.3, 2.0):
+        for c in (0.0, 0.9):
+            for x0 in np.linspace(-0.49, 0.49, 99):
+                f = [c - a * (k - x0) ** 2 for k in (-1.0, 0.0, 1.0)]
+                assert abs(po.parabola(*f) - x0) < 1e-12
+    assert po.parabola(1.0, 1.0, 1.0) == 0.0            # flat: no step
+    assert po.parabola(1.0, 0.0, 1.0) == 0.0            # a minimum: no step
+    assert po.parabola(0.0, 1.0, 1.5) == 0.5            # vertex at 1.5: clamped
+    assert po.parabola(1.5, 1.0, 0.0) == -0.5
+
+
+def zoom_full_answer(g, zf):
+    z = po.zoom_full(np.nan_to_num(g), zf)
+    p, q = np.unravel_index(np.argmax(z), z.shape)
+    Z = z.shape[0]
+    dp = po.parabola(z[p - 1, q], z[p, q], z[p + 1, q]) if 0 < p < Z - 1 else 0.0
+    dq = po.parabola(z[p, q - 1], z[p, q], z[p, q + 1]) if 0 < q < Z - 1 else 0.0
+    return (int(p), int(q)), z[p, q], (dp, dq)
+
+
+def test_zoom_search_is_the_full_grid_maximum():
+    """The pipeline's zoom search (oracle_zoom_search) returns the first maximum of
+    the whole zoomed grid and the parabolas through its neighbours, on the
+    golden surfaces, random surfaces and refined surfaces of a known shift."""
+    surfaces = []
+    for path in sorted(glob.glob(os.path.join(GOLDEN, "zoom_*.npz"))):
+        f = np.load(path)
+        surfaces.append((f["g"], int(f["zf"])))
+    rng = np.random.default_rng(17)
+    for R, zf in ((9, 4), (11, 3), (17, 2)):
+        for _ in range(4):
+            surfaces.append((rng.uniform(-1, 1, (R, R)), zf))
+    f = np.load(os.path.join(GOLDEN, "shift_offgrid.npz"))
+    C, W, rf, rm, zf = (int(v) for v in f["geometry"])
+    out = po.adjust(f["reference"], f["target"], f["ref_begin"][:4], f["tgt_begin"][:4], C, W, rf,
+                    rm, zf, surfaces=True)
+    surfaces += [(g.astype(np.float64), zf) for g in out["gamma_refined"]]
+    far = 0
+    for g, zf in surfaces:
+        (p, q), pk, (dp, dq) = po.zoom_search(g, zf)
+        (ep, eq), epk, (edp, edq) = zoom_full_answer(g, zf)
+        assert (p, q) == (ep, eq)
+        assert abs(pk - epk) <= 1e-12 * max(1.0, abs(epk))
+        assert abs(dp - edp) < 1e-9 and abs(dq - edq) < 1e-9
+        a, b = np.unravel_index(np.argmax(g), g.shape)
+        far += max(abs(p - zf * a), abs(q - zf * b)) > zf
+    # on several of them the maximum is more than a refined sample away from
+    # the largest sample: a search near that sample would miss it
+    assert far >= 3

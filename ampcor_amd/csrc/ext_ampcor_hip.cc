@@ -6,6 +6,8 @@
 //              refineMargin, refineFactor, zoomFactor[, precision, device]) -> worker
 //   addReference(worker, raster, idx, (bLine, bSample), (eLine, eSample))  -> None
 //   addTarget(worker, raster, idx, (bLine, bSample), (eLine, eSample))     -> None
+//     (raster: a device raster capsule, or a (lines, samples) complex64 /
+//     float32 array, uploaded once per worker)
 //   adjust(worker) -> tuple of (dLine, dSample), one per pair
 // plus the raster constructors that replace the host SLC capsule of the CPU
 // module (ext/ampcor/slc.cc:50-140): slc_map(filename, samples, lines),
@@ -47,6 +49,10 @@ struct Worker {
     PyObject *rasters;  // list: keeps every raster the plan points into alive
     size_t pairs;
     bool busy;          // a call released the GIL inside the plan (adjust, autotune)
+    int device;         // the plan's device
+    // buffers (NumPy arrays) passed as rasters: the device copy made at their
+    // first use, by object (the object itself is kept alive in `rasters`)
+    std::unordered_map<PyObject *, ampcor_hip_raster *> uploads;
 };
 
 PyObject *raise(int status)
@@ -94,6 +100,94 @@ ampcor_hip_raster *get_raster(PyObject *o)
         return nullptr;
     }
     return static_cast<ampcor_hip_raster *>(PyCapsule_GetPointer(o, kRasterCapsule));
+}
+
+// the CPU module's SLC capsule (ext/ampcor/capsules.h:26, made by slc_map in
+// ext/ampcor/slc.cc:50-93): a memory map inside that module, read through its
+// own C++ types, which this module does not link
+const char *const kHostSlcCapsule = "ampcor::slc::capsule_t";
+
+// keep `o` alive as long as the worker (by identity: `==` on arrays compares pixels)
+int worker_keeps(Worker *w, PyObject *o)
+{
+    const Py_ssize_t n = PyList_GET_SIZE(w->rasters);
+    for (Py_ssize_t k = 0; k < n; ++k)
+        if (PyList_GET_ITEM(w->rasters, k) == o) return 0;
+    return PyList_Append(w->rasters, o);
+}
+
+// the device raster behind a raster argument of addReference / addTarget /
+// addPairs (ext/ampcor_cuda/sequential.cc:85-143 takes the SLC capsule there):
+//   - a raster capsule of this module (raster(), slc_map()): used as it is;
+//   - a 2-D C-contiguous buffer of complex64 or float32 pixels (a NumPy array
+//     of shape (lines, samples)): uploaded at its first use by this worker, and
+//     that copy serves every later tile of the worker (the pixels are read
+//     once: changes to the array after its first use are not seen);
+//   - the CPU module's SLC capsule: TypeError naming the replacements.
+// The worker keeps the argument alive.
+ampcor_hip_raster *worker_raster(Worker *w, PyObject *o)
+{
+    if (PyCapsule_CheckExact(o)) {
+        const char *name = PyCapsule_GetName(o);
+        if (name && !strcmp(name, kRasterCapsule)) {
+            if (worker_keeps(w, o)) return nullptr;
+            return static_cast<ampcor_hip_raster *>(PyCapsule_GetPointer(o, kRasterCapsule));
+        }
+        if (name && !strcmp(name, kHostSlcCapsule)) {
+            PyErr_SetString(PyExc_TypeError,
+                            "an SLC capsule of the CPU module (ampcor.ext.ampcor.slc_map) maps "
+                            "the raster into host memory through types this module does not "
+                            "link; open the same file with ampcor_hip.slc_map(filename, samples, "
+                            "lines), or pass the pixels as a (lines, samples) complex64 array");
+            return nullptr;
+        }
+        PyErr_SetString(PyExc_TypeError, "invalid raster capsule");
+        return nullptr;
+    }
+    auto hit = w->uploads.find(o);
+    if (hit != w->uploads.end()) return hit->second;
+    if (!PyObject_CheckBuffer(o)) {
+        PyErr_SetString(PyExc_TypeError, "a raster is a raster capsule (ampcor_hip.raster, "
+                                         "ampcor_hip.slc_map) or a 2-D complex64 / float32 array");
+        return nullptr;
+    }
+    Py_buffer view;
+    if (PyObject_GetBuffer(o, &view, PyBUF_C_CONTIGUOUS | PyBUF_FORMAT)) return nullptr;
+    // the pixel type from the buffer's struct format, native byte order
+    const char *fmt = view.format ? view.format : "B";
+    if (*fmt == '@' || *fmt == '=' || *fmt == '<') ++fmt;
+    int dtype = -1;
+    if (!strcmp(fmt, "Zf") && view.itemsize == 8) dtype = AMPCOR_HIP_COMPLEX64;
+    else if (!strcmp(fmt, "f") && view.itemsize == 4) dtype = AMPCOR_HIP_FLOAT32;
+    if (dtype < 0 || view.ndim != 2) {
+        PyErr_Format(PyExc_TypeError,
+                     "a raster array must be 2-D (lines, samples) complex64 or float32 in native "
+                     "byte order (got %d-D, format '%s')",
+                     view.ndim, view.format ? view.format : "B");
+        PyBuffer_Release(&view);
+        return nullptr;
+    }
+    const size_t lines = (size_t)view.shape[0], samples = (size_t)view.shape[1];
+    ampcor_hip_raster *r = nullptr;
+    int rc;
+    Py_BEGIN_ALLOW_THREADS
+    rc = ampcor_hip_raster_create(w->device, dtype, view.buf, lines, samples, &r);
+    Py_END_ALLOW_THREADS
+    PyBuffer_Release(&view);
+    if (rc) {
+        raise(rc);
+        return nullptr;
+    }
+    PyObject *capsule = PyCapsule_New(r, kRasterCapsule, free_raster);
+    if (!capsule) {
+        ampcor_hip_raster_destroy(r);
+        return nullptr;
+    }
+    const int kept = worker_keeps(w, capsule) || worker_keeps(w, o);
+    Py_DECREF(capsule);   // the worker's list holds it
+    if (kept) return nullptr;
+    w->uploads[o] = r;
+    return r;
 }
 
 int parse_dtype(const char *name, int *dtype)
@@ -236,7 +330,7 @@ PyObject *sequential(PyObject *, PyObject *args, PyObject *kwds)
     int rc = ampcor_hip_plan_create(device, pairs, refLines, refSamples, tgtLines, tgtSamples,
                                     refineMargin, refineFactor, zoomFactor, precision, &plan);
     if (rc) return raise(rc);
-    auto *w = new Worker{plan, PyList_New(0), pairs, false};
+    auto *w = new Worker{plan, PyList_New(0), pairs, false, device, {}};
     return PyCapsule_New(w, kWorkerCapsule, free_worker);
 }
 
@@ -244,20 +338,16 @@ PyObject *add_tile(PyObject *args, bool reference)
 {
     PyObject *pyWorker, *pyRaster;
     unsigned long idx, bl, bs, el, es;
-    if (!PyArg_ParseTuple(args, reference ? "O!O!k(kk)(kk):addReference" : "O!O!k(kk)(kk):addTarget",
-                          &PyCapsule_Type, &pyWorker, &PyCapsule_Type, &pyRaster, &idx, &bl, &bs,
-                          &el, &es))
+    if (!PyArg_ParseTuple(args, reference ? "O!Ok(kk)(kk):addReference" : "O!Ok(kk)(kk):addTarget",
+                          &PyCapsule_Type, &pyWorker, &pyRaster, &idx, &bl, &bs, &el, &es))
         return nullptr;
     Worker *w = get_worker(pyWorker);
     if (!w) return nullptr;
-    ampcor_hip_raster *r = get_raster(pyRaster);
+    ampcor_hip_raster *r = worker_raster(w, pyRaster);
     if (!r) return nullptr;
     int rc = reference ? ampcor_hip_add_reference(w->plan, r, idx, bl, bs, el, es)
                        : ampcor_hip_add_target(w->plan, r, idx, bl, bs, el, es);
     if (rc) return raise(rc);
-    int found = PySequence_Contains(w->rasters, pyRaster);
-    if (found < 0) return nullptr;
-    if (!found && PyList_Append(w->rasters, pyRaster)) return nullptr;
     Py_RETURN_NONE;
 }
 
@@ -271,13 +361,13 @@ PyObject *addPairs(PyObject *, PyObject *args)
     PyObject *pyWorker, *pyRef, *pyTgt;
     unsigned long first;
     Py_buffer rb, tb;
-    if (!PyArg_ParseTuple(args, "O!O!O!ky*y*:addPairs", &PyCapsule_Type, &pyWorker, &PyCapsule_Type,
-                          &pyRef, &PyCapsule_Type, &pyTgt, &first, &rb, &tb))
+    if (!PyArg_ParseTuple(args, "O!OOky*y*:addPairs", &PyCapsule_Type, &pyWorker, &pyRef, &pyTgt,
+                          &first, &rb, &tb))
         return nullptr;
     PyObject *result = nullptr;
     Worker *w = get_worker(pyWorker);
-    ampcor_hip_raster *ref = w ? get_raster(pyRef) : nullptr;
-    ampcor_hip_raster *tgt = ref ? get_raster(pyTgt) : nullptr;
+    ampcor_hip_raster *ref = w ? worker_raster(w, pyRef) : nullptr;
+    ampcor_hip_raster *tgt = ref ? worker_raster(w, pyTgt) : nullptr;
     if (tgt) {
         if (rb.len != tb.len || rb.len % 16) {
             PyErr_SetString(PyExc_ValueError, "corner buffers must hold equal counts of int64 pairs");
@@ -287,15 +377,8 @@ PyObject *addPairs(PyObject *, PyObject *args)
                                           static_cast<const int64_t *>(tb.buf));
             if (rc) raise(rc);
             else {
-                bool ok = true;
-                for (PyObject *o : {pyRef, pyTgt}) {
-                    int found = PySequence_Contains(w->rasters, o);
-                    if (found < 0 || (!found && PyList_Append(w->rasters, o))) ok = false;
-                }
-                if (ok) {
-                    Py_INCREF(Py_None);
-                    result = Py_None;
-                }
+                Py_INCREF(Py_None);
+                result = Py_None;
             }
         }
     }

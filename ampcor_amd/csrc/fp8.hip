@@ -1,6 +1,8 @@
 // fp8.hip -- tiling and launch of the FP8 coarse contraction (fp8_correlate.h).
 #include "fp8_correlate.h"
 
+#include <algorithm>
+
 namespace ampcor_hip {
 
 Fp8Shape choose_fp8_shape(int C, int P, int nsplit, int max_vt)
@@ -42,22 +44,34 @@ hipError_t launch_fp8_nk(hipStream_t s, const Fp8Shape &sh, const uint8_t *chip_
 {
     using namespace fp8_detail;
     const int P = W - C + 1;
-    const dim3 grid((unsigned)pairs * sh.tiles_u * sh.tiles_v * sh.nsplit), block(64 * sh.wg_ut);
+    const size_t per = (size_t)sh.tiles_u * sh.tiles_v * sh.nsplit;   // workgroups per pair
+    const dim3 block(64 * sh.wg_ut);
+    // a launch's work-items fit in 32 bits: pairs go in chunks
+    const int chunk = std::min(pairs, launch_pair_cap(per * block.x));
 #define AMPCOR_FP8_VT(VTV)                                                                         \
     case VTV:                                                                                      \
-        hipLaunchKernelGGL((k_correlate_fp8<NK, VTV>), grid, block, sh.lds_bytes, s, chip_q, tgt_q, \
-                           C, W, P, sh.ilen, sh.wg_ut, sh.tiles_u, sh.tiles_v, sh.nsplit, partial); \
+        hipLaunchKernelGGL((k_correlate_fp8<NK, VTV>), grid, block, sh.lds_bytes, s, cq, tq, C, W, \
+                           P, sh.ilen, sh.wg_ut, sh.tiles_u, sh.tiles_v, sh.nsplit, pt);          \
         break
-    switch (sh.wg_vt) {
-        AMPCOR_FP8_VT(1);
-        AMPCOR_FP8_VT(2);
-        AMPCOR_FP8_VT(3);
-        AMPCOR_FP8_VT(4);
-        AMPCOR_FP8_VT(5);
-    default: return hipErrorInvalidValue;
+    for (int first = 0; first < pairs; first += chunk) {
+        const int n = std::min(chunk, pairs - first);
+        const dim3 grid((unsigned)(n * per));
+        const uint8_t *cq = chip_q + (size_t)first * C * sh.cs;
+        const uint8_t *tq = tgt_q + (size_t)first * W * W;
+        float *pt = partial + (size_t)first * sh.nsplit * P * P;
+        switch (sh.wg_vt) {
+            AMPCOR_FP8_VT(1);
+            AMPCOR_FP8_VT(2);
+            AMPCOR_FP8_VT(3);
+            AMPCOR_FP8_VT(4);
+            AMPCOR_FP8_VT(5);
+        default: return hipErrorInvalidValue;
+        }
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
     }
 #undef AMPCOR_FP8_VT
-    return hipGetLastError();
+    return hipSuccess;
 }
 
 }  // namespace

@@ -205,6 +205,14 @@ hipError_t launch_zoom(hipStream_t s, const float *gamma_refined, int pairs, int
                        int margin, int rf, float *offsets, float *refined_peak, int2 *zoom_loc,
                        const ZoomScratch &scratch, unsigned *stats = nullptr);
 
+// pairs that one launch may take when each pair brings `items` work-items: a
+// dispatch's work-items (workgroups x threads) must fit in 32 bits
+inline int launch_pair_cap(size_t items)
+{
+    const size_t cap = ((size_t)0xffffffffu - 1023) / (items ? items : 1);
+    return cap < 1 ? 1 : cap > (size_t)0x7fffffff ? 0x7fffffff : (int)cap;
+}
+
 // the logical index of workgroup b of n: the hardware hands consecutive
 // workgroups to the eight XCDs in turn, and this gives each XCD a contiguous
 // run of logical indices, so the workgroups that read one pair's operands (or

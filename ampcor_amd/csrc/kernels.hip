@@ -3122,8 +3122,8 @@ hipError_t launch_correlate(hipStream_t s, const CorrelateShape &sh, const float
     const size_t blocks = (size_t)sh.tiles_u * sh.tiles_v * sh.nsplit;
     if (sh.exact && (sh.nsplit != 1 || !sums || !stats || !gamma)) return hipErrorInvalidValue;
     if (!sh.exact && !partial) return hipErrorInvalidValue;
-    // at most 2^31 - 1 workgroups per launch: pairs go in chunks
-    const int chunk = (int)std::min<size_t>((size_t)pairs, (size_t)0x7fffffff / blocks);
+    // a launch's work-items fit in 32 bits: pairs go in chunks
+    const int chunk = std::min(pairs, launch_pair_cap(blocks * sh.threads));
     for (int first = 0; first < pairs; first += chunk) {
         const int n = std::min(chunk, pairs - first);
         const dim3 grid((unsigned)(n * blocks)), block(sh.threads);
@@ -3162,11 +3162,18 @@ hipError_t launch_normalize(hipStream_t s, const float *partial, int nsplit, con
                             const double2 *stats, int pairs, int C, int W, float *gamma)
 {
     const int P = W - C + 1;
-    const size_t total = (size_t)pairs * P * P;
-    const unsigned blocks = (unsigned)((total + 255) / 256);
-    hipLaunchKernelGGL(k_normalize, dim3(blocks), dim3(256), 0, s, partial, nsplit, sums, stats, C,
-                       P, pairs, gamma);
-    return hipGetLastError();
+    const size_t pp = (size_t)P * P;
+    const int chunk = std::min(pairs, launch_pair_cap(pp));
+    for (int first = 0; first < pairs; first += chunk) {
+        const int n = std::min(chunk, pairs - first);
+        const unsigned blocks = (unsigned)((n * pp + 255) / 256);
+        hipLaunchKernelGGL(k_normalize, dim3(blocks), dim3(256), 0, s,
+                           partial + (size_t)first * nsplit * pp, nsplit, sums + first * pp,
+                           stats + first, C, P, n, gamma + first * pp);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 bool normalize_fuses(int W, int C)

@@ -18,6 +18,8 @@
 
 #include <math.h>
 
+#include <algorithm>
+
 namespace ampcor_hip {
 
 namespace {
@@ -262,10 +264,17 @@ hipError_t launch_normalize_mixed(hipStream_t s, const float *partial, int nspli
                                   int W, float *gamma)
 {
     const int P = W - C + 1;
-    const size_t total = (size_t)pairs * P * P;
-    hipLaunchKernelGGL(k_normalize_mixed, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
-                       partial, nsplit, sums, stats, C, P, pairs, gamma);
-    return hipGetLastError();
+    const size_t pp = (size_t)P * P;
+    const int chunk = std::min(pairs, launch_pair_cap(pp));
+    for (int first = 0; first < pairs; first += chunk) {
+        const int n = std::min(chunk, pairs - first);
+        hipLaunchKernelGGL(k_normalize_mixed, dim3((unsigned)((n * pp + 255) / 256)), dim3(256), 0,
+                           s, partial + (size_t)first * nsplit * pp, nsplit, sums + first * pp,
+                           stats + first, C, P, n, gamma + first * pp);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 }  // namespace ampcor_hip

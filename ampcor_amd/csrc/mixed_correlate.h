@@ -30,6 +30,8 @@
 
 #include "kernels.h"
 
+#include <algorithm>
+
 namespace ampcor_hip {
 
 namespace mixed_detail {
@@ -283,11 +285,13 @@ hipError_t launch_correlate_mixed_nk(hipStream_t s, const MixedShape &sh, const 
 {
     using namespace mixed_detail;
     const int P = W - C + 1;
-    const dim3 grid((unsigned)pairs * sh.tiles_u * sh.tiles_v * sh.nsplit), block(64 * sh.wg_ut);
+    const size_t per = (size_t)sh.tiles_u * sh.tiles_v * sh.nsplit;   // workgroups per pair
+    const dim3 block(64 * sh.wg_ut);
+    // a launch's work-items fit in 32 bits: pairs go in chunks
+    const int chunk = std::min(pairs, launch_pair_cap(per * block.x));
 #define AMPCOR_MIXED_ONE(VTV, F16, SPL)                                                            \
-    hipLaunchKernelGGL((k_correlate_mfma<NK, VTV, F16, SPL>), grid, block, sh.lds_bytes, s,       \
-                       chip_q, tgt_q, C, W, P, sh.wg_ut, sh.tiles_u, sh.tiles_v, sh.nsplit,       \
-                       partial, chip_lo, tgt_lo)
+    hipLaunchKernelGGL((k_correlate_mfma<NK, VTV, F16, SPL>), grid, block, sh.lds_bytes, s, cq,   \
+                       tq, C, W, P, sh.wg_ut, sh.tiles_u, sh.tiles_v, sh.nsplit, pt, cl, tl)
 #define AMPCOR_MIXED_VT(VTV)                                                                       \
     do {                                                                                           \
         if (sh.split) {                                                                            \
@@ -298,17 +302,28 @@ hipError_t launch_correlate_mixed_nk(hipStream_t s, const MixedShape &sh, const 
             else AMPCOR_MIXED_ONE(VTV, false, false);                                              \
         }                                                                                          \
     } while (0)
-    switch (sh.wg_vt) {
-    case 1: AMPCOR_MIXED_VT(1); break;
-    case 2: AMPCOR_MIXED_VT(2); break;
-    case 3: AMPCOR_MIXED_VT(3); break;
-    case 4: AMPCOR_MIXED_VT(4); break;
-    case 5: AMPCOR_MIXED_VT(5); break;
-    default: return hipErrorInvalidValue;
+    for (int first = 0; first < pairs; first += chunk) {
+        const int n = std::min(chunk, pairs - first);
+        const dim3 grid((unsigned)(n * per));
+        const size_t oc = (size_t)first * C * sh.cs, ot = (size_t)first * W * W;
+        const uint16_t *cq = chip_q + oc, *tq = tgt_q + ot;
+        const uint16_t *cl = chip_lo ? chip_lo + oc : nullptr;
+        const uint16_t *tl = tgt_lo ? tgt_lo + ot : nullptr;
+        float *pt = partial + (size_t)first * sh.nsplit * P * P;
+        switch (sh.wg_vt) {
+        case 1: AMPCOR_MIXED_VT(1); break;
+        case 2: AMPCOR_MIXED_VT(2); break;
+        case 3: AMPCOR_MIXED_VT(3); break;
+        case 4: AMPCOR_MIXED_VT(4); break;
+        case 5: AMPCOR_MIXED_VT(5); break;
+        default: return hipErrorInvalidValue;
+        }
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
     }
 #undef AMPCOR_MIXED_VT
 #undef AMPCOR_MIXED_ONE
-    return hipGetLastError();
+    return hipSuccess;
 }
 #else
     ;

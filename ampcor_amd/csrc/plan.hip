@@ -242,7 +242,11 @@ int plan_configure(ampcor_hip_plan *p, int device, size_t pairs, size_t ref_line
                    size_t refine_factor, size_t zoom_factor, int precision)
 {
     if (pairs == 0) return fail(AMPCOR_HIP_EINVAL, "a plan needs at least one pair");
-    if (pairs > (size_t)1 << 24) return fail(AMPCOR_HIP_EINVAL, "too many pairs for one plan");
+    // launches with one workgroup (at most 1024 threads) per pair stay within
+    // 32-bit work-item counts; those with more workgroups per pair go in chunks
+    if (pairs > kMaxPlanPairs)
+        return fail(AMPCOR_HIP_EINVAL, "at most " + std::to_string(kMaxPlanPairs) +
+                                           " pairs per plan: split the batch");
     // the kernels assume square tiles (Sequential.icc:56-60 makes the same assumption)
     if (ref_lines != ref_samples || tgt_lines != tgt_samples)
         return fail(AMPCOR_HIP_EINVAL, "chips and windows must be square");

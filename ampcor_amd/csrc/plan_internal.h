@@ -76,6 +76,7 @@ struct DeviceBuffer {
 };
 
 constexpr int kStages = 16;  // room for every stage of any form
+constexpr size_t kMaxPlanPairs = (size_t)1 << 21;  // pairs one plan may hold
 
 // makes the plan's device current for the duration of a call
 struct DeviceGuard {

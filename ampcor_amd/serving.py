@@ -7,7 +7,10 @@ field of up to 8 px plus noise) with a chip edge from {32, 64, 128}, a
 search margin from {16, 32, 64}, a pair count log-uniform in [64, 4096],
 refine factor 2, refine margin 8 and zoom 8; pair centers are uniform over
 the raster area the window fits in.  Latency is measured by the server from
-submission to the moment the job's last batch finishes.
+submission to the moment the job's last batch finishes.  The benchmark runs
+the stream three times on one server: at the given rate, with every job
+submitted at once (its throughput is the saturated rate), and at a fixed
+fraction of that saturated rate.
 """
 
 import threading
@@ -123,9 +126,27 @@ def run_stream(lib, server, rasters, jobs, precision):
     return lat, wall, results
 
 
+def stream_stats(jobs, lat, wall):
+    pairs = sum(j.pairs for j in jobs)
+    return {"pairs": int(pairs), "wall_s": round(wall, 3), "pairs_per_sec": round(pairs / wall, 1),
+            "p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "max_ms": round(float(lat.max()), 3)}
+
+
+def at_rate(jobs, rate):
+    """The same jobs with their arrival times rescaled to `rate` jobs/s (0: all at once)."""
+    first = jobs[-1].arrival / len(jobs) if jobs else 1.0
+    scale = 0.0 if rate <= 0 else 1.0 / (rate * first)
+    return [Job(j.raster, j.chip, j.margin, j.ref_begin, j.tgt_begin, j.arrival * scale)
+            for j in jobs]
+
+
 def serve_benchmark(precision="bf16", count=256, rate=100.0, seed=5, batch=2048, slots=2,
-                    budget_gb=96.0, pool=None, log=None):
-    """Run the whole stream; returns a dict of latency percentiles and throughput."""
+                    budget_gb=96.0, pool=None, log=None, load=0.75):
+    """Run the stream at `rate`, then all at once (the saturated throughput),
+    then at `load` x that throughput; returns latency percentiles and pairs/s
+    of each, the first at the top level."""
     lib = ext.require()
     pool = pool if pool is not None else raster_pool(seed=seed)
     rasters = upload(lib, pool)
@@ -146,16 +167,20 @@ def serve_benchmark(precision="bf16", count=256, rate=100.0, seed=5, batch=2048,
     if log:
         log(f"serving warm-up ({len(warmup)} geometries): {time.perf_counter() - t:.1f} s")
     lat, wall, _ = run_stream(lib, server, rasters, jobs, precision)
+    out = {"precision": precision, "jobs": count, "arrival_rate_jobs_per_s": rate}
+    out.update(stream_stats(jobs, lat, wall))
+    # saturation: every job submitted at once
+    lat, wall, _ = run_stream(lib, server, rasters, at_rate(jobs, 0), precision)
+    sat = stream_stats(jobs, lat, wall)
+    out["saturation"] = sat
+    # a fixed rate at `load` x the saturated pairs/s
+    mean_pairs = sat["pairs"] / count
+    loaded_rate = load * sat["pairs_per_sec"] / mean_pairs
+    lat, wall, _ = run_stream(lib, server, rasters, at_rate(jobs, loaded_rate), precision)
+    out["loaded"] = {"load_of_saturation": load,
+                     "arrival_rate_jobs_per_s": round(loaded_rate, 1)}
+    out["loaded"].update(stream_stats(jobs, lat, wall))
     stats = lib.server_stats(server)
-    pairs = sum(j.pairs for j in jobs)
-    return {
-        "precision": precision, "jobs": count, "arrival_rate_jobs_per_s": rate,
-        "pairs": int(pairs), "wall_s": round(wall, 3),
-        "pairs_per_sec": round(pairs / wall, 1),
-        "p50_ms": round(float(np.percentile(lat, 50)), 3),
-        "p99_ms": round(float(np.percentile(lat, 99)), 3),
-        "max_ms": round(float(lat.max()), 3),
-        "batch_pairs": batch, "slots_per_lane": slots,
-        "batches": int(stats["batches"]), "graph_launches": int(stats["graph_launches"]),
-        "lanes": int(stats["lanes"]),
-    }
+    out.update({"batch_pairs": batch, "slots_per_lane": slots, "batches": int(stats["batches"]),
+                "graph_launches": int(stats["graph_launches"]), "lanes": int(stats["lanes"])})
+    return out

@@ -16,7 +16,8 @@ with HIP events on the plan's stream during the timed steps; the CPU baseline
 is the FP64 oracle on a bounded sample of the same pairs.  The "serving"
 object reports the continuous-batching server on a Poisson stream of 256 jobs
 of mixed geometries (ampcor_amd/serving.py): p50/p99 job latency and pairs/s
-at a fixed arrival rate, for each form.
+at a fixed arrival rate, with every job submitted at once (the saturated
+throughput), and at a fixed rate of 75 % of that throughput, for each form.
 """
 
 import argparse

@@ -141,11 +141,14 @@ struct MixedShape {
     bool split;
     int wg_ut, wg_vt, tiles_u, tiles_v, nsplit;
     size_t lds_bytes;
+    bool stream;   // the streaming form: all chip rows per workgroup, staged chunk by chunk
 };
 // nsplit > 0 asks for at least that many chip-row ranges (tuning)
 // split: hi/lo operands (three products per pair of elements), twice the LDS
 // max_vt > 0 caps the v-tiles per workgroup (tuning)
 MixedShape choose_mixed_shape(int C, int P, int nsplit = 0, bool split = false, int max_vt = 0);
+// the streaming form's shape (plain operands only); nk = 0 where it does not apply
+MixedShape choose_mixed_stream_shape(int C, int P, int max_vt = 0);
 // chip_lo / tgt_lo (optional): the rounding remainders, for the split form
 hipError_t launch_prep_mixed(hipStream_t s, const float *ref_amp, const float *tgt_amp, int pairs,
                              int C, int W, int CS, bool fp16, uint16_t *chip_q, uint16_t *tgt_q,

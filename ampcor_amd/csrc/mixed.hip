@@ -225,6 +225,33 @@ MixedShape choose_mixed_shape(int C, int P, int nsplit, bool split, int max_vt)
     return s;
 }
 
+MixedShape choose_mixed_stream_shape(int C, int P, int max_vt)
+{
+    MixedShape s{};
+    const int nk = (C + 15 + 15) / 16, tiles = (P + 15) / 16;
+    // whole chunks of chip rows; window rows of whole 16-byte loads (W = P + C - 1)
+    if (C % mixed_detail::kStreamRows || (P + C - 1) % 8) return s;
+    const int vmax = max_vt > 0 && max_vt < mixed_detail::kMaxVT ? max_vt : mixed_detail::kMaxVT;
+    const int vblocks = (tiles + vmax - 1) / vmax;
+    const int wg_vt = (tiles + vblocks - 1) / vblocks;
+    const int max_ut = mixed_detail::max_waves(nk);
+    const int ublocks = (tiles + max_ut - 1) / max_ut;
+    const int wg_ut = (tiles + ublocks - 1) / ublocks;
+    // each thread brings at most kStreamItems loads of a chunk
+    if (mixed_stream_items(nk, wg_vt) > mixed_detail::kStreamItems * 64 * wg_ut) return s;
+    s.nk = nk;
+    s.cs = 16 * nk + 16;
+    s.split = false;
+    s.wg_ut = wg_ut;
+    s.wg_vt = wg_vt;
+    s.tiles_u = (tiles + wg_ut - 1) / wg_ut;
+    s.tiles_v = (tiles + wg_vt - 1) / wg_vt;
+    s.nsplit = 1;
+    s.lds_bytes = mixed_stream_lds_bytes(nk, wg_vt, wg_ut);
+    s.stream = true;
+    return s;
+}
+
 hipError_t launch_prep_mixed(hipStream_t s, const float *ref_amp, const float *tgt_amp, int pairs,
                              int C, int W, int CS, bool fp16, uint16_t *chip_q, uint16_t *tgt_q,
                              MixedStats *stats, uint16_t *chip_lo, uint16_t *tgt_lo)
@@ -244,6 +271,11 @@ hipError_t launch_correlate_mixed(hipStream_t s, const MixedShape &sh, const uin
 {
     if (sh.split && (!chip_lo || !tgt_lo)) return hipErrorInvalidValue;
     if (sh.nk != (C + 15 + 15) / 16 || sh.cs != 16 * sh.nk + 16 || (C / sh.nsplit) % 2 != 0)
+        return hipErrorInvalidValue;
+    if (sh.stream &&
+        (sh.split || sh.nsplit != 1 || C % mixed_detail::kStreamRows || W % 8 ||
+         mixed_stream_items(sh.nk, sh.wg_vt) > mixed_detail::kStreamItems * 64 * sh.wg_ut ||
+         sh.lds_bytes != mixed_stream_lds_bytes(sh.nk, sh.wg_vt, sh.wg_ut)))
         return hipErrorInvalidValue;
 #define AMPCOR_NK(N)                                                                               \
     case N:                                                                                        \

@@ -266,6 +266,207 @@ __global__ __launch_bounds__(64 * max_waves(NK)) void k_correlate_mfma(
     }
 }
 
+// ---- the streaming form -------------------------------------------------------
+//
+// One workgroup per (pair, block of u- and v-tiles) over all C chip rows, which
+// pass through LDS kStreamRows at a time: while the waves multiply one chunk,
+// each thread's global loads for the next chunk are in flight, and their data
+// go to LDS slots the current chunk does not read, so one barrier per chunk
+// suffices.  Target rows live in a ring of RS = 16 wg_ut + 2 kStreamRows rows
+// (the rows of the current chunk and the next chunk's new rows); the chip rows
+// of a chunk take one of two slots.  The fragments, the DPP row shifts and the
+// MFMA steps are those of k_correlate_mfma; the shift that crosses into the
+// next chunk waits for the barrier.  All chip rows in one workgroup: one
+// partial surface per pair.
+constexpr int kStreamRows = 8;   // chip rows per chunk
+constexpr int kStreamItems = 4;  // 16-byte loads per thread per chunk, at most
+
+template <int NK, int VT>
+struct StreamGeometry {
+    using G = Geometry<NK, VT>;
+    static constexpr int CPR = G::ncols / 8;   // 16-byte chunks per staged target row
+    static constexpr int CCH = G::CS / 8;      // 16-byte chunks per chip row
+    // loads of one chunk: kStreamRows new target rows and kStreamRows chip rows
+    static constexpr int items = kStreamRows * (CPR + CCH);
+};
+
+template <int NK, int VT, bool FP16>
+__global__ __launch_bounds__(64 * max_waves(NK)) void k_correlate_stream(
+    const uint16_t *__restrict__ chip_q, const uint16_t *__restrict__ tgt_q, int C, int W, int P,
+    int wg_ut, int tiles_u, int tiles_v, float *__restrict__ partial)
+{
+    using G = Geometry<NK, VT>;
+    using SG = StreamGeometry<NK, VT>;
+    constexpr int CS = G::CS, NF = G::NF, TS = G::TS, R = G::kReload;
+    constexpr int CPR = SG::CPR, CCH = SG::CCH, CH = kStreamRows;
+    extern __shared__ uint4 lds_u4[];
+    uint16_t *lds = reinterpret_cast<uint16_t *>(lds_u4);
+    int bidx = xcd_block(blockIdx.x, gridDim.x);
+    const int tv = bidx % tiles_v;
+    bidx /= tiles_v;
+    const int tu = bidx % tiles_u;
+    const int pair = bidx / tiles_u;
+
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x >> 6;
+    const int lm = lane & 15, lk = lane >> 4;
+    const int ut0 = tu * wg_ut, vt0 = tv * VT;
+    const int RS = 16 * wg_ut + 2 * CH;            // target ring rows
+    uint16_t *ring = lds;                          // [RS][TS]
+    uint16_t *chip = lds + (size_t)RS * TS;        // [2 slots][CH rows][2 copies][CS]
+    const int r0 = 16 * ut0, c0 = 16 * vt0;        // block origin in the window
+    const uint16_t *src = tgt_q + (size_t)pair * W * W;
+    const uint16_t *csrc = chip_q + (size_t)pair * C * CS;
+    const int nchunks = C / CH;
+    const int first_rows = 16 * wg_ut + CH - 1;    // target rows of chunk 0
+
+    // item k of the loads that bring chunk c (c >= 1): its new target rows,
+    // then its chip rows; `prev` is the dword before a chip chunk (copy 1's
+    // first element), zero at the start of a row
+    auto load_item = [&](const int c, const int k, uint4 &v, uint32_t &prev) {
+        v = make_uint4(0u, 0u, 0u, 0u);
+        prev = 0u;
+        if (k < CH * CPR) {
+            const int rr = k / CPR, cc = 8 * (k - rr * CPR);
+            const int gr = r0 + first_rows + CH * (c - 1) + rr, gc = c0 + cc;
+            const bool in = gr < W && gc < W;
+            const uint4 x = *reinterpret_cast<const uint4 *>(src + (in ? (size_t)gr * W + gc : 0));
+            if (in) v = x;
+        } else if (k < SG::items) {
+            const int kc = k - CH * CPR, i = kc / CCH, q = kc - i * CCH;
+            const uint16_t *row = csrc + (size_t)(CH * c + i) * CS;
+            v = *reinterpret_cast<const uint4 *>(row + 8 * q);
+            if (q) prev = *reinterpret_cast<const uint32_t *>(row + 8 * q - 2);
+        }
+    };
+    auto store_chip = [&](const int slot, const int i, const int q, const uint4 v,
+                          const uint32_t prev) {
+        uint16_t *d0 = chip + ((size_t)(slot * CH + i) * 2) * CS + 8 * q;
+        *reinterpret_cast<uint4 *>(d0) = v;
+        *reinterpret_cast<uint4 *>(d0 + CS) =
+            make_uint4((prev >> 16) | (v.x << 16), (v.x >> 16) | (v.y << 16),
+                       (v.y >> 16) | (v.z << 16), (v.z >> 16) | (v.w << 16));
+    };
+    auto store_item = [&](const int c, const int k, const uint4 v, const uint32_t prev) {
+        if (k < CH * CPR) {
+            const int rr = k / CPR, cc = 8 * (k - rr * CPR);
+            const int lr = (first_rows + CH * (c - 1) + rr) % RS;
+            *reinterpret_cast<uint4 *>(ring + (size_t)lr * TS + cc) = v;
+        } else if (k < SG::items) {
+            const int kc = k - CH * CPR, i = kc / CCH;
+            store_chip(c & 1, i, kc - i * CCH, v, prev);
+        }
+    };
+
+    // chunk 0: its target rows and chip rows
+    for (int k = threadIdx.x; k < first_rows * CPR; k += blockDim.x) {
+        const int rr = k / CPR, cc = 8 * (k - rr * CPR);
+        const int gr = r0 + rr, gc = c0 + cc;
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (gr < W && gc < W) v = *reinterpret_cast<const uint4 *>(src + (size_t)gr * W + gc);
+        *reinterpret_cast<uint4 *>(ring + (size_t)rr * TS + cc) = v;
+    }
+    for (int kc = threadIdx.x; kc < CH * CCH; kc += blockDim.x) {
+        const int i = kc / CCH, q = kc - i * CCH;
+        const uint16_t *row = csrc + (size_t)i * CS;
+        store_chip(0, i, q, *reinterpret_cast<const uint4 *>(row + 8 * q),
+                   q ? *reinterpret_cast<const uint32_t *>(row + 8 * q - 2) : 0u);
+    }
+    __syncthreads();
+
+    const int ut = ut0 + wave;
+    // waves past the surface still load and meet every barrier
+    const bool active = wave < wg_ut && 16 * ut < P;
+    const int d = lk >> 1, h = lk & 1;
+    // this lane's target row (ring index) at the current step
+    int row = 16 * wave + d + lm;
+    const uint16_t *twc = ring + 8 * h;            // column origin of this lane's fragments
+    uint4 A[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+        A[f] = *reinterpret_cast<const uint4 *>(twc + (size_t)row * TS + 16 * f);
+
+    floatx4 acc[VT], acc2[VT];
+#pragma unroll
+    for (int v = 0; v < VT; ++v) acc[v] = acc2[v] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+
+    // the shift from one step to the next: rows two further down
+    auto advance = [&]() {
+        int nr = row + 2;
+        nr -= nr >= RS ? RS : 0;
+        uint4 Nx[NF], Rl[R > 0 ? R : 1];
+        if (lm >= 14) {
+#pragma unroll
+            for (int f = R; f < NF; ++f)
+                Nx[f] = *reinterpret_cast<const uint4 *>(twc + (size_t)nr * TS + 16 * f);
+        }
+#pragma unroll
+        for (int f = 0; f < R; ++f)
+            Rl[f] = *reinterpret_cast<const uint4 *>(twc + (size_t)nr * TS + 16 * f);
+#pragma unroll
+        for (int f = 0; f < R; ++f) A[f] = Rl[f];
+#pragma unroll
+        for (int f = R; f < NF; ++f) A[f] = shift_up2(A[f], Nx[f]);
+        row = nr;
+    };
+    // the MFMAs of one pair of chip rows (cr: this lane's row d of them)
+    auto multiply = [&](const uint32_t *cr) {
+        uint4 B[NK];
+#pragma unroll
+        for (int kap = 0; kap < NK; ++kap)
+            B[kap] = make_uint4(cr[8 * kap], cr[8 * kap + 1], cr[8 * kap + 2], cr[8 * kap + 3]);
+#pragma unroll
+        for (int kap = 0; kap < NK; ++kap)
+#pragma unroll
+            for (int v = 0; v < VT; ++v) {
+                floatx4 &c = (kap & 1) ? acc2[v] : acc[v];
+                c = mfma32<FP16>(A[v + kap], B[kap], c);
+            }
+    };
+    // chip fragment of K-step kappa: dwords 8 kappa + 4 h + 8 - (lm >> 1) .. +3
+    // of copy (lm & 1) of chip row d of the step
+    const uint32_t *cw = reinterpret_cast<const uint32_t *>(chip) + d * CS + (lm & 1) * (CS / 2) +
+                         4 * h + 8 - (lm >> 1);
+
+    for (int c = 0; c < nchunks; ++c) {
+        const bool more = c + 1 < nchunks;
+        uint4 nv[kStreamItems];
+        uint32_t np[kStreamItems];
+        if (more) {
+#pragma unroll
+            for (int j = 0; j < kStreamItems; ++j)
+                load_item(c + 1, j * (int)blockDim.x + (int)threadIdx.x, nv[j], np[j]);
+        }
+        if (active) {
+            const uint32_t *slot = cw + (size_t)(c & 1) * CH * CS;
+#pragma unroll
+            for (int sr = 0; sr < CH; sr += 2) {
+                multiply(slot + (size_t)sr * CS);
+                if (sr + 2 < CH) advance();
+            }
+        }
+        if (more) {
+#pragma unroll
+            for (int j = 0; j < kStreamItems; ++j)
+                store_item(c + 1, j * (int)blockDim.x + (int)threadIdx.x, nv[j], np[j]);
+        }
+        __syncthreads();
+        if (active && more) advance();
+    }
+    if (!active) return;
+
+    float *dst = partial + (size_t)pair * P * P;
+#pragma unroll
+    for (int v = 0; v < VT; ++v) {
+        const int vv = 16 * (vt0 + v) + lm;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int uu = 16 * ut + 4 * lk + e;
+            if (uu < P && vv < P) dst[(size_t)uu * P + vv] = acc[v][e] + acc2[v][e];
+        }
+    }
+}
+
 }  // namespace mixed_detail
 
 // LDS bytes of one workgroup
@@ -274,6 +475,19 @@ inline size_t mixed_lds_bytes(int C, int nk, int vt, int wg_ut, int nsplit, bool
     const int ilen = C / nsplit, cs = 16 * nk + 16;
     const size_t rows = 16 * wg_ut + ilen - 1, cols = 16 * (vt + nk - 1) + 8;
     return (rows * cols + (size_t)ilen * 2 * cs) * sizeof(uint16_t) * (split ? 2 : 1);
+}
+
+// the streaming form: LDS bytes of one workgroup (target ring and two chip
+// slots), and the 16-byte loads that bring one chunk
+inline size_t mixed_stream_lds_bytes(int nk, int vt, int wg_ut)
+{
+    using namespace mixed_detail;
+    const size_t ring = 16 * wg_ut + 2 * kStreamRows, cols = 16 * (vt + nk - 1) + 8;
+    return (ring * cols + (size_t)2 * kStreamRows * 2 * (16 * nk + 16)) * sizeof(uint16_t);
+}
+inline int mixed_stream_items(int nk, int vt)
+{
+    return mixed_detail::kStreamRows * (2 * (vt + nk - 1) + 2 * nk + 2);
 }
 
 template <int NK>
@@ -302,6 +516,36 @@ hipError_t launch_correlate_mixed_nk(hipStream_t s, const MixedShape &sh, const 
             else AMPCOR_MIXED_ONE(VTV, false, false);                                              \
         }                                                                                          \
     } while (0)
+    if (sh.stream) {
+        for (int first = 0; first < pairs; first += chunk) {
+            const int n = std::min(chunk, pairs - first);
+            const dim3 grid((unsigned)(n * per));
+            const uint16_t *cq = chip_q + (size_t)first * C * sh.cs;
+            const uint16_t *tq = tgt_q + (size_t)first * W * W;
+            float *pt = partial + (size_t)first * P * P;
+#define AMPCOR_STREAM(VTV)                                                                         \
+    case VTV:                                                                                      \
+        if (fp16)                                                                                  \
+            hipLaunchKernelGGL((k_correlate_stream<NK, VTV, true>), grid, block, sh.lds_bytes, s,   \
+                               cq, tq, C, W, P, sh.wg_ut, sh.tiles_u, sh.tiles_v, pt);             \
+        else                                                                                       \
+            hipLaunchKernelGGL((k_correlate_stream<NK, VTV, false>), grid, block, sh.lds_bytes, s,  \
+                               cq, tq, C, W, P, sh.wg_ut, sh.tiles_u, sh.tiles_v, pt);             \
+        break
+            switch (sh.wg_vt) {
+                AMPCOR_STREAM(1);
+                AMPCOR_STREAM(2);
+                AMPCOR_STREAM(3);
+                AMPCOR_STREAM(4);
+                AMPCOR_STREAM(5);
+            default: return hipErrorInvalidValue;
+            }
+#undef AMPCOR_STREAM
+            const hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
     for (int first = 0; first < pairs; first += chunk) {
         const int n = std::min(chunk, pairs - first);
         const dim3 grid((unsigned)(n * per));

@@ -311,6 +311,12 @@ int plan_configure(ampcor_hip_plan *p, int device, size_t pairs, size_t ref_line
     if (p->mixed) {
         p->mixed_coarse = choose_mixed_shape(p->C, p->P);
         p->mixed_refined = choose_mixed_shape(p->Cr, p->R, 0, p->split);
+        // the refined contraction streams its chip rows where it can (one
+        // partial surface per pair); the autotuner weighs both forms
+        if (!p->split) {
+            const MixedShape st = choose_mixed_stream_shape(p->Cr, p->R);
+            if (st.nk) p->mixed_refined = st;
+        }
     }
     if (p->fp8) p->fp8_coarse = choose_fp8_shape(p->C, p->P);
     return AMPCOR_HIP_OK;
@@ -1114,6 +1120,18 @@ int ampcor_hip_plan_autotune(ampcor_hip_plan *plan, int reps)
             if (plan->fp8 && st.shape == &plan->mixed_coarse) continue;   // tuned below
             float best = 1e30f;
             MixedShape pick = *st.shape;
+            const bool coarse = st.shape == &plan->mixed_coarse;
+            auto consider = [&](const MixedShape &sh) {
+                float ms;
+                if ((rc = timeit([&] {
+                         hipError_t e = launch_correlate_mixed(s, sh, st.cq, st.tq, N, st.C, st.W,
+                                                               fp16, st.partial, st.cl, st.tl);
+                         if (e != hipSuccess) return e;
+                         return coarse ? normalize_coarse(sh.nsplit) : normalize_refined(sh.nsplit);
+                     }, &ms)))
+                    return;
+                if (ms < best) best = ms, pick = sh;
+            };
             // v-tiles per workgroup: the widest, and 3 (which divides some tile
             // counts more evenly, e.g. 9 = 3 x 3 rather than 5 + 5)
             for (int vt : {0, 3})
@@ -1121,16 +1139,15 @@ int ampcor_hip_plan_autotune(ampcor_hip_plan *plan, int reps)
                 MixedShape sh = choose_mixed_shape(st.C, st.P, ns, st.split, vt);
                 if (vt && sh.wg_vt == choose_mixed_shape(st.C, st.P, ns, st.split).wg_vt) continue;
                 if (sh.nsplit > st.maxsplit || sh.lds_bytes > 150 * 1024) continue;
-                float ms;
-                const bool coarse = st.shape == &plan->mixed_coarse;
-                if ((rc = timeit([&] {
-                         hipError_t e = launch_correlate_mixed(s, sh, st.cq, st.tq, N, st.C, st.W,
-                                                               fp16, st.partial, st.cl, st.tl);
-                         if (e != hipSuccess) return e;
-                         return coarse ? normalize_coarse(sh.nsplit) : normalize_refined(sh.nsplit);
-                     }, &ms)))
-                    break;
-                if (ms < best) best = ms, pick = sh;
+                consider(sh);
+            }
+            // the streaming form (plain operands): all chip rows per workgroup
+            for (int vt : {0, 3}) {
+                if (st.split || rc) break;
+                MixedShape sh = choose_mixed_stream_shape(st.C, st.P, vt);
+                if (!sh.nk || (vt && sh.wg_vt == choose_mixed_stream_shape(st.C, st.P).wg_vt))
+                    continue;
+                consider(sh);
             }
             *st.shape = pick;
         }

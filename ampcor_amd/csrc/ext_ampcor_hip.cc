@@ -14,8 +14,8 @@
 // raster(buffer, lines, samples[, dtype, device]), slc_pixelSize(),
 // slc_getitem(raster, offset); the CPU module's refine(worker); and the
 // batched and product accessors: addPairs, products, dims, stage_gamma,
-// stage_gamma_refined, stage_locations, stage_upsample, stage_zoom, zoom_stats, fp8_guard,
-// set_timing, stage_times.
+// stage_gamma_refined, stage_locations, stage_upsample, stage_contract, stage_zoom, zoom_stats,
+// fp8_guard, set_timing, stage_times.
 //
 // Unlike the reference, adjust releases the GIL (a worker busy in another
 // thread raises RuntimeError rather than racing it), and failures raise
@@ -577,6 +577,57 @@ PyObject *stage_upsample(PyObject *, PyObject *args, PyObject *kwds)
     return out;
 }
 
+// stage_contract(worker, stage, form, chip, c, window, w, count) -> bytes: one
+// contraction of a plain BF16 / FP16 worker on 16-bit operand patterns
+// chip [count][c][c], window [count][w][w]; returns float32 numerators
+// [count][p][p], p = w - c + 1 (form: 0 current, 1 tiles, 2 streaming, 3 packed)
+PyObject *stage_contract(PyObject *, PyObject *args)
+{
+    PyObject *pyWorker;
+    int stage, form, c, wd;
+    Py_buffer chip, win;
+    unsigned long count;
+    if (!PyArg_ParseTuple(args, "O!iiy*iy*ik:stage_contract", &PyCapsule_Type, &pyWorker, &stage,
+                          &form, &chip, &c, &win, &wd, &count))
+        return nullptr;
+    Worker *w = get_worker(pyWorker);
+    auto release = [&] {
+        PyBuffer_Release(&chip);
+        PyBuffer_Release(&win);
+    };
+    if (!w) {
+        release();
+        return nullptr;
+    }
+    if (c <= 0 || wd < c || (size_t)chip.len != 2 * count * (size_t)c * c ||
+        (size_t)win.len != 2 * count * (size_t)wd * wd) {
+        release();
+        PyErr_SetString(PyExc_ValueError, "chip / window buffers do not hold count x c^2 / w^2 "
+                                          "16-bit values");
+        return nullptr;
+    }
+    const size_t p = (size_t)(wd - c + 1);
+    PyObject *out = bytes_of(4 * count * p * p);
+    if (!out) {
+        release();
+        return nullptr;
+    }
+    int rc;
+    w->busy = true;
+    Py_BEGIN_ALLOW_THREADS
+    rc = ampcor_hip_stage_contract(w->plan, stage, form, static_cast<const uint16_t *>(chip.buf), c,
+                                   static_cast<const uint16_t *>(win.buf), wd, count,
+                                   reinterpret_cast<float *>(PyBytes_AS_STRING(out)));
+    Py_END_ALLOW_THREADS
+    w->busy = false;
+    release();
+    if (rc) {
+        Py_DECREF(out);
+        return raise(rc);
+    }
+    return out;
+}
+
 // stage_zoom(worker, surfaces): the zoom stage alone on float32 refined
 // surfaces [pairs][R][R]; products() and stage_locations() then read its results
 PyObject *stage_zoom(PyObject *, PyObject *args)
@@ -860,6 +911,7 @@ PyMethodDef methods[] = {
     {"stage_zoom", stage_zoom, METH_VARARGS, "run the zoom stage on given refined surfaces"},
     {"stage_upsample", (PyCFunction)(void (*)(void))stage_upsample, METH_VARARGS | METH_KEYWORDS,
      "run the target upsampling on given tiles"},
+    {"stage_contract", stage_contract, METH_VARARGS, "run one contraction on given operands"},
     {"zoom_stats", zoom_stats, METH_VARARGS, "the zoom screen's exact re-evaluations"},
     {"set_timing", set_timing, METH_VARARGS, "enable per-stage device timing"},
     {"stage_times", stage_times, METH_VARARGS, "per-stage device milliseconds of the last adjust"},

@@ -142,6 +142,10 @@ struct MixedShape {
     int wg_ut, wg_vt, tiles_u, tiles_v, nsplit;
     size_t lds_bytes;
     bool stream;   // the streaming form: all chip rows per workgroup, staged chunk by chunk
+    // the packed form (P = 32 NB + 1): chip rows per chunk, 0 otherwise; one
+    // workgroup per pair of wg_ut = NB x tiles_v waves, each wg_vt v-tiles wide
+    int packed;
+    int wpe;   // packed form: waves per SIMD its registers are held to (2 or 3)
 };
 // nsplit > 0 asks for at least that many chip-row ranges (tuning)
 // split: hi/lo operands (three products per pair of elements), twice the LDS
@@ -149,6 +153,9 @@ struct MixedShape {
 MixedShape choose_mixed_shape(int C, int P, int nsplit = 0, bool split = false, int max_vt = 0);
 // the streaming form's shape (plain operands only); nk = 0 where it does not apply
 MixedShape choose_mixed_stream_shape(int C, int P, int max_vt = 0);
+// the packed form's shape (plain operands, P = 32 NB + 1, chips of whole
+// 16-column blocks); nk = 0 where it does not apply
+MixedShape choose_mixed_packed_shape(int C, int P, int max_vt = 0, int wpe = 0);
 // chip_lo / tgt_lo (optional): the rounding remainders, for the split form
 hipError_t launch_prep_mixed(hipStream_t s, const float *ref_amp, const float *tgt_amp, int pairs,
                              int C, int W, int CS, bool fp16, uint16_t *chip_q, uint16_t *tgt_q,

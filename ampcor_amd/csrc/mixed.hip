@@ -15,6 +15,7 @@
 // Replaces the reference's per-placement launches of correlate.cu:38-156 with
 // one launch over all pairs; no counterpart exists for the precision modes.
 #include "mixed_correlate.h"
+#include "packed_correlate.h"
 
 #include <math.h>
 
@@ -252,6 +253,51 @@ MixedShape choose_mixed_stream_shape(int C, int P, int max_vt)
     return s;
 }
 
+MixedShape choose_mixed_packed_shape(int C, int P, int max_vt, int wpe)
+{
+    MixedShape s{};
+    const int nk = (C + 15 + 15) / 16;
+    if (P < 33 || (P - 1) % 32 || C % 16 || nk > 11) return s;
+    const int nb = (P - 1) / 32, vtt = 2 * nb + 1;
+    // VGPRs of a wave: two tiles' fragments (8 per fragment), accumulators
+    // (8 per v-tile) and about 40 for the chip fragments, the packed row's
+    // loads, the refill and addresses (measured on the compiled kernels)
+    auto vgprs = [&](int vt) { return 16 * vt + 8 * nk + 40; };
+    // the instantiated (v-tiles, chunk rows, waves per SIMD) combinations
+    static const int kinds[][3] = {{2, 4, 3}, {3, 4, 3}, {3, 4, 2}, {5, 4, 2}, {2, 8, 3},
+                                   {3, 8, 3}, {3, 8, 2}, {5, 8, 3}, {5, 8, 2}};
+    int best = -1, best_cost = 1 << 30;
+    for (int k = 0; k < (int)(sizeof(kinds) / sizeof(kinds[0])); ++k) {
+        const int vt = kinds[k][0], ch = kinds[k][1], w = kinds[k][2];
+        if ((max_vt > 0 && vt > max_vt) || (wpe && w != wpe)) continue;
+        if (vgprs(vt) > (w == 3 ? 168 : 256)) continue;
+        const int nvg = (vtt + vt - 1) / vt, waves = nb * nvg;
+        if (waves > packed_detail::kMaxWaves || C % ch) continue;
+        // one-wave workgroups refill four chip rows at a time (a smaller LDS
+        // footprint, more workgroups per CU); larger ones eight
+        if (ch != (waves == 1 ? 4 : 8)) continue;
+        if (!packed_refill_fits(nk, vt, ch, nb, nvg)) continue;
+        // fewest wasted v-tiles, then the higher occupancy, then fewer waves
+        const int cost = 1000 * (nvg * vt - vtt) + 100 * (3 - w) + waves;
+        if (cost < best_cost) best_cost = cost, best = k;
+    }
+    if (best < 0) return s;
+    const int vt = kinds[best][0], nvg = (vtt + vt - 1) / vt;
+    s.nk = nk;
+    s.cs = 16 * nk + 16;
+    s.split = false;
+    s.wg_ut = nb * nvg;
+    s.wg_vt = vt;
+    s.tiles_u = nb;
+    s.tiles_v = nvg;
+    s.nsplit = 1;
+    s.packed = kinds[best][1];
+    s.wpe = kinds[best][2];
+    s.lds_bytes = packed_lds_bytes(nk, vt, s.packed, nb, nvg);
+    s.stream = false;
+    return s;
+}
+
 hipError_t launch_prep_mixed(hipStream_t s, const float *ref_amp, const float *tgt_amp, int pairs,
                              int C, int W, int CS, bool fp16, uint16_t *chip_q, uint16_t *tgt_q,
                              MixedStats *stats, uint16_t *chip_lo, uint16_t *tgt_lo)
@@ -277,6 +323,24 @@ hipError_t launch_correlate_mixed(hipStream_t s, const MixedShape &sh, const uin
          mixed_stream_items(sh.nk, sh.wg_vt) > mixed_detail::kStreamItems * 64 * sh.wg_ut ||
          sh.lds_bytes != mixed_stream_lds_bytes(sh.nk, sh.wg_vt, sh.wg_ut)))
         return hipErrorInvalidValue;
+    if (sh.packed) {
+        const int P = W - C + 1;
+        if (sh.split || sh.stream || sh.nsplit != 1 || P < 33 || (P - 1) % 32 || C % 16 ||
+            C % sh.packed || sh.tiles_u != (P - 1) / 32 || sh.wg_ut != sh.tiles_u * sh.tiles_v ||
+            sh.wg_ut > packed_detail::kMaxWaves || (sh.wpe != 2 && sh.wpe != 3) ||
+            sh.tiles_v * sh.wg_vt < (P + 15) / 16 ||
+            !packed_refill_fits(sh.nk, sh.wg_vt, sh.packed, sh.tiles_u, sh.tiles_v) ||
+            sh.lds_bytes != packed_lds_bytes(sh.nk, sh.wg_vt, sh.packed, sh.tiles_u, sh.tiles_v))
+            return hipErrorInvalidValue;
+#define AMPCOR_PK(N)                                                                               \
+    case N: return launch_correlate_packed_nk<N>(s, sh, chip_q, tgt_q, pairs, C, W, fp16, partial)
+        switch (sh.nk) {
+            AMPCOR_PK(2); AMPCOR_PK(3); AMPCOR_PK(4); AMPCOR_PK(5); AMPCOR_PK(6); AMPCOR_PK(7);
+            AMPCOR_PK(8); AMPCOR_PK(9); AMPCOR_PK(10); AMPCOR_PK(11);
+        default: return hipErrorInvalidValue;
+        }
+#undef AMPCOR_PK
+    }
 #define AMPCOR_NK(N)                                                                               \
     case N:                                                                                        \
         return launch_correlate_mixed_nk<N>(s, sh, chip_q, tgt_q, pairs, C, W, fp16, partial,      \

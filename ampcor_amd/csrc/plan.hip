@@ -311,11 +311,17 @@ int plan_configure(ampcor_hip_plan *p, int device, size_t pairs, size_t ref_line
     if (p->mixed) {
         p->mixed_coarse = choose_mixed_shape(p->C, p->P);
         p->mixed_refined = choose_mixed_shape(p->Cr, p->R, 0, p->split);
-        // the refined contraction streams its chip rows where it can (one
-        // partial surface per pair); the autotuner weighs both forms
+        // plain operands: the packed form where the surface has 32 NB + 1
+        // placements per axis (no padded placement rows), else the refined
+        // contraction streams its chip rows where it can (one partial surface
+        // per pair); the autotuner weighs every form
         if (!p->split) {
+            const MixedShape pc = choose_mixed_packed_shape(p->C, p->P);
+            if (pc.nk) p->mixed_coarse = pc;
+            const MixedShape pr = choose_mixed_packed_shape(p->Cr, p->R);
             const MixedShape st = choose_mixed_stream_shape(p->Cr, p->R);
-            if (st.nk) p->mixed_refined = st;
+            if (pr.nk) p->mixed_refined = pr;
+            else if (st.nk) p->mixed_refined = st;
         }
     }
     if (p->fp8) p->fp8_coarse = choose_fp8_shape(p->C, p->P);
@@ -947,6 +953,72 @@ int ampcor_hip_stage_upsample(ampcor_hip_plan *plan, int dtype, const void *tile
     return rc;
 }
 
+int ampcor_hip_stage_contract(ampcor_hip_plan *plan, int stage, int form, const uint16_t *chip,
+                              int c, const uint16_t *window, int w, size_t count,
+                              float *numerators)
+{
+    if (!plan || !chip || !window || !numerators) return fail(AMPCOR_HIP_EINVAL, "null argument");
+    if (!plan->mixed || plan->split || plan->fp8)
+        return fail(AMPCOR_HIP_EINVAL, "stage_contract runs the plain BF16 / FP16 forms");
+    if (stage != 0 && stage != 1) return fail(AMPCOR_HIP_EINVAL, "stage must be 0 or 1");
+    if (count == 0 || count > plan->pairs)
+        return fail(AMPCOR_HIP_EINVAL, "count must be in [1, pairs]");
+    const int C = stage ? plan->Cr : plan->C, W = stage ? plan->Wr : plan->W, P = W - C + 1;
+    if (c != C || w != W) return fail(AMPCOR_HIP_EINVAL, "chip / window sizes are not the stage's");
+    const MixedShape cur = stage ? plan->mixed_refined : plan->mixed_coarse;
+    MixedShape sh{};
+    switch (form) {
+    case 0: sh = cur; break;
+    case 1: sh = choose_mixed_shape(C, P); break;
+    case 2: sh = choose_mixed_stream_shape(C, P); break;
+    case 3: sh = choose_mixed_packed_shape(C, P); break;
+    default: return fail(AMPCOR_HIP_EINVAL, "form must be 0 .. 3");
+    }
+    if (!sh.nk) return fail(AMPCOR_HIP_EINVAL, "the form does not apply to this geometry");
+    // the plan's buffers hold the partials of at most its own splits
+    const size_t maxsplit = stage ? kTuneMaxSplitRefined : kTuneMaxSplitCoarse;
+    if ((size_t)sh.nsplit > std::max<size_t>(maxsplit, cur.nsplit))
+        return fail(AMPCOR_HIP_EINVAL, "the form's chip-row split exceeds the plan's buffers");
+    DeviceGuard guard(plan->device);
+    hipStream_t s = plan->stream;
+    uint16_t *cq = stage ? plan->chip_q_r.ptr : plan->chip_q.ptr;
+    uint16_t *tq = stage ? plan->tgt_q_r.ptr : plan->tgt_q.ptr;
+    float *partial = stage ? plan->partial_r.ptr : plan->partial.ptr;
+    // chip rows in the contraction's layout: 16 zeros, the C values, zeros to cs
+    std::vector<uint16_t> hc(count * C * (size_t)sh.cs, 0);
+    for (size_t k = 0; k < count; ++k)
+        for (int i = 0; i < C; ++i)
+            std::memcpy(hc.data() + (k * C + i) * sh.cs + 16, chip + (k * C + i) * C,
+                        C * sizeof(uint16_t));
+    const size_t pp = (size_t)P * P;
+    std::vector<float> part(count * sh.nsplit * pp);
+    int rc = AMPCOR_HIP_OK;
+    hipError_t e = hipMemcpyAsync(cq, hc.data(), hc.size() * sizeof(uint16_t),
+                                  hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(tq, window, count * (size_t)W * W * sizeof(uint16_t),
+                           hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = launch_correlate_mixed(s, sh, cq, tq, (int)count, C, W, plan->fp16, partial);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(part.data(), partial, part.size() * sizeof(float),
+                           hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) rc = hip_fail(e, "running the contraction");
+    (void)hipStreamSynchronize(s);
+    if (rc == AMPCOR_HIP_OK) {
+        // chip-row ranges in order, as the normalizers sum them
+        for (size_t k = 0; k < count; ++k)
+            for (size_t q = 0; q < pp; ++q) {
+                float acc = part[(k * sh.nsplit) * pp + q];
+                for (int r = 1; r < sh.nsplit; ++r) acc += part[(k * sh.nsplit + r) * pp + q];
+                numerators[k * pp + q] = acc;
+            }
+    }
+    plan->adjusted = false;
+    return rc;
+}
+
 int ampcor_hip_stage_zoom(ampcor_hip_plan *plan, const float *surfaces)
 {
     if (!plan || !surfaces) return fail(AMPCOR_HIP_EINVAL, "null argument");
@@ -1148,6 +1220,12 @@ int ampcor_hip_plan_autotune(ampcor_hip_plan *plan, int reps)
                 if (!sh.nk || (vt && sh.wg_vt == choose_mixed_stream_shape(st.C, st.P).wg_vt))
                     continue;
                 consider(sh);
+            }
+            // the packed form (plain operands, P = 32 NB + 1) at both register budgets
+            for (int wpe : {3, 2}) {
+                if (st.split || rc) break;
+                MixedShape sh = choose_mixed_packed_shape(st.C, st.P, 0, wpe);
+                if (sh.nk) consider(sh);
             }
             *st.shape = pick;
         }

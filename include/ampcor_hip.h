@@ -154,6 +154,19 @@ int ampcor_hip_stage_locations(ampcor_hip_plan *plan, int32_t *coarse, int32_t *
  * intermediate buffers (the next adjust re-registers the pairs). */
 int ampcor_hip_stage_upsample(ampcor_hip_plan *plan, int dtype, const void *tiles, size_t count,
                               float *amp);
+/* run one contraction of a plain BF16 / FP16 plan alone on host operands:
+ * stage 0 the coarse one (chip C, window W), 1 the refined one (rf C, rf E);
+ * chip [count][c][c] and window [count][w][w] hold the operands' 16-bit
+ * patterns (bfloat16 or binary16, as the plan's form), count <= pairs; c and
+ * w must be the stage's chip and window sizes.  numerators receives
+ * sum_ij window[u + i][v + j] chip[i][j] [count][p][p] (p = w - c + 1) as the
+ * form's tiling computes it: form 0 the plan's current tiling, 1 the 16 x 16
+ * tiles (chip-row ranges summed in order), 2 the streaming form, 3 the packed
+ * form (p = 32 n + 1); AMPCOR_HIP_EINVAL where the form does not apply.
+ * Overwrites the last adjust's intermediate buffers. */
+int ampcor_hip_stage_contract(ampcor_hip_plan *plan, int stage, int form, const uint16_t *chip,
+                              int c, const uint16_t *window, int w, size_t count,
+                              float *numerators);
 /* run the zoom stage alone on host refined surfaces [pairs][R][R] (the
  * reference drives _zoomcor the same way in its tests): the products then hold
  * its results -- refined peak, zoomed location (stage_locations), and offsets

@@ -34,6 +34,8 @@
 
 #include "mixed_correlate.h"
 
+#include <type_traits>
+
 namespace ampcor_hip {
 
 namespace packed_detail {
@@ -43,6 +45,7 @@ using mixed_detail::mfma32;
 constexpr int kWave = 64;
 constexpr int kMaxWaves = 12;   // at most three waves per SIMD (168 VGPRs)
 constexpr int kRefill = 2;  // wave-wide refill loads per step, at most
+constexpr int kShift = 1;   // fragments shifted (and their lane-15 rows loaded) together
 // dword 3 of a raw buffer resource on gfx9 targets (bounds-checked loads, zeros
 // past num_records)
 constexpr int kBufferWord3 = 0x00020000;
@@ -164,9 +167,9 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, WPE == 3 ? 768 : 512),
     // the rows entering lane 15 of each tile, and the packed row's fragments
     // (lanes m < VT of the last u-block's waves: fragment m + kappa)
     const bool in15 = lm == 15;
-    const bool inE = edge && lm < VT;
     const uint16_t *rb = ring + (size_t)b * RR * TS + col0 + 8 * h;
-    const uint16_t *eb = rb + 16 * lm;
+    // (lanes m >= VT load fragment VT - 1 + kappa as well: their M-rows are not kept)
+    const uint16_t *eb = rb + 16 * min(lm, VT - 1);
 
     // the refill of the next chunk is spread over the steps: batch s of its
     // loads is issued before step s and stored after it (the slots it fills are
@@ -214,35 +217,48 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, WPE == 3 ? 768 : 512),
             auto ldB = [&](int kap) {
                 return make_uint4(cr[8 * kap], cr[8 * kap + 1], cr[8 * kap + 2], cr[8 * kap + 3]);
             };
-            uint4 B[NK];
-            B[0] = ldB(0);
+            // the step's K-steps; EG: this wave holds the packed row too
+            auto body = [&](auto edge_tag) {
+                constexpr bool EG = decltype(edge_tag)::value;
+                uint4 B[NK];
+                B[0] = ldB(0);
 #pragma unroll
-            for (int kap = 0; kap < NF; ++kap) {
-                if (kap + 1 < NK) B[kap + 1] = ldB(kap + 1);
-                // the packed row's fragment: loaded at the start of the K-step and
-                // used after the step's other MFMAs have covered its latency
-                uint4 E;
-                if (edge && kap < NK) {
-                    if (inE) E = *reinterpret_cast<const uint4 *>(e0p + 16 * kap);
-                }
-                if (kap < NK) {
+                for (int kap = 0; kap < NF; ++kap) {
+                    if (kap + 1 < NK) B[kap + 1] = ldB(kap + 1);
+                    if (kap < NK) {
+                        // the packed row's fragment, loaded at the start of the
+                        // K-step; its MFMA goes last
+                        uint4 E;
+                        if (edge) E = *reinterpret_cast<const uint4 *>(e0p + 16 * kap);
 #pragma unroll
-                    for (int v = 0; v < VT; ++v) {
-                        acc0[v] = mfma32<FP16>(A0[v + kap], B[kap], acc0[v]);
-                        acc1[v] = mfma32<FP16>(A1[v + kap], B[kap], acc1[v]);
+                        for (int v = 0; v < VT; ++v) {
+                            acc0[v] = mfma32<FP16>(A0[v + kap], B[kap], acc0[v]);
+                            acc1[v] = mfma32<FP16>(A1[v + kap], B[kap], acc1[v]);
+                        }
+                        if (edge) accE = mfma32<FP16>(E, B[kap], accE);
                     }
-                    if (edge) accE = mfma32<FP16>(E, B[kap], accE);
+                    // fragments that have served their last K-step of this step,
+                    // kShift at a time: lanes 0..14 take the next M-row's target
+                    // row, lane 15 the new row
+                    if (kap % kShift == kShift - 1 || kap == NF - 1) {
+                        const int f0 = kap - kap % kShift;
+#pragma unroll
+                        for (int f = f0; f <= kap; ++f) {
+                            A0[f] = shift_up1(A0[f], A0[f]);
+                            A1[f] = shift_up1(A1[f], A1[f]);
+                        }
+                        if (in15) {
+#pragma unroll
+                            for (int f = f0; f <= kap; ++f) {
+                                A0[f] = *reinterpret_cast<const uint4 *>(x0p + 16 * f);
+                                A1[f] = *reinterpret_cast<const uint4 *>(x1p + 16 * f);
+                            }
+                        }
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
                 }
-                // fragment kap has served its last K-step of this step: lanes
-                // 0..14 take the next M-row's target row, lane 15 the new row
-                A0[kap] = shift_up1(A0[kap], A0[kap]);
-                A1[kap] = shift_up1(A1[kap], A1[kap]);
-                if (in15) {
-                    A0[kap] = *reinterpret_cast<const uint4 *>(x0p + 16 * kap);
-                    A1[kap] = *reinterpret_cast<const uint4 *>(x1p + 16 * kap);
-                }
-                __builtin_amdgcn_sched_barrier(0);
-            }
+            };
+            body(std::true_type{});
 #pragma unroll
             for (int j = 0; j < KB; ++j) {
                 const int p2 = (wave + (s * KB + j) * nw) * RPI;

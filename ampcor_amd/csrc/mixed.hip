@@ -253,7 +253,7 @@ MixedShape choose_mixed_stream_shape(int C, int P, int max_vt)
     return s;
 }
 
-MixedShape choose_mixed_packed_shape(int C, int P, int max_vt, int wpe)
+MixedShape choose_mixed_packed_shape(int C, int P, int max_vt, int wpe, bool split)
 {
     MixedShape s{};
     const int nk = (C + 15 + 15) / 16;
@@ -285,7 +285,7 @@ MixedShape choose_mixed_packed_shape(int C, int P, int max_vt, int wpe)
     const int vt = kinds[best][0], nvg = (vtt + vt - 1) / vt;
     s.nk = nk;
     s.cs = 16 * nk + 16;
-    s.split = false;
+    s.split = split;
     s.wg_ut = nb * nvg;
     s.wg_vt = vt;
     s.tiles_u = nb;
@@ -325,7 +325,7 @@ hipError_t launch_correlate_mixed(hipStream_t s, const MixedShape &sh, const uin
         return hipErrorInvalidValue;
     if (sh.packed) {
         const int P = W - C + 1;
-        if (sh.split || sh.stream || sh.nsplit != 1 || P < 33 || (P - 1) % 32 || C % 16 ||
+        if (sh.stream || sh.nsplit != 1 || P < 33 || (P - 1) % 32 || C % 16 ||
             C % sh.packed || sh.tiles_u != (P - 1) / 32 || sh.wg_ut != sh.tiles_u * sh.tiles_v ||
             sh.wg_ut > packed_detail::kMaxWaves || (sh.wpe != 2 && sh.wpe != 3) ||
             sh.tiles_v * sh.wg_vt < (P + 15) / 16 ||
@@ -333,7 +333,9 @@ hipError_t launch_correlate_mixed(hipStream_t s, const MixedShape &sh, const uin
             sh.lds_bytes != packed_lds_bytes(sh.nk, sh.wg_vt, sh.packed, sh.tiles_u, sh.tiles_v))
             return hipErrorInvalidValue;
 #define AMPCOR_PK(N)                                                                               \
-    case N: return launch_correlate_packed_nk<N>(s, sh, chip_q, tgt_q, pairs, C, W, fp16, partial)
+    case N:                                                                                        \
+        return launch_correlate_packed_nk<N>(s, sh, chip_q, tgt_q, pairs, C, W, fp16, partial,     \
+                                             chip_lo, tgt_lo)
         switch (sh.nk) {
             AMPCOR_PK(2); AMPCOR_PK(3); AMPCOR_PK(4); AMPCOR_PK(5); AMPCOR_PK(6); AMPCOR_PK(7);
             AMPCOR_PK(8); AMPCOR_PK(9); AMPCOR_PK(10); AMPCOR_PK(11);

@@ -20,14 +20,18 @@ template hipError_t launch_correlate_mixed_nk<5>(hipStream_t, const MixedShape &
                                                 const uint16_t *, const uint16_t *);
 template hipError_t launch_correlate_packed_nk<2>(hipStream_t, const MixedShape &,
                                                   const uint16_t *, const uint16_t *, int, int, int,
-                                                  bool, float *);
+                                                  bool, float *, const uint16_t *,
+                                                  const uint16_t *);
 template hipError_t launch_correlate_packed_nk<3>(hipStream_t, const MixedShape &,
                                                   const uint16_t *, const uint16_t *, int, int, int,
-                                                  bool, float *);
+                                                  bool, float *, const uint16_t *,
+                                                  const uint16_t *);
 template hipError_t launch_correlate_packed_nk<4>(hipStream_t, const MixedShape &,
                                                   const uint16_t *, const uint16_t *, int, int, int,
-                                                  bool, float *);
+                                                  bool, float *, const uint16_t *,
+                                                  const uint16_t *);
 template hipError_t launch_correlate_packed_nk<5>(hipStream_t, const MixedShape &,
                                                   const uint16_t *, const uint16_t *, int, int, int,
-                                                  bool, float *);
+                                                  bool, float *, const uint16_t *,
+                                                  const uint16_t *);
 }  // namespace ampcor_hip

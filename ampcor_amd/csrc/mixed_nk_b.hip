@@ -20,14 +20,18 @@ template hipError_t launch_correlate_mixed_nk<9>(hipStream_t, const MixedShape &
                                                 const uint16_t *, const uint16_t *);
 template hipError_t launch_correlate_packed_nk<6>(hipStream_t, const MixedShape &,
                                                   const uint16_t *, const uint16_t *, int, int, int,
-                                                  bool, float *);
+                                                  bool, float *, const uint16_t *,
+                                                  const uint16_t *);
 template hipError_t launch_correlate_packed_nk<7>(hipStream_t, const MixedShape &,
                                                   const uint16_t *, const uint16_t *, int, int, int,
-                                                  bool, float *);
+                                                  bool, float *, const uint16_t *,
+                                                  const uint16_t *);
 template hipError_t launch_correlate_packed_nk<8>(hipStream_t, const MixedShape &,
                                                   const uint16_t *, const uint16_t *, int, int, int,
-                                                  bool, float *);
+                                                  bool, float *, const uint16_t *,
+                                                  const uint16_t *);
 template hipError_t launch_correlate_packed_nk<9>(hipStream_t, const MixedShape &,
                                                   const uint16_t *, const uint16_t *, int, int, int,
-                                                  bool, float *);
+                                                  bool, float *, const uint16_t *,
+                                                  const uint16_t *);
 }  // namespace ampcor_hip

@@ -20,8 +20,10 @@ template hipError_t launch_correlate_mixed_nk<13>(hipStream_t, const MixedShape 
                                                 const uint16_t *, const uint16_t *);
 template hipError_t launch_correlate_packed_nk<10>(hipStream_t, const MixedShape &,
                                                   const uint16_t *, const uint16_t *, int, int, int,
-                                                  bool, float *);
+                                                  bool, float *, const uint16_t *,
+                                                  const uint16_t *);
 template hipError_t launch_correlate_packed_nk<11>(hipStream_t, const MixedShape &,
                                                   const uint16_t *, const uint16_t *, int, int, int,
-                                                  bool, float *);
+                                                  bool, float *, const uint16_t *,
+                                                  const uint16_t *);
 }  // namespace ampcor_hip

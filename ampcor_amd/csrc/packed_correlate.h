@@ -76,7 +76,7 @@ template <int NK, int VT, int CH, int WPE, bool FP16>
 __global__ __attribute__((amdgpu_flat_work_group_size(64, WPE == 3 ? 768 : 512),
                           amdgpu_waves_per_eu(WPE, WPE))) void k_correlate_packed(
     const uint16_t *__restrict__ chip_q, const uint16_t *__restrict__ tgt_q, int C, int W, int P,
-    int nvg, float *__restrict__ partial)
+    int nvg, float *__restrict__ partial, int accumulate)
 {
     constexpr int NF = VT + NK - 1;          // target fragments per tile
     constexpr int CS = 16 * NK + 16;         // chip row: 16 zeros, C values, zeros
@@ -286,8 +286,10 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, WPE == 3 ? 768 : 512),
         base = nbase;
     }
 
-    // ---- outputs: lane (n, lk) holds M-rows 4 lk + e of column n ---------------
+    // ---- outputs: lane (n, lk) holds M-rows 4 lk + e of column n; with
+    // accumulate set they are added to what the surface holds ----------------
     float *dst = partial + (size_t)pair * P * P;
+    auto put = [&](size_t i, float x) { dst[i] = accumulate ? dst[i] + x : x; };
 #pragma unroll
     for (int v = 0; v < VT; ++v) {
         const int vv = col0 + 16 * v + lm;
@@ -295,15 +297,15 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, WPE == 3 ? 768 : 512),
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const int u0 = 32 * b + 2 * (4 * lk + e);
-            dst[(size_t)u0 * P + vv] = acc0[v][e];
-            dst[(size_t)(u0 + 1) * P + vv] = acc1[v][e];
+            put((size_t)u0 * P + vv, acc0[v][e]);
+            put((size_t)(u0 + 1) * P + vv, acc1[v][e]);
         }
     }
     if (edge) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const int m = 4 * lk + e, vv = col0 + 16 * m + lm;
-            if (m < VT && vv < P) dst[(size_t)(P - 1) * P + vv] = accE[e];
+            if (m < VT && vv < P) put((size_t)(P - 1) * P + vv, accE[e]);
         }
     }
 }
@@ -328,30 +330,36 @@ inline bool packed_refill_fits(int nk, int vt, int ch, int nb, int nvg)
     return (pairs + waves - 1) / waves <= packed_detail::kRefill * (ch / 2);
 }
 
+// split shapes (hi / lo operands) run three passes over the same tiling:
+// hi x hi written, then hi x lo and lo x hi added
 template <int NK>
 hipError_t launch_correlate_packed_nk(hipStream_t s, const MixedShape &sh, const uint16_t *chip_q,
                                       const uint16_t *tgt_q, int pairs, int C, int W, bool fp16,
-                                      float *partial)
+                                      float *partial, const uint16_t *chip_lo,
+                                      const uint16_t *tgt_lo)
 #ifdef AMPCOR_MIXED_INSTANTIATE
 {
     using namespace packed_detail;
     const int P = W - C + 1;
     const dim3 block(64 * sh.wg_ut);
+    const int passes = sh.split ? 3 : 1;
     // one workgroup per pair: at most 2^31 - 1 of them per launch
     const int chunk = std::min(pairs, 0x7fffffff);
+    for (int pass = 0; pass < passes; ++pass)
     for (int first = 0; first < pairs; first += chunk) {
         const int n = std::min(chunk, pairs - first);
-        const uint16_t *cq = chip_q + (size_t)first * C * sh.cs;
-        const uint16_t *tq = tgt_q + (size_t)first * W * W;
+        const uint16_t *cq = (pass == 2 ? chip_lo : chip_q) + (size_t)first * C * sh.cs;
+        const uint16_t *tq = (pass == 1 ? tgt_lo : tgt_q) + (size_t)first * W * W;
         float *pt = partial + (size_t)first * P * P;
+        const int acc = pass > 0;
 #define AMPCOR_PACKED(VTV, CHV, WPEV)                                                              \
     if (sh.wg_vt == VTV && sh.packed == CHV && sh.wpe == WPEV) {                                   \
         if (fp16)                                                                                  \
             hipLaunchKernelGGL((k_correlate_packed<NK, VTV, CHV, WPEV, true>), dim3(n), block,     \
-                               sh.lds_bytes, s, cq, tq, C, W, P, sh.tiles_v, pt);                  \
+                               sh.lds_bytes, s, cq, tq, C, W, P, sh.tiles_v, pt, acc);             \
         else                                                                                       \
             hipLaunchKernelGGL((k_correlate_packed<NK, VTV, CHV, WPEV, false>), dim3(n), block,    \
-                               sh.lds_bytes, s, cq, tq, C, W, P, sh.tiles_v, pt);                  \
+                               sh.lds_bytes, s, cq, tq, C, W, P, sh.tiles_v, pt, acc);             \
     } else
         AMPCOR_PACKED(2, 4, 3)
         AMPCOR_PACKED(3, 4, 3)

@@ -311,18 +311,16 @@ int plan_configure(ampcor_hip_plan *p, int device, size_t pairs, size_t ref_line
     if (p->mixed) {
         p->mixed_coarse = choose_mixed_shape(p->C, p->P);
         p->mixed_refined = choose_mixed_shape(p->Cr, p->R, 0, p->split);
-        // plain operands: the packed form where the surface has 32 NB + 1
-        // placements per axis (no padded placement rows), else the refined
-        // contraction streams its chip rows where it can (one partial surface
-        // per pair); the autotuner weighs every form
-        if (!p->split) {
-            const MixedShape pc = choose_mixed_packed_shape(p->C, p->P);
-            if (pc.nk) p->mixed_coarse = pc;
-            const MixedShape pr = choose_mixed_packed_shape(p->Cr, p->R);
-            const MixedShape st = choose_mixed_stream_shape(p->Cr, p->R);
-            if (pr.nk) p->mixed_refined = pr;
-            else if (st.nk) p->mixed_refined = st;
-        }
+        // the packed form where the surface has 32 NB + 1 placements per axis
+        // (no padded placement rows; split operands in three passes), else the
+        // refined contraction streams its chip rows where it can (plain
+        // operands; one partial surface per pair); the autotuner weighs every form
+        const MixedShape pc = choose_mixed_packed_shape(p->C, p->P);
+        if (pc.nk) p->mixed_coarse = pc;
+        const MixedShape pr = choose_mixed_packed_shape(p->Cr, p->R, 0, 0, p->split);
+        const MixedShape st = p->split ? MixedShape{} : choose_mixed_stream_shape(p->Cr, p->R);
+        if (pr.nk) p->mixed_refined = pr;
+        else if (st.nk) p->mixed_refined = st;
     }
     if (p->fp8) p->fp8_coarse = choose_fp8_shape(p->C, p->P);
     return AMPCOR_HIP_OK;
@@ -1221,10 +1219,10 @@ int ampcor_hip_plan_autotune(ampcor_hip_plan *plan, int reps)
                     continue;
                 consider(sh);
             }
-            // the packed form (plain operands, P = 32 NB + 1) at both register budgets
+            // the packed form (P = 32 NB + 1) at both register budgets
             for (int wpe : {3, 2}) {
-                if (st.split || rc) break;
-                MixedShape sh = choose_mixed_packed_shape(st.C, st.P, 0, wpe);
+                if (rc) break;
+                MixedShape sh = choose_mixed_packed_shape(st.C, st.P, 0, wpe, st.split);
                 if (sh.nk) consider(sh);
             }
             *st.shape = pick;

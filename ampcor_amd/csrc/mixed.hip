@@ -272,7 +272,7 @@ MixedShape choose_mixed_packed_shape(int C, int P, int max_vt, int wpe, bool spl
         if ((max_vt > 0 && vt > max_vt) || (wpe && w != wpe)) continue;
         if (vgprs(vt) > (w == 3 ? 168 : 256)) continue;
         const int nvg = (vtt + vt - 1) / vt, waves = nb * nvg;
-        if (waves > packed_detail::kMaxWaves || C % ch) continue;
+        if (waves > std::min(packed_detail::kMaxWaves, 4 * w) || C % ch) continue;
         // one-wave workgroups refill four chip rows at a time (a smaller LDS
         // footprint, more workgroups per CU); larger ones eight
         if (ch != (waves == 1 ? 4 : 8)) continue;
@@ -327,7 +327,8 @@ hipError_t launch_correlate_mixed(hipStream_t s, const MixedShape &sh, const uin
         const int P = W - C + 1;
         if (sh.stream || sh.nsplit != 1 || P < 33 || (P - 1) % 32 || C % 16 ||
             C % sh.packed || sh.tiles_u != (P - 1) / 32 || sh.wg_ut != sh.tiles_u * sh.tiles_v ||
-            sh.wg_ut > packed_detail::kMaxWaves || (sh.wpe != 2 && sh.wpe != 3) ||
+            (sh.wpe != 2 && sh.wpe != 3) ||
+            sh.wg_ut > std::min(packed_detail::kMaxWaves, 4 * sh.wpe) ||
             sh.tiles_v * sh.wg_vt < (P + 15) / 16 ||
             !packed_refill_fits(sh.nk, sh.wg_vt, sh.packed, sh.tiles_u, sh.tiles_v) ||
             sh.lds_bytes != packed_lds_bytes(sh.nk, sh.wg_vt, sh.packed, sh.tiles_u, sh.tiles_v))

@@ -71,9 +71,10 @@ __host__ __device__ constexpr int packed_row_stride(int nk, int vt, int nvg)
 }
 
 // WPE: the occupancy the register allocation is held to (waves per SIMD: 3
-// leaves 168 VGPRs per lane, 2 leaves 256)
+// leaves 168 VGPRs per lane, 2 leaves 256); a workgroup then has at most
+// 4 WPE waves (one CU's four SIMDs)
 template <int NK, int VT, int CH, int WPE, bool FP16>
-__global__ __attribute__((amdgpu_flat_work_group_size(64, WPE == 3 ? 768 : 512),
+__global__ __attribute__((amdgpu_flat_work_group_size(64, 256 * WPE),
                           amdgpu_waves_per_eu(WPE, WPE))) void k_correlate_packed(
     const uint16_t *__restrict__ chip_q, const uint16_t *__restrict__ tgt_q, int C, int W, int P,
     int nvg, float *__restrict__ partial, int accumulate)

@@ -99,3 +99,24 @@ def test_packed_form_is_the_default_where_it_applies(lib):
         cur = lib.stage_contract(worker, stage, 0, cq, c, tq, w, n)
         packed = lib.stage_contract(worker, stage, 3, cq, c, tq, w, n)
         assert cur == packed
+
+
+def test_autotune_large_geometry(lib):
+    """Chip 128 / window 256 (a 129 x 129 surface: four u-blocks, several
+    v-groups) in FP16: every tiling the autotuner launches must fit its
+    kernel's workgroup limit, and the tuned plan gives the same offsets to
+    the forms' accumulation-order tolerance."""
+    from ampcor_amd import synth
+
+    s = synth.amplitude_scene(seed=3, size=1024, grid=(4, 4), max_shift=2.0)
+    worker = lib.sequential(s.pairs, (s.chip, s.chip), (s.window, s.window), s.refine_margin,
+                            s.refine_factor, s.zoom_factor, precision="fp16")
+    ref = lib.raster(s.reference, *s.reference.shape, dtype="float32")
+    tgt = lib.raster(s.target, *s.target.shape, dtype="float32")
+    lib.addPairs(worker, ref, tgt, 0, s.ref_begin.tobytes(), s.tgt_begin.tobytes())
+    before = np.array(lib.adjust(worker), dtype=np.float64)
+    lib.autotune(worker, 1)
+    after = np.array(lib.adjust(worker), dtype=np.float64)
+    print(f"{s.pairs} pairs: max |tuned - default| = {np.abs(after - before).max():.2e} px",
+          flush=True)
+    assert np.abs(after - before).max() < 1e-3

@@ -114,11 +114,29 @@ def cpu_baseline(scene, seconds):
             idx, exact, idx_t, twin)
 
 
+def covered_pixels(scene):
+    """Raster pixels the gathers must read at least: the union of the chips on
+    the reference and of the windows on the target (neighbouring windows
+    overlap, and each pixel need come from HBM once)."""
+    cached = getattr(scene, "_covered", None)
+    if cached is None:
+        def union(shape, begin, size):
+            mask = np.zeros(shape[:2], dtype=bool)
+            for line, sample in begin:
+                mask[line:line + size, sample:sample + size] = True
+            return int(mask.sum())
+        cached = (union(scene.reference.shape, scene.ref_begin, scene.chip),
+                  union(scene.target.shape, scene.tgt_begin, scene.window))
+        scene._covered = cached
+    return cached
+
+
 def stage_work(precision, scene, dtype):
     """Algorithmic work of one launch of each stage: (bound, amount, unit).
 
     HBM stages count the bytes they must move at least (each operand read once,
-    each product written once, per pair); the contractions count 2 flops per
+    each product written once, per pair; the gathers read each raster pixel the
+    chips and windows cover once); the contractions count 2 flops per
     multiply-add of the NCC numerator over all placements.  The fused stages
     carry the work of the reference kernels they absorb: gather_prep = gather +
     detect (detect.cu) + chip statistics (refStats.cu) + quantization;
@@ -136,12 +154,14 @@ def stage_work(precision, scene, dtype):
     qr = 4 if precision == "fp32" else 2              # refined operand bytes (fp8: FP16)
     hbm = lambda b: ("hbm", float(b) * N, "B")
     mfma = "valu" if precision == "fp32" else "mfma"
+    ref_px, tgt_px = covered_pixels(scene)
+    raster_read = eb * (ref_px + tgt_px)                # unique raster bytes, whole launch
     win_r = (4 * Er * Er + 2 * qr * Er * Er) if split else qr * Er * Er
     return {
-        "gather_detect": hbm((eb + 4) * (C * C + W * W)),
+        "gather_detect": ("hbm", raster_read + 4.0 * (C * C + W * W) * N, "B"),
         "chip_stats": hbm(8 * C * C),
         "window_sums": hbm(4 * W * W + 16 * P * P),
-        "gather_prep": hbm(eb * (C * C + W * W) + qc * (C * C + W * W)),
+        "gather_prep": ("hbm", raster_read + float(qc) * (C * C + W * W) * N, "B"),
         "correlate": (mfma, 2.0 * C * C * P * P * N, "FLOP"),
         "normalize": hbm(24 * P * P if precision == "fp32" else qc * W * W + 8 * P * P),
         "coarse_peak": hbm(4 * P * P),

@@ -2,9 +2,19 @@
 
 import types
 
+import numpy as np
+
 import bench
 
-SCENE = types.SimpleNamespace(pairs=1, chip=64, window=128, refine_factor=2, refine_margin=8)
+
+def scene(ref_begin, tgt_begin, shape=(512, 512)):
+    raster = np.zeros(shape, np.complex64)
+    return types.SimpleNamespace(pairs=len(ref_begin), chip=64, window=128, refine_factor=2,
+                                 refine_margin=8, reference=raster, target=raster,
+                                 ref_begin=np.array(ref_begin), tgt_begin=np.array(tgt_begin))
+
+
+SCENE = scene([(0, 0)], [(0, 0)])
 
 
 def test_contraction_flops_per_pair():
@@ -41,3 +51,11 @@ def test_stage_roofline_fraction():
     assert st["zoom"]["unit"] == "GB/s" and st["zoom"]["peak"] == 8000.0
     fp8 = bench.stage_roofline("fp8", SCENE, "complex64", {"correlate": 1e-3, "correlate_refined": 1e-3})
     assert fp8["correlate"]["peak"] == 5000.0 and fp8["correlate_refined"]["peak"] == 2500.0
+
+
+def test_gathers_count_covered_pixels_once():
+    # two pairs whose windows overlap by half: 128 x 192 target pixels, not 2 x 128^2;
+    # the chips do not overlap
+    two = scene([(0, 0), (0, 64)], [(0, 0), (0, 64)])
+    g = bench.stage_work("bf16", two, "complex64")["gather_prep"][1]
+    assert g == 8 * (2 * 64 * 64 + 128 * 192) + 2 * 2 * (64 * 64 + 128 * 128)

@@ -242,9 +242,12 @@ __global__ __launch_bounds__(kWave) void k_window_sums(const float *__restrict__
 // addresses (scalar loads).  The target streams through LDS in blocks of
 // `jblock` chip columns: a block needs every target row of the tile but only
 // vcount + jblock - 1 target columns, so the LDS footprint need not grow with
-// the chip width (one block covers the whole range when it fits).  The
-// accumulation order (j ascending, i ascending) is the same whatever the
-// tiling.
+// the chip width (one block covers the whole range when it fits).  The block
+// is stored column by column with an odd column stride: a lane's walk down a
+// chip column reads consecutive words (one base address per column, the row
+// in the instruction's offset), and the lanes of consecutive placement
+// columns fall on distinct banks.  The accumulation order (j ascending,
+// i ascending) is the same whatever the tiling.
 //
 // EXACT (the FP32 drop-in's coarse stage, one chip-column range): each chip
 // column is its own FP32 fused multiply-add chain of C terms, started at
@@ -256,6 +259,36 @@ __global__ __launch_bounds__(kWave) void k_window_sums(const float *__restrict__
 // chip-column ranges) each lane writes its range's FP32 partial numerator.
 // ---------------------------------------------------------------------------
 constexpr int kJB = 16;  // column block when the whole chip range does not fit
+
+// LDS column stride of the staged target block (odd; a stride of 2 mod 32
+// measured slower: the larger block costs occupancy)
+__host__ __device__ constexpr int correlate_stride(int rows)
+{
+    return rows | 1;
+}
+// dword 3 of a raw buffer resource on gfx9 targets (bounds-checked loads)
+constexpr int kBufferWord3 = 0x00020000;
+
+// BU consecutive floats at byte offset `off` of a buffer (wave-uniform offset)
+template <int BU>
+__device__ inline void chip_block(__amdgpu_buffer_rsrc_t rs, int off, float (&r)[BU])
+{
+    if constexpr (BU % 4 == 0) {
+#pragma unroll
+        for (int q = 0; q < BU / 4; ++q) {
+            const float4 x = __builtin_bit_cast(
+                float4, __builtin_amdgcn_raw_buffer_load_b128(rs, off + 16 * q, 0, 0));
+            r[4 * q] = x.x;
+            r[4 * q + 1] = x.y;
+            r[4 * q + 2] = x.z;
+            r[4 * q + 3] = x.w;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < BU; ++q)
+            r[q] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off + 4 * q, 0, 0));
+    }
+}
 
 // gamma from the FP64 numerator and the FP64 window sums / chip statistics:
 // (num - mu_t sum r') / (|r'| sqrt(sum t^2 - mu_t sum t)); NaN where the
@@ -296,6 +329,7 @@ __global__ __launch_bounds__(1024) void k_correlate(const float *__restrict__ rT
     const int j0 = part * jlen;
     const int rows = nub * BU + C - 1;
     const int cols = vcount + jblock - 1;
+    const int cs = correlate_stride(rows);
 
     const int task = threadIdx.x;
     const bool active = task < vcount * nub;
@@ -309,33 +343,62 @@ __global__ __launch_bounds__(1024) void k_correlate(const float *__restrict__ rT
     for (int a = 0; a < BU; ++a) acc[a] = 0.0f;
 #pragma unroll
     for (int a = 0; a < (EXACT ? BU : 1); ++a) total[a] = 0.0;
+    // the chip columns of this range are one run of blocks of BU values
+    // (column j at rp + j C); a block is loaded (vector loads: the LDS reads
+    // keep their own counter) while the one before it is used
+    const __amdgpu_buffer_rsrc_t crs =
+        __builtin_amdgcn_make_buffer_rsrc((void *)rp, (short)0, C * C * 4, kBufferWord3);
+    const int last = ((j0 + jlen) * C - BU) * 4;
+    int off = j0 * C * 4;
+    float ra[BU], rb[BU];
+    chip_block<BU>(crs, off, ra);
+    float w[BU];
+    // chain steps i0 .. i0 + BU - 1 of the lane's column window tc
+    auto step = [&](const float *tc, int i0, const float(&r)[BU]) {
+#pragma unroll
+        for (int s = 0; s < BU; ++s) {
+            w[(s + BU - 1) % BU] = tc[i0 + s + BU - 1];
+#pragma unroll
+            for (int a = 0; a < BU; ++a) acc[a] = fmaf(r[s], w[(s + a) % BU], acc[a]);
+        }
+    };
 
     for (int jb = 0; jb < jlen; jb += jblock) {
         const int jn = min(jblock, jlen - jb);
         // stage target rows [u0, u0 + rows) x columns [v0 + j0 + jb, + cols)
         __syncthreads();
         const int cbase = v0 + j0 + jb;
-        for (int k = threadIdx.x; k < rows * cols; k += blockDim.x) {
-            const int r = k / cols, c = k - r * cols;
-            const int gr = u0 + r, gc = cbase + c;
-            tile[k] = (gr < W && gc < W) ? src[(size_t)gr * W + gc] : 0.0f;
+        {
+            // (r, c) of element k = r cols + c, advanced without a division
+            const int dr = blockDim.x / cols, dc = blockDim.x - dr * cols;
+            int r = threadIdx.x / cols, c = threadIdx.x - r * cols;
+            for (; r < rows; r += dr, c += dc) {
+                if (c >= cols) c -= cols, ++r;
+                if (r >= rows) break;
+                const int gr = u0 + r, gc = cbase + c;
+                tile[c * cs + r] = (gr < W && gc < W) ? src[(size_t)gr * W + gc] : 0.0f;
+            }
         }
         __syncthreads();
         if (!active) continue;
         for (int jj = 0; jj < jn; ++jj) {
-            const float *rc = rp + (size_t)(j0 + jb + jj) * C;
-            const float *tc = tile + (size_t)(ubk * BU) * cols + v + jj;
-            float w[BU];
+            const float *tc = tile + (size_t)(v + jj) * cs + ubk * BU;
 #pragma unroll
-            for (int a = 0; a < BU - 1; ++a) w[a] = tc[a * cols];
-            for (int i0 = 0; i0 < C; i0 += BU) {
+            for (int a = 0; a < BU - 1; ++a) w[a] = tc[a];
+            int i0 = 0;
+            for (; i0 + 2 * BU <= C; i0 += 2 * BU) {
+                chip_block<BU>(crs, min(off + 4 * BU, last), rb);
+                step(tc, i0, ra);
+                chip_block<BU>(crs, min(off + 8 * BU, last), ra);
+                step(tc, i0 + BU, rb);
+                off += 8 * BU;
+            }
+            if (i0 < C) {  // C / BU odd: one block left
+                chip_block<BU>(crs, min(off + 4 * BU, last), rb);
+                step(tc, i0, ra);
 #pragma unroll
-                for (int s = 0; s < BU; ++s) {
-                    w[(s + BU - 1) % BU] = tc[(i0 + s + BU - 1) * cols];
-                    const float r = rc[i0 + s];
-#pragma unroll
-                    for (int a = 0; a < BU; ++a) acc[a] = fmaf(r, w[(s + a) % BU], acc[a]);
-                }
+                for (int s = 0; s < BU; ++s) ra[s] = rb[s];
+                off += 4 * BU;
             }
             if constexpr (EXACT) {
 #pragma unroll
@@ -3027,7 +3090,7 @@ __global__ __launch_bounds__(kScreenWaves * kWave) void k_zoom_screen1(
 static size_t correlate_lds(int C, int bu, int ucount, int vcount, int jblock)
 {
     const int nub = (ucount + bu - 1) / bu;
-    return (size_t)(nub * bu + C - 1) * (vcount + jblock - 1) * sizeof(float);
+    return (size_t)correlate_stride(nub * bu + C - 1) * (vcount + jblock - 1) * sizeof(float);
 }
 
 CorrelateShape choose_correlate_shape(int C, int P, bool exact, int bu, int nsplit)

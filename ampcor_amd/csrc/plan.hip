@@ -1160,7 +1160,7 @@ int ampcor_hip_plan_autotune(ampcor_hip_plan *plan, int reps)
         best = 1e30f;
         pick = plan->refined_shape;
         for (int bu : {4, 8, 16})
-            for (int ns : {4, 8, 16}) {
+            for (int ns : {1, 2, 4, 8, 16}) {
                 if (rc || plan->Cr % bu || plan->Cr % ns || plan->Cr / ns < 8) continue;
                 CorrelateShape sh = choose_correlate_shape(plan->Cr, plan->R, false, bu, ns);
                 if (sh.lds_bytes > 150 * 1024) continue;

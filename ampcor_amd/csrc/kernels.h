@@ -193,11 +193,16 @@ bool zoom_fits(int R, int zf);
 // operand, b = 32 kc + 4 (l / 16) + e for e < 4 and 32 kc + 16 + 4 (l / 16) +
 // e - 4 otherwise (pfrag); zero past zf R rows or R columns
 int zoom_fragment_count(int R, int zf);
-// per-pair scratch of the zoom's three kernels: [pairs] each
+// cells outside its window that the zoom's screen lists per pair for exact
+// evaluation (a pair with more gets the full search)
+constexpr int kZoomList = 512;
+// per-pair scratch of the zoom's kernels: [pairs] each
 struct ZoomScratch {
     double *thr;
     int32_t *win;  // [pairs][4]
     int *flag;
+    int *ncand;
+    int *cand;     // [pairs][kZoomList]
 };
 // the caller's bounds on the zoom screen's FP16 error (g = the refined surface):
 // |f_screen - f| <= ecoef max|g| + eabs over the two products, |H_screen - H|

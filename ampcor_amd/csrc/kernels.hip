@@ -2441,7 +2441,8 @@ __host__ __device__ inline ZoomLayout zoom_layout(int R, int zf, bool stage_d, i
 }
 
 // the whole search of one pair (window, screen, exact cells): for the pairs
-// the window kernel or the batched screen flagged (flag[pair] != 0)
+// flagged for it (flag[pair] != 0: no batched screen for this geometry, a
+// block screen's hit, or more listed cells than the list holds)
 template <bool STAGE, int KC>
 __global__ __launch_bounds__(kZoomThreads) void k_zoom_full(
     const float *__restrict__ gamma_refined, int R, int zf, const double *__restrict__ D,
@@ -2653,16 +2654,20 @@ __global__ __launch_bounds__(kZoomThreads) void k_zoom_full(
 
 // per-pair state the zoom's kernels hand on: the threshold the screen must
 // reach (exact window maximum minus the screen's error bound), the window
-// (p0, p1, q0, q1) and whether the pair needs the full search
+// (p0, p1, q0, q1), whether the pair needs the full search, and the cells
+// outside the window that the screen passed (count, list of p Z + q)
 struct ZoomHandoff {
     double *thr;
     int4 *win;
     int *flag;
+    int *ncand;
+    int *cand;   // [pairs][kZoomList]
 };
 
-// the exact window alone, with provisional results; flags the pair for the
-// full search when its maximum lies on a window edge inside the grid (a
-// parabola neighbour is then outside the window) or when `force`
+// the exact window alone, with provisional results (final unless the screen
+// lists cells for the pair or its maximum lies on a window edge inside the
+// grid, where a parabola neighbour is outside the window); flags the pair for
+// the full search when `force` (no screen for this geometry)
 template <bool STAGE>
 __global__ __launch_bounds__(kZoomThreads) void k_zoom_window(
     const float *__restrict__ gamma_refined, int R, int zf, const double *__restrict__ D,
@@ -2737,8 +2742,6 @@ __global__ __launch_bounds__(kZoomThreads) void k_zoom_window(
     if (fi == 0x7fffffff) fi = 0;
     const int pp = fi / Z, pq = fi - pp * Z;
     const int ip = pp - p0, iq = pq - q0;
-    const bool edge = (pp == p0 && p0 > 0) || (pp == p1 && p1 < Z - 1) ||
-                      (pq == q0 && q0 > 0) || (pq == q1 && q1 < Z - 1);
     const double dp = (ip > 0 && ip < np - 1)
                           ? parabola_vertex(f[(ip - 1) * nq + iq], fb, f[(ip + 1) * nq + iq]) : 0.0;
     const double dq = (iq > 0 && iq < nq - 1)
@@ -2758,7 +2761,8 @@ __global__ __launch_bounds__(kZoomThreads) void k_zoom_window(
     zoom_loc[pair] = make_int2(pp, pq);
     ho.thr[pair] = none ? __builtin_nan("") : fb - (ecoef * (double)gmax + eabs);
     ho.win[pair] = make_int4(p0, p1, q0, q1);
-    ho.flag[pair] = !none && (force || edge);
+    ho.flag[pair] = !none && force;
+    ho.ncand[pair] = 0;
 }
 
 // the FP16 screen of the whole zoomed grid, kZoomScreenPairs pairs per
@@ -2937,7 +2941,7 @@ __host__ __device__ inline size_t zoom_screen1_lds(int R, int zf)
 }
 
 template <int KC>
-__global__ __launch_bounds__(kScreenWaves * kWave) void k_zoom_screen1(
+__global__ __launch_bounds__(kScreenWaves * kWave, 4) void k_zoom_screen1(
     const float *__restrict__ gamma_refined, int pairs, int R, int zf,
     const uint4 *__restrict__ dfrag, const uint4 *__restrict__ pfrag, float lambda, float hcoef,
     float habs, ZoomHandoff ho)
@@ -2997,8 +3001,13 @@ __global__ __launch_bounds__(kScreenWaves * kWave) void k_zoom_screen1(
         for (int d = kWave / 2; d > 0; d >>= 1) gmax = fmaxf(gmax, __shfl_xor(gmax, d, kWave));
         // bound on |H - H_screen| (the first product's error)
         const float eh = hcoef * gmax + habs;
-        float mx = -INFINITY;
-        bool any = false;
+        // a cell outside the window that reaches the threshold goes on the
+        // pair's list; past the list's end the pair gets the full search
+        auto list = [&](int p, int q) {
+            const int slot = atomicAdd(&ho.ncand[pair], 1);
+            if (slot < kZoomList) ho.cand[(size_t)pair * kZoomList + slot] = p * Z + q;
+            else ho.flag[pair] = 1;
+        };
         for (int pt = 0; pt < ZT; ++pt) {
             // H^T tiles (b rows, p columns) for this row tile of the grid
             floatx4 h[2 * KC];
@@ -3059,25 +3068,208 @@ __global__ __launch_bounds__(kScreenWaves * kWave) void k_zoom_screen1(
                 for (int u = 0; u < 8; ++u) {
                     const int j = j0 + u;
                     if (j >= ZT) break;
-                    // lane: f[16 pt + lm][16 j + 4 lk + r]
+                    // lane: f[16 pt + lm][16 j + 4 lk + r]; the tiles inside the
+                    // grid and clear of the window test four cells at once
                     const float m4 =
                         fmaxf(fmaxf(acc[u][0], acc[u][1]), fmaxf(acc[u][2], acc[u][3]));
-                    if (!((slow >> j) & 1)) {
-                        mx = fmaxf(mx, m4);
-                        continue;
-                    }
+                    const bool fast = !((slow >> j) & 1);
+                    if (fast && !(m4 >= thr_f)) continue;
                     const int p = 16 * pt + lm;
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         const int q = 16 * j + 4 * lk + r;
                         const bool inside = p >= w.x && p <= w.y && q >= w.z && q <= w.w;
-                        any |= p < Z && q < Z && !inside && acc[u][r] >= thr_f;
+                        if (p < Z && q < Z && !inside && acc[u][r] >= thr_f) list(p, q);
                     }
                 }
             }
         }
-        any |= mx >= thr_f;
-        if (__any(any) && lane == 0) ho.flag[pair] = 1;
+    }
+}
+
+// the listed cells of the pairs not flagged for the full search, and the pairs
+// whose window maximum lies on a window edge inside the grid: the exact value
+// of each listed cell, the first maximum of the window's and the listed
+// cells' values, and the sub-pixel fit from exact neighbour values.  Every
+// cell outside the window that can reach the window's maximum is on the list,
+// so the result is the full search's.
+//
+// A cell's value is f(p, q) = sum_b H(p, b) D(q - zf b) with H(p, b) =
+// sum_a D(p - zf a) g(a, b), both sums in index order -- the window's order,
+// so a cell's value has the bits the window or the full search computes.  The
+// rows H(p, .) of the listed cells' rows (up to nr of them) are computed once
+// into LDS, one lane per (row, b); a cell in a row past those takes both sums
+// itself.  A workgroup takes the pairs blockIdx.x + k gridDim.x that need the
+// stage, one after the other.
+__host__ __device__ inline size_t zoom_cands_lds(int R, int zf, bool stage_d, int nr)
+{
+    const size_t Z = (size_t)zf * R;
+    return (stage_d ? align16(sizeof(double) * (2 * Z + 1)) : 0) +
+           align16(sizeof(float) * (size_t)R * R) + align16(sizeof(int) * Z) +
+           sizeof(double) * (size_t)nr * R;
+}
+
+constexpr int kZoomSharedRows = 64;   // at most kZoomThreads
+constexpr int kZoomCandBlocks = 2048;  // workgroups of the stage (each looks up its pairs at once)
+
+template <bool STAGE>
+__global__ __launch_bounds__(kZoomThreads) void k_zoom_cands(
+    const float *__restrict__ gamma_refined, int pairs, int R, int zf,
+    const double *__restrict__ D, int nr, const int2 *__restrict__ window,
+    const float *__restrict__ coarse_peak, int margin, int rf, float *__restrict__ offsets,
+    float *__restrict__ refined_peak, int2 *__restrict__ zoom_loc, unsigned *__restrict__ stats,
+    ZoomHandoff ho)
+{
+    extern __shared__ __align__(16) unsigned char zlds[];
+    __shared__ int ri[kZoomThreads / kWave];
+    __shared__ double rbd[kZoomThreads / kWave];
+    __shared__ double nb[4];
+    __shared__ int nrows, nact;
+    __shared__ int rowlist[kZoomSharedRows];
+    __shared__ int act[kZoomThreads];
+    const int t = threadIdx.x;
+    const int Z = zf * R;
+    size_t off = 0;
+    double *Dt = reinterpret_cast<double *>(zlds);   // [2 Z + 1], when staged
+    if (STAGE) off = align16(sizeof(double) * (2 * (size_t)Z + 1));
+    float *g = reinterpret_cast<float *>(zlds + off);   // [R][R]
+    off += align16(sizeof(float) * (size_t)R * R);
+    int *rowslot = reinterpret_cast<int *>(zlds + off);   // [Z]: H slot of row p, or -1
+    off += align16(sizeof(int) * (size_t)Z);
+    double *Hr = reinterpret_cast<double *>(zlds + off);   // [nr][R]
+    if (STAGE)
+        for (int k = t; k < 2 * Z + 1; k += kZoomThreads) Dt[k] = D[k];
+    const double *Dw = STAGE ? Dt : D;
+
+    // the pairs base + k gridDim.x (k < kZoomThreads) of this workgroup that
+    // need the stage: looked up at once, then taken one after the other
+    for (int base = blockIdx.x; base < pairs; base += gridDim.x * kZoomThreads) {
+        if (t == 0) nact = 0;
+        __syncthreads();
+        {
+            const int pair = base + t * gridDim.x;
+            if (pair < pairs && ho.flag[pair] == 0) {
+                const double thr = ho.thr[pair];
+                const int4 w = ho.win[pair];
+                const int2 loc = zoom_loc[pair];
+                const bool edge = (loc.x == w.x && w.x > 0) || (loc.x == w.y && w.y < Z - 1) ||
+                                  (loc.y == w.z && w.z > 0) || (loc.y == w.w && w.w < Z - 1);
+                if (thr == thr && (ho.ncand[pair] > 0 || edge)) act[atomicAdd(&nact, 1)] = pair;
+            }
+        }
+        __syncthreads();
+        const int na = nact;
+        for (int ia = 0; ia < na; ++ia) {
+            const int pair = act[ia];
+            const int2 loc = zoom_loc[pair];
+            const int n = min(ho.ncand[pair], kZoomList);
+            const float *gr = gamma_refined + (size_t)pair * R * R;
+            for (int k = t; k < R * R; k += kZoomThreads) {
+                const float x = gr[k];
+                g[k] = (x != x) ? 0.0f : x;
+            }
+            for (int k = t; k < Z; k += kZoomThreads) rowslot[k] = -1;
+            if (t == 0) nrows = 0;
+            __syncthreads();
+            // the rows to share: the listed cells' and the window maximum's
+            const int *cl = ho.cand + (size_t)pair * kZoomList;
+            for (int k = t; k < n; k += kZoomThreads) rowslot[cl[k] / Z] = -2;
+            if (t == 0) rowslot[loc.x] = -2;
+            __syncthreads();
+            for (int p = t; p < Z; p += kZoomThreads) {
+                if (rowslot[p] != -2) continue;
+                const int slot = atomicAdd(&nrows, 1);
+                if (slot < nr) rowlist[slot] = p;
+                rowslot[p] = slot < nr ? slot : -1;
+            }
+            __syncthreads();
+            // H of shared rows [s0, s1)
+            auto rows_h = [&](int s0, int s1) {
+                for (int k = t; k < (s1 - s0) * R; k += kZoomThreads) {
+                    const int sl = s0 + k / R, bb = k - (sl - s0) * R;
+                    const double *wp = Dw + rowlist[sl] + Z;
+                    double h = 0.0;
+#pragma unroll 8
+                    for (int aa = 0; aa < R; ++aa) h += wp[-zf * aa] * (double)g[aa * R + bb];
+                    Hr[sl * R + bb] = h;
+                }
+            };
+            const int ns = min(nrows, nr);
+            rows_h(0, ns);
+            __syncthreads();
+            auto value = [&](int p, int q) -> double {
+                const double *wq = Dw + q + Z;
+                const int sl = rowslot[p];
+                double acc = 0.0;
+                if (sl >= 0) {
+#pragma unroll 8
+                    for (int bb = 0; bb < R; ++bb) acc += Hr[sl * R + bb] * wq[-zf * bb];
+                } else {
+                    const double *wp = Dw + p + Z;
+                    for (int bb = 0; bb < R; ++bb) {
+                        double h = 0.0;
+#pragma unroll 8
+                        for (int aa = 0; aa < R; ++aa) h += wp[-zf * aa] * (double)g[aa * R + bb];
+                        acc += h * wq[-zf * bb];
+                    }
+                }
+                return acc;
+            };
+            double cb = -INFINITY;
+            int ci = 0x7fffffff;
+            for (int k = t; k < n; k += kZoomThreads) {
+                const int idx = cl[k];
+                better_d(cb, ci, value(idx / Z, idx - (idx / Z) * Z), idx);
+            }
+            if (stats && t == 0 && n > 0) {
+                atomicAdd(&stats[0], (unsigned)n);
+                atomicAdd(&stats[1], 1u);
+            }
+            // the window's maximum (its value recomputed)
+            double fb = value(loc.x, loc.y);
+            int fi = loc.x * Z + loc.y;
+            block_argmax_d(rbd, ri, cb, ci);
+            better_d(fb, fi, cb, ci);
+            const int pp = fi / Z, pq = fi - pp * Z;
+            // the rows of the neighbours above and below join the shared rows
+            // while there is room
+            if (t == 0) {
+                nrows = ns;
+                for (int r = pp - 1; r <= pp + 1; r += 2)
+                    if (r >= 0 && r < Z && rowslot[r] < 0 && nrows < nr) {
+                        rowslot[r] = nrows;
+                        rowlist[nrows++] = r;
+                    }
+            }
+            __syncthreads();
+            rows_h(ns, nrows);
+            __syncthreads();
+            if (t < 4) {
+                const int p = pp + (t == 0 ? -1 : t == 1 ? 1 : 0);
+                const int q = pq + (t == 2 ? -1 : t == 3 ? 1 : 0);
+                nb[t] = (p >= 0 && p < Z && q >= 0 && q < Z) ? value(p, q) : 0.0;
+            }
+            __syncthreads();
+            if (t == 0) {
+                const double dp = (pp > 0 && pp < Z - 1) ? parabola_vertex(nb[0], fb, nb[1]) : 0.0;
+                const double dq = (pq > 0 && pq < Z - 1) ? parabola_vertex(nb[2], fb, nb[3]) : 0.0;
+                const int2 tl = window[pair];
+                const double zoom = (double)rf * zf;
+                float dl = (float)((double)(tl.x - margin) + ((double)pp + dp) / zoom);
+                float ds = (float)((double)(tl.y - margin) + ((double)pq + dq) / zoom);
+                float fpk = (float)fb;
+                const float cp = coarse_peak[pair];
+                if (cp != cp) {
+                    dl = ds = fpk = __builtin_nanf("");
+                }
+                offsets[2 * pair] = dl;
+                offsets[2 * pair + 1] = ds;
+                refined_peak[pair] = fpk;
+                zoom_loc[pair] = make_int2(pp, pq);
+            }
+            __syncthreads();   // (the next pair reuses the LDS)
+        }
+        __syncthreads();   // (the next lookup reuses act and nact)
     }
 }
 
@@ -3486,7 +3678,8 @@ hipError_t launch_zoom(hipStream_t s, const float *gamma_refined, int pairs, int
                        const ZoomScratch &scratch, unsigned *stats)
 {
     const double ecoef = bounds.ecoef, eabs = bounds.eabs;
-    if (!zoom_fits(R, zf) || !scratch.thr || !scratch.win || !scratch.flag)
+    if (!zoom_fits(R, zf) || !scratch.thr || !scratch.win || !scratch.flag || !scratch.ncand ||
+        !scratch.cand)
         return hipErrorInvalidValue;
     if (pairs == 0) return hipSuccess;
     if (stats) {
@@ -3494,7 +3687,8 @@ hipError_t launch_zoom(hipStream_t s, const float *gamma_refined, int pairs, int
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    const ZoomHandoff ho{scratch.thr, reinterpret_cast<int4 *>(scratch.win), scratch.flag};
+    const ZoomHandoff ho{scratch.thr, reinterpret_cast<int4 *>(scratch.win), scratch.flag,
+                         scratch.ncand, scratch.cand};
     const auto *df = reinterpret_cast<const uint4 *>(dfrag);
     // the batched screen keeps a wave's fragments (NT tiles x KC chunks) in registers
     const int ZT = (zf * R + 15) / 16, KC = (R + 31) / 32;
@@ -3560,7 +3754,30 @@ hipError_t launch_zoom(hipStream_t s, const float *gamma_refined, int pairs, int
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    // 3. the full search of the flagged pairs
+    // 3. the listed cells and the edge maxima of the other pairs (shared rows:
+    // up to kZoomSharedRows, as many as fit beside the weights and the surface)
+    {
+        const bool stage_d = zoom_cands_lds(R, zf, true, 0) <= kZoomLdsStage;
+        const size_t base = zoom_cands_lds(R, zf, stage_d, 0);
+        const int nr = base >= kZoomLdsStage
+                           ? 0
+                           : (int)std::min<size_t>(kZoomSharedRows,
+                                                   (kZoomLdsStage - base) / (sizeof(double) * R));
+        const size_t lds = zoom_cands_lds(R, zf, stage_d, nr);
+        if (lds > kZoomLdsMax) return hipErrorInvalidValue;
+        const int blocks = std::min(pairs, kZoomCandBlocks);
+        if (stage_d)
+            hipLaunchKernelGGL(k_zoom_cands<true>, dim3(blocks), dim3(kZoomThreads), lds, s,
+                               gamma_refined, pairs, R, zf, dirichlet, nr, window, coarse_peak,
+                               margin, rf, offsets, refined_peak, zoom_loc, stats, ho);
+        else
+            hipLaunchKernelGGL(k_zoom_cands<false>, dim3(blocks), dim3(kZoomThreads), lds, s,
+                               gamma_refined, pairs, R, zf, dirichlet, nr, window, coarse_peak,
+                               margin, rf, offsets, refined_peak, zoom_loc, stats, ho);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    // 4. the full search of the flagged pairs
     const bool stage_d = zoom_layout(R, zf, true, zoom_row_tiles(R, zf, true)).total <=
                          kZoomLdsStage;
     const int rt = zoom_row_tiles(R, zf, stage_d);

@@ -385,6 +385,8 @@ int plan_build(ampcor_hip_plan *p, Carver &cv, hipStream_t s, bool upload)
     A(p->zthr.carve(cv, N));
     A(p->zwin.carve(cv, 4 * N));
     A(p->zflag.carve(cv, N));
+    A(p->zncand.carve(cv, N));
+    A(p->zcand.carve(cv, N * kZoomList));
     if (r.e != hipSuccess) {
         std::string msg = std::string("allocating plan buffers for ") + std::to_string(N) +
                           " pairs: " + hipGetErrorString(r.e);

@@ -133,7 +133,12 @@ struct ampcor_hip_plan {
     ampcor_hip::DeviceBuffer<double> zthr;  // the zoom kernels' hand-off, per pair
     ampcor_hip::DeviceBuffer<int32_t> zwin;
     ampcor_hip::DeviceBuffer<int> zflag;
-    ampcor_hip::ZoomScratch zoom_scratch() const { return {zthr.ptr, zwin.ptr, zflag.ptr}; }
+    ampcor_hip::DeviceBuffer<int> zncand;
+    ampcor_hip::DeviceBuffer<int> zcand;
+    ampcor_hip::ZoomScratch zoom_scratch() const
+    {
+        return {zthr.ptr, zwin.ptr, zflag.ptr, zncand.ptr, zcand.ptr};
+    }
     ampcor_hip::DeviceBuffer<uint16_t> chip_q, tgt_q, chip_q_r, tgt_q_r, chip_lo_r, tgt_lo_r;
     ampcor_hip::DeviceBuffer<uint16_t> hfrag_ref, hfrag_tgt;  // FP16 circulant fragments
     // scratch of the banded upsampler (tiles above 96 samples)
@@ -188,6 +193,8 @@ struct ampcor_hip_plan {
         zthr.release();
         zwin.release();
         zflag.release();
+        zncand.release();
+        zcand.release();
         stats.release();
         stats_r.release();
         loc.release();

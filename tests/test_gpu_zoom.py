@@ -106,3 +106,27 @@ def test_zoom_refined_surfaces_of_shifted_scenes(lib):
                               s.refine_factor, s.refine_margin, s.zoom_factor, surfaces=True)
         surfaces.append(out["gamma_refined"])
     check(lib, np.concatenate(surfaces), 4)
+
+
+def test_zoom_listed_cells_and_full_search(lib):
+    """Both ways the screen hands cells on: broad smooth peaks at zoom 16 leave
+    a few cells per surface near the window for exact evaluation (listed), and
+    surfaces of independent noise leave more than the list holds (the full
+    search); either way the result is the full-grid search's."""
+    R, zf = 33, 16
+    rng = np.random.default_rng(77)
+    y, x = np.mgrid[0:R, 0:R]
+    broad = np.empty((24, R, R), np.float32)
+    for k in range(len(broad)):
+        c = rng.uniform(R / 3, 2 * R / 3, size=2)
+        w = rng.uniform(3.0, 6.0)
+        broad[k] = np.exp(-((y - c[0]) ** 2 + (x - c[1]) ** 2) / (2 * w * w))
+        broad[k] += 1e-3 * rng.standard_normal((R, R)).astype(np.float32)
+    _, stats = check(lib, broad, zf)
+    print(f"broad peaks: {stats}", flush=True)
+    # (a searched surface counts among the pairs with exactly evaluated cells too)
+    assert stats["pairs"] - stats["searched"] >= len(broad) // 2
+    noise = rng.uniform(-1.0, 1.0, size=(16, R, R)).astype(np.float32)
+    _, stats = check(lib, noise, zf)
+    print(f"noise: {stats}", flush=True)
+    assert stats["searched"] > 0

@@ -257,20 +257,17 @@ MixedShape choose_mixed_packed_shape(int C, int P, int max_vt, int wpe, bool spl
 {
     MixedShape s{};
     const int nk = (C + 15 + 15) / 16;
-    if (P < 33 || (P - 1) % 32 || C % 16 || nk > 11) return s;
+    if (P < 33 || (P - 1) % 32 || C % 16 || !packed_nk_built(nk)) return s;
     const int nb = (P - 1) / 32, vtt = 2 * nb + 1;
-    // VGPRs of a wave: two tiles' fragments (8 per fragment), accumulators
-    // (8 per v-tile) and about 40 for the chip fragments, the packed row's
-    // loads, the refill and addresses (measured on the compiled kernels)
-    auto vgprs = [&](int vt) { return 16 * vt + 8 * nk + 40; };
-    // the instantiated (v-tiles, chunk rows, waves per SIMD) combinations
+    // the (v-tiles, chunk rows, waves per SIMD) combinations, built where
+    // their registers fit
     static const int kinds[][3] = {{2, 4, 3}, {3, 4, 3}, {3, 4, 2}, {5, 4, 2}, {2, 8, 3},
                                    {3, 8, 3}, {3, 8, 2}, {5, 8, 3}, {5, 8, 2}};
     int best = -1, best_cost = 1 << 30;
     for (int k = 0; k < (int)(sizeof(kinds) / sizeof(kinds[0])); ++k) {
         const int vt = kinds[k][0], ch = kinds[k][1], w = kinds[k][2];
         if ((max_vt > 0 && vt > max_vt) || (wpe && w != wpe)) continue;
-        if (vgprs(vt) > (w == 3 ? 168 : 256)) continue;
+        if (!packed_kind_fits(nk, vt, w)) continue;
         const int nvg = (vtt + vt - 1) / vt, waves = nb * nvg;
         if (waves > std::min(packed_detail::kMaxWaves, 4 * w) || C % ch) continue;
         // one-wave workgroups refill four chip rows at a time (a smaller LDS
@@ -327,7 +324,8 @@ hipError_t launch_correlate_mixed(hipStream_t s, const MixedShape &sh, const uin
         const int P = W - C + 1;
         if (sh.stream || sh.nsplit != 1 || P < 33 || (P - 1) % 32 || C % 16 ||
             C % sh.packed || sh.tiles_u != (P - 1) / 32 || sh.wg_ut != sh.tiles_u * sh.tiles_v ||
-            (sh.wpe != 2 && sh.wpe != 3) ||
+            (sh.wpe != 2 && sh.wpe != 3) || !packed_nk_built(sh.nk) ||
+            !packed_kind_fits(sh.nk, sh.wg_vt, sh.wpe) ||
             sh.wg_ut > std::min(packed_detail::kMaxWaves, 4 * sh.wpe) ||
             sh.tiles_v * sh.wg_vt < (P + 15) / 16 ||
             !packed_refill_fits(sh.nk, sh.wg_vt, sh.packed, sh.tiles_u, sh.tiles_v) ||
@@ -339,7 +337,7 @@ hipError_t launch_correlate_mixed(hipStream_t s, const MixedShape &sh, const uin
                                              chip_lo, tgt_lo)
         switch (sh.nk) {
             AMPCOR_PK(2); AMPCOR_PK(3); AMPCOR_PK(4); AMPCOR_PK(5); AMPCOR_PK(6); AMPCOR_PK(7);
-            AMPCOR_PK(8); AMPCOR_PK(9); AMPCOR_PK(10); AMPCOR_PK(11);
+            AMPCOR_PK(8); AMPCOR_PK(9); AMPCOR_PK(10); AMPCOR_PK(11); AMPCOR_PK(17);
         default: return hipErrorInvalidValue;
         }
 #undef AMPCOR_PK

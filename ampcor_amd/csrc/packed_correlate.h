@@ -44,7 +44,9 @@ using mixed_detail::floatx4;
 using mixed_detail::mfma32;
 constexpr int kWave = 64;
 constexpr int kMaxWaves = 12;   // at most three waves per SIMD (168 VGPRs)
-constexpr int kRefill = 2;  // wave-wide refill loads per step, at most
+// wave-wide refill loads per thread and step, at most: rows of 17 K-steps
+// (one row per load) need twice as many
+__host__ __device__ constexpr int refill_loads(int nk) { return nk > 11 ? 4 : 2; }
 constexpr int kShift = 1;   // fragments shifted (and their lane-15 rows loaded) together
 // dword 3 of a raw buffer resource on gfx9 targets (bounds-checked loads, zeros
 // past num_records)
@@ -176,7 +178,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 256 * WPE),
     // loads is issued before step s and stored after it (the slots it fills are
     // not read in this chunk)
     constexpr int NS = CH / 2;                     // steps per chunk
-    constexpr int KB = kRefill;                    // refill loads per thread and step
+    constexpr int KB = refill_loads(NK);           // refill loads per thread and step
     // CS as an opaque value: the step's chip base is then computed once per
     // step, and each K-step's offset (32 kappa bytes) fits the loads' offset field
     int csr = CS;
@@ -321,14 +323,35 @@ inline size_t packed_lds_bytes(int nk, int vt, int ch, int nb, int nvg)
 }
 
 // whether one chunk's refill (the target rows of every u-block and the chip
-// rows, one or two rows per wave-wide load) fits kRefill loads per thread and step
+// rows, one or two rows per wave-wide load) fits the refill loads per thread and step
 inline bool packed_refill_fits(int nk, int vt, int ch, int nb, int nvg)
 {
     const int cpr = (packed_detail::packed_row_stride(nk, vt, nvg) - 8) / 8, cch = 2 * nk + 2;
     if (cpr > 64 || cch > 64) return false;
     const int rpi = (cpr <= 32 && cch <= 32) ? 2 : 1;
     const int pairs = (nb * ch + ch + rpi - 1) / rpi, waves = nb * nvg;
-    return (pairs + waves - 1) / waves <= packed_detail::kRefill * (ch / 2);
+    return (pairs + waves - 1) / waves <= packed_detail::refill_loads(nk) * (ch / 2);
+}
+
+// the K-step counts built: 2 .. 11 (chips of 16 .. 160 samples) and 17 (256, the
+// refined chips of 128-sample chips)
+__host__ __device__ constexpr bool packed_nk_built(int nk)
+{
+    return (nk >= 2 && nk <= 11) || nk == 17;
+}
+
+// VGPRs of a wave: two tiles' fragments (8 per fragment), accumulators (8 per
+// v-tile), about 40 for the chip fragments, the packed row's loads and
+// addresses, and 4 per refill load past two (fitted to the compiled kernels);
+// a (K-steps, v-tiles, waves per SIMD) kind is built only where they fit the
+// register file at that occupancy (168 at three waves, 256 at two)
+__host__ __device__ constexpr int packed_vgprs(int nk, int vt)
+{
+    return 16 * vt + 8 * nk + 40 + 4 * (packed_detail::refill_loads(nk) - 2);
+}
+__host__ __device__ constexpr bool packed_kind_fits(int nk, int vt, int wpe)
+{
+    return packed_vgprs(nk, vt) <= (wpe == 3 ? 168 : 256);
 }
 
 // split shapes (hi / lo operands) run three passes over the same tiling:
@@ -353,26 +376,38 @@ hipError_t launch_correlate_packed_nk(hipStream_t s, const MixedShape &sh, const
         const uint16_t *tq = (pass == 1 ? tgt_lo : tgt_q) + (size_t)first * W * W;
         float *pt = partial + (size_t)first * P * P;
         const int acc = pass > 0;
-#define AMPCOR_PACKED(VTV, CHV, WPEV)                                                              \
-    if (sh.wg_vt == VTV && sh.packed == CHV && sh.wpe == WPEV) {                                   \
-        if (fp16)                                                                                  \
-            hipLaunchKernelGGL((k_correlate_packed<NK, VTV, CHV, WPEV, true>), dim3(n), block,     \
-                               sh.lds_bytes, s, cq, tq, C, W, P, sh.tiles_v, pt, acc);             \
-        else                                                                                       \
-            hipLaunchKernelGGL((k_correlate_packed<NK, VTV, CHV, WPEV, false>), dim3(n), block,    \
-                               sh.lds_bytes, s, cq, tq, C, W, P, sh.tiles_v, pt, acc);             \
-    } else
-        AMPCOR_PACKED(2, 4, 3)
-        AMPCOR_PACKED(3, 4, 3)
-        AMPCOR_PACKED(3, 4, 2)
-        AMPCOR_PACKED(5, 4, 2)
-        AMPCOR_PACKED(2, 8, 3)
-        AMPCOR_PACKED(3, 8, 3)
-        AMPCOR_PACKED(3, 8, 2)
-        AMPCOR_PACKED(5, 8, 3)
-        AMPCOR_PACKED(5, 8, 2)
-        return hipErrorInvalidValue;
-#undef AMPCOR_PACKED
+        bool launched = false;
+        auto kind = [&](auto vt, auto ch, auto wpe) {
+            constexpr int VTV = decltype(vt)::value, CHV = decltype(ch)::value;
+            constexpr int WPEV = decltype(wpe)::value;
+            if constexpr (packed_kind_fits(NK, VTV, WPEV)) {
+                if (launched || sh.wg_vt != VTV || sh.packed != CHV || sh.wpe != WPEV) return;
+                launched = true;
+                if (fp16)
+                    hipLaunchKernelGGL((k_correlate_packed<NK, VTV, CHV, WPEV, true>), dim3(n),
+                                       block, sh.lds_bytes, s, cq, tq, C, W, P, sh.tiles_v, pt,
+                                       acc);
+                else
+                    hipLaunchKernelGGL((k_correlate_packed<NK, VTV, CHV, WPEV, false>), dim3(n),
+                                       block, sh.lds_bytes, s, cq, tq, C, W, P, sh.tiles_v, pt,
+                                       acc);
+            }
+        };
+        using I2 = std::integral_constant<int, 2>;
+        using I3 = std::integral_constant<int, 3>;
+        using I4 = std::integral_constant<int, 4>;
+        using I5 = std::integral_constant<int, 5>;
+        using I8 = std::integral_constant<int, 8>;
+        kind(I2{}, I4{}, I3{});
+        kind(I3{}, I4{}, I3{});
+        kind(I3{}, I4{}, I2{});
+        kind(I5{}, I4{}, I2{});
+        kind(I2{}, I8{}, I3{});
+        kind(I3{}, I8{}, I3{});
+        kind(I3{}, I8{}, I2{});
+        kind(I5{}, I8{}, I3{});
+        kind(I5{}, I8{}, I2{});
+        if (!launched) return hipErrorInvalidValue;
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

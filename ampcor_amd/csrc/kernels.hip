@@ -2525,6 +2525,7 @@ __global__ __launch_bounds__(kZoomThreads) void k_zoom_full(
         const int ip = k / R, bb = k - ip * R;
         const double *w = Dw + (p0 + ip) + Z;   // weight of surface row aa: w[-zf aa]
         double acc = 0.0;
+#pragma unroll 8
         for (int aa = 0; aa < R; ++aa) acc += w[-zf * aa] * (double)g[aa * R + bb];
         H[k] = acc;
     }
@@ -2535,6 +2536,7 @@ __global__ __launch_bounds__(kZoomThreads) void k_zoom_full(
         const int ip = k / nq, iq = k - ip * nq;
         const double *w = Dw + (q0 + iq) + Z;
         double acc = 0.0;
+#pragma unroll 8
         for (int bb = 0; bb < R; ++bb) acc += H[ip * R + bb] * w[-zf * bb];
         f[k] = acc;
         better_d(fb, fi, acc, (p0 + ip) * Z + q0 + iq);
@@ -2723,6 +2725,7 @@ __global__ __launch_bounds__(kZoomThreads) void k_zoom_window(
         const int ip = k / R, bb = k - ip * R;
         const double *w = Dw + (p0 + ip) + Z;   // weight of surface row aa: w[-zf aa]
         double acc = 0.0;
+#pragma unroll 8
         for (int aa = 0; aa < R; ++aa) acc += w[-zf * aa] * (double)g[aa * R + bb];
         H[k] = acc;
     }
@@ -2733,6 +2736,7 @@ __global__ __launch_bounds__(kZoomThreads) void k_zoom_window(
         const int ip = k / nq, iq = k - ip * nq;
         const double *w = Dw + (q0 + iq) + Z;
         double acc = 0.0;
+#pragma unroll 8
         for (int bb = 0; bb < R; ++bb) acc += H[ip * R + bb] * w[-zf * bb];
         f[k] = acc;
         better_d(fb, fi, acc, (p0 + ip) * Z + q0 + iq);

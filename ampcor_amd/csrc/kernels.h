@@ -49,8 +49,10 @@ struct CorrelateShape {
 
 // chooses the tiling for chip C over P x P placements; exact = keep each
 // placement's whole contraction in one lane (the FP32 drop-in order);
-// bu / nsplit > 0 request a register block / chip-column split (tuning)
-CorrelateShape choose_correlate_shape(int C, int P, bool exact, int bu = 0, int nsplit = 0);
+// bu / nsplit / jblock > 0 request a register block / chip-column split /
+// chip columns per LDS pass (tuning)
+CorrelateShape choose_correlate_shape(int C, int P, bool exact, int bu = 0, int nsplit = 0,
+                                      int jblock = 0);
 
 hipError_t launch_gather_detect(hipStream_t s, const PairDesc *desc, int pairs, int dtype, int C,
                                 int W, float *ref_amp, float *tgt_amp);

@@ -3289,7 +3289,7 @@ static size_t correlate_lds(int C, int bu, int ucount, int vcount, int jblock)
     return (size_t)correlate_stride(nub * bu + C - 1) * (vcount + jblock - 1) * sizeof(float);
 }
 
-CorrelateShape choose_correlate_shape(int C, int P, bool exact, int bu, int nsplit)
+CorrelateShape choose_correlate_shape(int C, int P, bool exact, int bu, int nsplit, int jblock)
 {
     CorrelateShape s{};
     s.exact = exact;
@@ -3327,6 +3327,8 @@ CorrelateShape choose_correlate_shape(int C, int P, bool exact, int bu, int nspl
     // stage the whole chip-column range at once when it fits
     const int jlen = C / s.nsplit;
     s.jblock = correlate_lds(C, s.bu, extent, extent, jlen) <= 80 * 1024 ? jlen : kJB;
+    // a smaller block: more workgroups per CU, more barriers
+    if (jblock > 0 && jblock < s.jblock) s.jblock = jblock;
     s.lds_bytes = correlate_lds(C, s.bu, extent, extent, s.jblock);
     const int tasks = extent * ((extent + s.bu - 1) / s.bu);
     s.threads = ((tasks + kWave - 1) / kWave) * kWave;

@@ -1147,34 +1147,40 @@ int ampcor_hip_plan_autotune(ampcor_hip_plan *plan, int reps)
     if (!plan->mixed) {
         float best = 1e30f;
         CorrelateShape pick = plan->coarse_shape;
-        for (int bu : {4, 8, 16}) {
-            if (plan->C % bu) continue;
-            CorrelateShape sh = choose_correlate_shape(plan->C, plan->P, true, bu, 0);
-            float ms;
-            if ((rc = timeit([&] { return launch_correlate(s, sh, plan->rT.ptr, plan->amp_tgt.ptr, N,
-                                                           plan->C, plan->W, nullptr, plan->sums.ptr,
-                                                           plan->stats.ptr, plan->gamma.ptr); },
-                             &ms)))
-                break;
-            if (ms < best) best = ms, pick = sh;
-        }
-        plan->coarse_shape = pick;
-        best = 1e30f;
-        pick = plan->refined_shape;
         for (int bu : {4, 8, 16})
-            for (int ns : {1, 2, 4, 8, 16}) {
-                if (rc || plan->Cr % bu || plan->Cr % ns || plan->Cr / ns < 8) continue;
-                CorrelateShape sh = choose_correlate_shape(plan->Cr, plan->R, false, bu, ns);
-                if (sh.lds_bytes > 150 * 1024) continue;
+            for (int jb : {0, 16, 8}) {
+                if (rc || plan->C % bu) continue;
+                CorrelateShape sh = choose_correlate_shape(plan->C, plan->P, true, bu, 0, jb);
+                if (jb && sh.jblock != jb) continue;   // (the default block already)
                 float ms;
                 if ((rc = timeit([&] {
-                         hipError_t e = launch_correlate(s, sh, plan->rT_r.ptr, plan->amp_tgt_r.ptr,
-                                                         N, plan->Cr, plan->Wr, plan->partial_r.ptr);
-                         return e != hipSuccess ? e : normalize_refined(sh.nsplit);
+                         return launch_correlate(s, sh, plan->rT.ptr, plan->amp_tgt.ptr, N, plan->C,
+                                                 plan->W, nullptr, plan->sums.ptr, plan->stats.ptr,
+                                                 plan->gamma.ptr);
                      }, &ms)))
                     break;
                 if (ms < best) best = ms, pick = sh;
             }
+        plan->coarse_shape = pick;
+        best = 1e30f;
+        pick = plan->refined_shape;
+        for (int bu : {4, 8, 16})
+            for (int ns : {1, 2, 4, 8, 16})
+                for (int jb : {0, 16, 8}) {
+                    if (rc || plan->Cr % bu || plan->Cr % ns || plan->Cr / ns < 8) continue;
+                    CorrelateShape sh =
+                        choose_correlate_shape(plan->Cr, plan->R, false, bu, ns, jb);
+                    if (sh.lds_bytes > 150 * 1024 || (jb && sh.jblock != jb)) continue;
+                    float ms;
+                    if ((rc = timeit([&] {
+                             hipError_t e = launch_correlate(s, sh, plan->rT_r.ptr,
+                                                             plan->amp_tgt_r.ptr, N, plan->Cr,
+                                                             plan->Wr, plan->partial_r.ptr);
+                             return e != hipSuccess ? e : normalize_refined(sh.nsplit);
+                         }, &ms)))
+                        break;
+                    if (ms < best) best = ms, pick = sh;
+                }
         plan->refined_shape = pick;
     } else {
         struct Stage {

@@ -4,7 +4,7 @@ import numpy as np
 
 
 def run_hip(lib, reference, target, ref_begin, tgt_begin, chip, window, rf, rm, zf,
-            per_pair=False, precision="fp32"):
+            per_pair=False, precision="fp32", tune=False):
     n = len(ref_begin)
     worker = lib.sequential(n, (chip, chip), (window, window), rm, rf, zf, precision=precision)
     dtype = "complex64" if reference.dtype == np.complex64 else "float32"
@@ -19,6 +19,9 @@ def run_hip(lib, reference, target, ref_begin, tgt_begin, chip, window, rf, rm, 
             lib.addTarget(worker, tgt, k, t0, (t0[0] + window, t0[1] + window))
     else:
         lib.addPairs(worker, ref, tgt, 0, rb.tobytes(), tb.tobytes())
+    if tune:   # the autotuner times every candidate tiling on this data, then keeps the fastest
+        lib.adjust(worker, offsets=False)
+        lib.autotune(worker, 1)
     offsets = np.array(lib.adjust(worker), dtype=np.float32).reshape(n, 2)
     P, R, Z = lib.dims(worker)
     prods = {k: np.frombuffer(v, np.float32) for k, v in lib.products(worker).items()}

@@ -57,6 +57,17 @@ def test_shifted_speckle_matches_oracles(lib):
     assert np.max(np.abs(hip["offsets"] - s.truth)) < 0.05
 
 
+def test_tuned_plan_matches_oracles(lib):
+    """After the autotuner has picked its tilings (every FP32 coarse tiling keeps
+    the exact order; the refined one may split the chip columns into 1 .. 16
+    ranges), the same bars hold."""
+    s = synth.shifted_scene(seed=5)
+    hip = run_hip(lib, s.reference, s.target, s.ref_begin, s.tgt_begin, s.chip, s.window,
+                  s.refine_factor, s.refine_margin, s.zoom_factor, tune=True)
+    twin, exact = oracles(s)
+    check_against_oracles(hip, twin, exact)
+
+
 def test_golden_shift_end_to_end(lib):
     """The committed fixture: every pair recovers the planted Fourier shift."""
     import os

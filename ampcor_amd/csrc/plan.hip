@@ -1148,7 +1148,7 @@ int ampcor_hip_plan_autotune(ampcor_hip_plan *plan, int reps)
         float best = 1e30f;
         CorrelateShape pick = plan->coarse_shape;
         for (int bu : {4, 8, 16})
-            for (int jb : {0, 16, 8}) {
+            for (int jb : {0, 16, 8, 4}) {
                 if (rc || plan->C % bu) continue;
                 CorrelateShape sh = choose_correlate_shape(plan->C, plan->P, true, bu, 0, jb);
                 if (jb && sh.jblock != jb) continue;   // (the default block already)
@@ -1166,7 +1166,7 @@ int ampcor_hip_plan_autotune(ampcor_hip_plan *plan, int reps)
         pick = plan->refined_shape;
         for (int bu : {4, 8, 16})
             for (int ns : {1, 2, 4, 8, 16})
-                for (int jb : {0, 16, 8}) {
+                for (int jb : {0, 16, 8, 4}) {
                     if (rc || plan->Cr % bu || plan->Cr % ns || plan->Cr / ns < 8) continue;
                     CorrelateShape sh =
                         choose_correlate_shape(plan->Cr, plan->R, false, bu, ns, jb);

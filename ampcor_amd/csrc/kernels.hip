@@ -239,10 +239,13 @@ __global__ __launch_bounds__(kWave) void k_window_sums(const float *__restrict__
 // and walks the chip column by column (j outer, i inner); down a column it
 // slides a BU-deep register window over the target rows, so every LDS read
 // feeds BU fused multiply-adds.  The zero-mean chip is read with wave-uniform
-// addresses (scalar loads).  The target streams through LDS in blocks of
-// `jblock` chip columns: a block needs every target row of the tile but only
-// vcount + jblock - 1 target columns, so the LDS footprint need not grow with
-// the chip width (one block covers the whole range when it fits).  The block
+// addresses, BU values at a time through vector buffer loads issued one block
+// ahead (their counter is not the LDS reads').  The target streams through LDS
+// in blocks of `jblock` chip columns: a block needs every target row of the
+// tile but only vcount + jblock - 1 target columns, so the LDS footprint need
+// not grow with the chip width (one block covers the whole range when it fits;
+// the autotuner also weighs smaller blocks, which leave room for more
+// workgroups per CU).  The block
 // is stored column by column with an odd column stride: a lane's walk down a
 // chip column reads consecutive words (one base address per column, the row
 // in the instruction's offset), and the lanes of consecutive placement

@@ -346,12 +346,13 @@ __global__ __launch_bounds__(1024) void k_correlate(const float *__restrict__ rT
     for (int a = 0; a < BU; ++a) acc[a] = 0.0f;
 #pragma unroll
     for (int a = 0; a < (EXACT ? BU : 1); ++a) total[a] = 0.0;
-    // the chip columns of this range are one run of blocks of BU values
-    // (column j at rp + j C); a block is loaded (vector loads: the LDS reads
-    // keep their own counter) while the one before it is used
+    // the chip columns of this range are one run of values (column j at
+    // rp + j C), taken in blocks of BU; a block is loaded (vector loads: the
+    // LDS reads keep their own counter) while the one before it is used.  A
+    // load past the pair's chip reads zeros (buffer bounds), so a block
+    // fetched ahead needs no clamp; only blocks inside a column are used.
     const __amdgpu_buffer_rsrc_t crs =
         __builtin_amdgcn_make_buffer_rsrc((void *)rp, (short)0, C * C * 4, kBufferWord3);
-    const int last = ((j0 + jlen) * C - BU) * 4;
     int off = j0 * C * 4;
     float ra[BU], rb[BU];
     chip_block<BU>(crs, off, ra);
@@ -363,6 +364,24 @@ __global__ __launch_bounds__(1024) void k_correlate(const float *__restrict__ rT
             w[(s + BU - 1) % BU] = tc[i0 + s + BU - 1];
 #pragma unroll
             for (int a = 0; a < BU; ++a) acc[a] = fmaf(r[s], w[(s + a) % BU], acc[a]);
+        }
+    };
+    // the last n < BU steps of a column (C not a multiple of BU), their chip
+    // values loaded one by one at off
+    auto step_part = [&](const float *tc, int i0, int n) {
+        float r[BU];
+#pragma unroll
+        for (int s = 0; s < BU; ++s)
+            r[s] = s < n ? __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                                         crs, off + 4 * s, 0, 0))
+                         : 0.0f;
+#pragma unroll
+        for (int s = 0; s < BU; ++s) {
+            if (s < n) {
+                w[(s + BU - 1) % BU] = tc[i0 + s + BU - 1];
+#pragma unroll
+                for (int a = 0; a < BU; ++a) acc[a] = fmaf(r[s], w[(s + a) % BU], acc[a]);
+            }
         }
     };
 
@@ -390,18 +409,25 @@ __global__ __launch_bounds__(1024) void k_correlate(const float *__restrict__ rT
             for (int a = 0; a < BU - 1; ++a) w[a] = tc[a];
             int i0 = 0;
             for (; i0 + 2 * BU <= C; i0 += 2 * BU) {
-                chip_block<BU>(crs, min(off + 4 * BU, last), rb);
+                chip_block<BU>(crs, off + 4 * BU, rb);
                 step(tc, i0, ra);
-                chip_block<BU>(crs, min(off + 8 * BU, last), ra);
+                chip_block<BU>(crs, off + 8 * BU, ra);
                 step(tc, i0 + BU, rb);
                 off += 8 * BU;
             }
-            if (i0 < C) {  // C / BU odd: one block left
-                chip_block<BU>(crs, min(off + 4 * BU, last), rb);
+            if (i0 + BU <= C) {  // one whole block left
+                chip_block<BU>(crs, off + 4 * BU, rb);
                 step(tc, i0, ra);
 #pragma unroll
                 for (int s = 0; s < BU; ++s) ra[s] = rb[s];
                 off += 4 * BU;
+                i0 += BU;
+            }
+            if (i0 < C) {  // part of a block ends the column
+                const int n = C - i0;
+                step_part(tc, i0, n);
+                off += 4 * n;
+                chip_block<BU>(crs, off, ra);   // the next column's first block
             }
             if constexpr (EXACT) {
 #pragma unroll
@@ -3296,7 +3322,7 @@ CorrelateShape choose_correlate_shape(int C, int P, bool exact, int bu, int nspl
 {
     CorrelateShape s{};
     s.exact = exact;
-    if (bu > 0 && C % bu == 0 && (bu == 1 || bu == 2 || bu == 4 || bu == 8 || bu == 16))
+    if (bu == 1 || bu == 2 || bu == 4 || bu == 5 || bu == 8 || bu == 16)
         s.bu = bu;
     else if (!exact && C % 4 == 0)
         s.bu = 4;  // measured best for the split refined contraction
@@ -3404,6 +3430,7 @@ hipError_t launch_correlate(hipStream_t s, const CorrelateShape &sh, const float
     switch (sh.bu) {                                 \
     case 16: AMPCOR_CORRELATE(16, EX); break;        \
     case 8: AMPCOR_CORRELATE(8, EX); break;          \
+    case 5: AMPCOR_CORRELATE(5, EX); break;          \
     case 4: AMPCOR_CORRELATE(4, EX); break;          \
     case 2: AMPCOR_CORRELATE(2, EX); break;          \
     case 1: AMPCOR_CORRELATE(1, EX); break;          \

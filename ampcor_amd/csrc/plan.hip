@@ -1149,7 +1149,7 @@ int ampcor_hip_plan_autotune(ampcor_hip_plan *plan, int reps)
         CorrelateShape pick = plan->coarse_shape;
         for (int bu : {4, 8, 16})
             for (int jb : {0, 16, 8, 4}) {
-                if (rc || plan->C % bu) continue;
+                if (rc) continue;
                 CorrelateShape sh = choose_correlate_shape(plan->C, plan->P, true, bu, 0, jb);
                 if (jb && sh.jblock != jb) continue;   // (the default block already)
                 float ms;
@@ -1164,10 +1164,10 @@ int ampcor_hip_plan_autotune(ampcor_hip_plan *plan, int reps)
         plan->coarse_shape = pick;
         best = 1e30f;
         pick = plan->refined_shape;
-        for (int bu : {4, 8, 16})
+        for (int bu : {4, 5, 8, 16})
             for (int ns : {1, 2, 4, 8, 16})
                 for (int jb : {0, 16, 8, 4}) {
-                    if (rc || plan->Cr % bu || plan->Cr % ns || plan->Cr / ns < 8) continue;
+                    if (rc || plan->Cr % ns || plan->Cr / ns < 8) continue;
                     CorrelateShape sh =
                         choose_correlate_shape(plan->Cr, plan->R, false, bu, ns, jb);
                     if (sh.lds_bytes > 150 * 1024 || (jb && sh.jblock != jb)) continue;

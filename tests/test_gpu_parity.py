@@ -68,6 +68,19 @@ def test_tuned_plan_matches_oracles(lib):
     check_against_oracles(hip, twin, exact)
 
 
+def test_blocks_that_do_not_divide_the_chip(lib, monkeypatch):
+    """Five placement rows per lane (the chips of 32 and 64 samples end each
+    column's chain with a part of a block): the coarse surfaces stay bitwise
+    the FP32 twin's, the refined ones within the FP64 bars."""
+    monkeypatch.setenv("AMPCOR_HIP_TILE_COARSE", "5")
+    monkeypatch.setenv("AMPCOR_HIP_TILE_REFINED", "5,4")
+    s = synth.shifted_scene(seed=6)
+    hip = run_hip(lib, s.reference, s.target, s.ref_begin, s.tgt_begin, s.chip, s.window,
+                  s.refine_factor, s.refine_margin, s.zoom_factor)
+    twin, exact = oracles(s)
+    check_against_oracles(hip, twin, exact)
+
+
 def test_golden_shift_end_to_end(lib):
     """The committed fixture: every pair recovers the planted Fourier shift."""
     import os

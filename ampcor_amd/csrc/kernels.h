@@ -56,8 +56,9 @@ CorrelateShape choose_correlate_shape(int C, int P, bool exact, int bu = 0, int 
 
 hipError_t launch_gather_detect(hipStream_t s, const PairDesc *desc, int pairs, int dtype, int C,
                                 int W, float *ref_amp, float *tgt_amp);
+// exact: the FP32 twin's order (the coarse stage); otherwise any order
 hipError_t launch_chip_stats(hipStream_t s, const float *amp, int pairs, int C, float *rT,
-                             double2 *stats);
+                             double2 *stats, bool exact = true);
 // FP64 sums of t and t^2 over every C x C placement of each W x W window: [pairs][P][P]
 hipError_t launch_window_sums(hipStream_t s, const float *amp, int pairs, int W, int C,
                               double2 *sums);

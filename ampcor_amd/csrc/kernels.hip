@@ -81,12 +81,14 @@ __global__ __launch_bounds__(256) void k_gather_detect(const PairDesc *__restric
 // ---------------------------------------------------------------------------
 constexpr int kStatThreads = 256;
 
-__device__ inline double tree256(double *p, double x)
+// the sum of one value per thread (NT threads) by a fixed halving tree
+template <int NT>
+__device__ inline double tree_sum(double *p, double x)
 {
     const int t = threadIdx.x;
     p[t] = x;
     __syncthreads();
-    for (int s = kStatThreads / 2; s > 0; s >>= 1) {
+    for (int s = NT / 2; s > 0; s >>= 1) {
         if (t < s) p[t] = p[t] + p[t + s];
         __syncthreads();
     }
@@ -97,14 +99,15 @@ __device__ inline double tree256(double *p, double x)
 
 // STAGE: the chip is read from global memory once, into LDS (C^2 floats), and
 // the three passes read it there -- the same values in the same order, so the
-// results are those of the unstaged kernel, bit for bit
-template <bool STAGE>
-__global__ __launch_bounds__(kStatThreads) void k_chip_stats(const float *__restrict__ amp, int C,
-                                                             float *__restrict__ rT,
-                                                             double2 *__restrict__ stats)
+// results are those of the unstaged kernel, bit for bit.  NT threads: the
+// order of the sums is NT-way, so only NT = kStatThreads reproduces the FP32
+// twin's coarse stage; the refined stage (checked by tolerance) may take 1024.
+template <bool STAGE, int NT>
+__global__ __launch_bounds__(NT) void k_chip_stats(const float *__restrict__ amp, int C,
+                                                   float *__restrict__ rT,
+                                                   double2 *__restrict__ stats)
 {
-    __shared__ double p1[kStatThreads];
-    __shared__ double p2[kStatThreads];
+    __shared__ double p1[NT];
     __shared__ float tile[32][33];
     extern __shared__ float staged[];
     const int pair = blockIdx.x;
@@ -115,40 +118,36 @@ __global__ __launch_bounds__(kStatThreads) void k_chip_stats(const float *__rest
     if constexpr (STAGE) {
         const float4 *a4 = reinterpret_cast<const float4 *>(a);
         float4 *s4 = reinterpret_cast<float4 *>(staged);
-        for (int k = t; k < cells / 4; k += kStatThreads) s4[k] = a4[k];
-        for (int k = cells / 4 * 4 + t; k < cells; k += kStatThreads) staged[k] = a[k];
+        for (int k = t; k < cells / 4; k += NT) s4[k] = a4[k];
+        for (int k = cells / 4 * 4 + t; k < cells; k += NT) staged[k] = a[k];
         __syncthreads();
         a = staged;
     }
 
     double acc = 0.0;
-    for (int k = t; k < cells; k += kStatThreads) acc = acc + (double)a[k];
-    const double mean = tree256(p1, acc) / (double)cells;
+    for (int k = t; k < cells; k += NT) acc = acc + (double)a[k];
+    const double mean = tree_sum<NT>(p1, acc) / (double)cells;
 
     double a1 = 0.0, a2 = 0.0;
-    for (int k = t; k < cells; k += kStatThreads) {
+    for (int k = t; k < cells; k += NT) {
         const double dr = (double)(float)((double)a[k] - mean);
         a1 = a1 + dr;
         a2 = a2 + dr * dr;
     }
-    p2[t] = a2;
-    const double s1 = tree256(p1, a1);
-    for (int s = kStatThreads / 2; s > 0; s >>= 1) {
-        if (t < s) p2[t] = p2[t] + p2[t + s];
-        __syncthreads();
-    }
-    if (t == 0) stats[pair] = make_double2(s1, sqrt(p2[0]));
+    const double s1 = tree_sum<NT>(p1, a1);
+    const double s2 = tree_sum<NT>(p1, a2);
+    if (t == 0) stats[pair] = make_double2(s1, sqrt(s2));
 
     // the zero-mean chip, transposed through LDS in 32 x 32 blocks
     const int c = t & 31, r0 = t >> 5;
     for (int bi = 0; bi < C; bi += 32)
         for (int bj = 0; bj < C; bj += 32) {
-            for (int r = r0; r < 32; r += kStatThreads / 32) {
+            for (int r = r0; r < 32; r += NT / 32) {
                 const int i = bi + r, j = bj + c;
                 tile[r][c] = (i < C && j < C) ? (float)((double)a[(size_t)i * C + j] - mean) : 0.0f;
             }
             __syncthreads();
-            for (int r = r0; r < 32; r += kStatThreads / 32) {
+            for (int r = r0; r < 32; r += NT / 32) {
                 const int j = bj + r, i = bi + c;
                 if (i < C && j < C) out[(size_t)j * C + i] = tile[c][r];
             }
@@ -3393,18 +3392,22 @@ hipError_t launch_gather_detect(hipStream_t s, const PairDesc *desc, int pairs, 
 }
 
 hipError_t launch_chip_stats(hipStream_t s, const float *amp, int pairs, int C, float *rT,
-                             double2 *stats)
+                             double2 *stats, bool exact)
 {
-    // staged in LDS where the chip fits 32 KiB (measured: 64-sample chips 0.100
-    // -> 0.064 ms; a 128-sample chip's 64 KiB leaves two workgroups per CU and
-    // ran slower, 0.41 -> 0.44 ms) and its rows start 16-byte aligned
+    // staged in LDS where the chip fits and its rows start 16-byte aligned:
+    // up to 32 KiB in 256-thread workgroups (measured: 64-sample chips 0.100 ->
+    // 0.064 ms); where the order need not be the twin's, up to 64 KiB in
+    // 1024-thread workgroups (two per CU still fill it)
     const size_t bytes = (size_t)C * C * sizeof(float);
     if (bytes <= 32 * 1024 && C % 4 == 0)
-        hipLaunchKernelGGL(k_chip_stats<true>, dim3(pairs), dim3(kStatThreads), bytes, s, amp, C,
+        hipLaunchKernelGGL((k_chip_stats<true, kStatThreads>), dim3(pairs), dim3(kStatThreads),
+                           bytes, s, amp, C, rT, stats);
+    else if (!exact && bytes <= 64 * 1024 && C % 4 == 0)
+        hipLaunchKernelGGL((k_chip_stats<true, 1024>), dim3(pairs), dim3(1024), bytes, s, amp, C,
                            rT, stats);
     else
-        hipLaunchKernelGGL(k_chip_stats<false>, dim3(pairs), dim3(kStatThreads), 0, s, amp, C, rT,
-                           stats);
+        hipLaunchKernelGGL((k_chip_stats<false, kStatThreads>), dim3(pairs), dim3(kStatThreads), 0,
+                           s, amp, C, rT, stats);
     return hipGetLastError();
 }
 

@@ -763,7 +763,8 @@ int plan_enqueue(ampcor_hip_plan *plan, int N)
                                    p->mstats_r.ptr, p->chip_lo_r.ptr, p->tgt_lo_r.ptr),
                  "prep_refined");
         else
-            STEP(launch_chip_stats(s, p->amp_ref_r.ptr, N, p->Cr, p->rT_r.ptr, p->stats_r.ptr),
+            STEP(launch_chip_stats(s, p->amp_ref_r.ptr, N, p->Cr, p->rT_r.ptr, p->stats_r.ptr,
+                                   false),
                  "chip_stats_refined");
     }
     if (!fuse_r)

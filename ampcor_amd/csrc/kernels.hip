@@ -3397,7 +3397,8 @@ hipError_t launch_chip_stats(hipStream_t s, const float *amp, int pairs, int C, 
     // staged in LDS where the chip fits and its rows start 16-byte aligned:
     // up to 32 KiB in 256-thread workgroups (measured: 64-sample chips 0.100 ->
     // 0.064 ms); where the order need not be the twin's, up to 64 KiB in
-    // 1024-thread workgroups (two per CU still fill it)
+    // 1024-thread workgroups (two per CU still fill it), and larger chips
+    // unstaged in 1024-thread workgroups
     const size_t bytes = (size_t)C * C * sizeof(float);
     if (bytes <= 32 * 1024 && C % 4 == 0)
         hipLaunchKernelGGL((k_chip_stats<true, kStatThreads>), dim3(pairs), dim3(kStatThreads),
@@ -3405,6 +3406,9 @@ hipError_t launch_chip_stats(hipStream_t s, const float *amp, int pairs, int C, 
     else if (!exact && bytes <= 64 * 1024 && C % 4 == 0)
         hipLaunchKernelGGL((k_chip_stats<true, 1024>), dim3(pairs), dim3(1024), bytes, s, amp, C,
                            rT, stats);
+    else if (!exact)
+        hipLaunchKernelGGL((k_chip_stats<false, 1024>), dim3(pairs), dim3(1024), 0, s, amp, C, rT,
+                           stats);
     else
         hipLaunchKernelGGL((k_chip_stats<false, kStatThreads>), dim3(pairs), dim3(kStatThreads), 0,
                            s, amp, C, rT, stats);

@@ -397,26 +397,40 @@ __global__ __launch_bounds__(1024) void k_correlate(const float *__restrict__ rT
         }
     };
 
+    // The LDS block is a ring of `cols` target columns: target column
+    // v0 + j0 + g (g >= 0) sits in slot g mod cols.  The first pass stages
+    // columns g < cols; each later pass (chip columns jb .. jb + jn - 1 of the
+    // range) needs g in [jb, jb + vcount + jn - 1) and stages only the jn
+    // columns entering, into the slots of the columns the previous pass was
+    // the last to use.
+    const int gbase = v0 + j0;
     for (int jb = 0; jb < jlen; jb += jblock) {
         const int jn = min(jblock, jlen - jb);
-        // stage target rows [u0, u0 + rows) x columns [v0 + j0 + jb, + cols)
         __syncthreads();
-        const int cbase = v0 + j0 + jb;
-        {
+        if (jb == 0) {
             // (r, c) of element k = r cols + c, advanced without a division
             const int dr = blockDim.x / cols, dc = blockDim.x - dr * cols;
             int r = threadIdx.x / cols, c = threadIdx.x - r * cols;
             for (; r < rows; r += dr, c += dc) {
                 if (c >= cols) c -= cols, ++r;
                 if (r >= rows) break;
-                const int gr = u0 + r, gc = cbase + c;
+                const int gr = u0 + r, gc = gbase + c;
                 tile[c * cs + r] = (gr < W && gc < W) ? src[(size_t)gr * W + gc] : 0.0f;
+            }
+        } else {
+            const int g0 = jb + vcount - 1;   // first entering column
+            for (int k = threadIdx.x; k < rows * jn; k += blockDim.x) {
+                const int r = k / jn, c = k - r * jn;
+                const int gr = u0 + r, gc = gbase + g0 + c;
+                tile[((g0 + c) % cols) * cs + r] =
+                    (gr < W && gc < W) ? src[(size_t)gr * W + gc] : 0.0f;
             }
         }
         __syncthreads();
         if (!active) continue;
-        for (int jj = 0; jj < jn; ++jj) {
-            const float *tc = tile + (size_t)(v + jj) * cs + ubk * BU;
+        int slot = (v + jb) % cols;
+        for (int jj = 0; jj < jn; ++jj, slot = slot + 1 == cols ? 0 : slot + 1) {
+            const float *tc = tile + (size_t)slot * cs + ubk * BU;
 #pragma unroll
             for (int a = 0; a < BU - 1; ++a) w[a] = tc[a];
             int i0 = 0;

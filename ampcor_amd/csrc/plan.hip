@@ -297,12 +297,14 @@ int plan_configure(ampcor_hip_plan *p, int device, size_t pairs, size_t ref_line
     p->have_ref.assign(pairs, 0);
     p->have_tgt.assign(pairs, 0);
     {
-        // tuning overrides: AMPCOR_HIP_TILE_COARSE="bu", AMPCOR_HIP_TILE_REFINED="bu,nsplit"
-        int cbu = 0, rbu = 0, rns = 0;
-        if (const char *e = std::getenv("AMPCOR_HIP_TILE_COARSE")) std::sscanf(e, "%d", &cbu);
-        if (const char *e = std::getenv("AMPCOR_HIP_TILE_REFINED")) std::sscanf(e, "%d,%d", &rbu, &rns);
-        p->coarse_shape = choose_correlate_shape(p->C, p->P, true, cbu, 0);
-        p->refined_shape = choose_correlate_shape(p->Cr, p->R, false, rbu, rns);
+        // tiling overrides of the FP32 contractions: AMPCOR_HIP_TILE_COARSE="bu[,jblock]",
+        // AMPCOR_HIP_TILE_REFINED="bu,nsplit[,jblock]"
+        int cbu = 0, cjb = 0, rbu = 0, rns = 0, rjb = 0;
+        if (const char *e = std::getenv("AMPCOR_HIP_TILE_COARSE")) std::sscanf(e, "%d,%d", &cbu, &cjb);
+        if (const char *e = std::getenv("AMPCOR_HIP_TILE_REFINED"))
+            std::sscanf(e, "%d,%d,%d", &rbu, &rns, &rjb);
+        p->coarse_shape = choose_correlate_shape(p->C, p->P, true, cbu, 0, cjb);
+        p->refined_shape = choose_correlate_shape(p->Cr, p->R, false, rbu, rns, rjb);
     }
     p->mixed = precision != AMPCOR_HIP_FP32;
     p->split = precision == AMPCOR_HIP_BF16X3 || precision == AMPCOR_HIP_FP16X3;

@@ -81,6 +81,20 @@ def test_blocks_that_do_not_divide_the_chip(lib, monkeypatch):
     check_against_oracles(hip, twin, exact)
 
 
+@pytest.mark.parametrize("jblock", [4, 7])
+def test_chip_columns_in_several_lds_passes(lib, monkeypatch, jblock):
+    """The FP32 contractions with the chip columns taken a few at a time (the
+    LDS block a ring of target columns; 7 leaves a shorter last pass): coarse
+    surfaces bitwise the twin's, refined ones within the FP64 bars."""
+    monkeypatch.setenv("AMPCOR_HIP_TILE_COARSE", f"8,{jblock}")
+    monkeypatch.setenv("AMPCOR_HIP_TILE_REFINED", f"4,2,{jblock}")
+    s = synth.shifted_scene(seed=8)
+    hip = run_hip(lib, s.reference, s.target, s.ref_begin, s.tgt_begin, s.chip, s.window,
+                  s.refine_factor, s.refine_margin, s.zoom_factor)
+    twin, exact = oracles(s)
+    check_against_oracles(hip, twin, exact)
+
+
 def test_golden_shift_end_to_end(lib):
     """The committed fixture: every pair recovers the planted Fourier shift."""
     import os

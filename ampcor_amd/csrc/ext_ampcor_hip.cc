@@ -14,7 +14,8 @@
 // raster(buffer, lines, samples[, dtype, device]), slc_pixelSize(),
 // slc_getitem(raster, offset); the CPU module's refine(worker); and the
 // batched and product accessors: addPairs, products, dims, stage_gamma,
-// stage_gamma_refined, stage_locations, stage_upsample, stage_contract, stage_zoom, zoom_stats,
+// stage_gamma_refined, stage_locations, stage_upsample, stage_contract, stage_window_sums,
+// stage_amplitudes, stage_zoom, zoom_stats,
 // fp8_guard, set_timing, stage_times.
 //
 // Unlike the reference, adjust releases the GIL (a worker busy in another
@@ -628,6 +629,91 @@ PyObject *stage_contract(PyObject *, PyObject *args)
     return out;
 }
 
+// stage_window_sums(worker, stage, windows, w, count) -> bytes: the FP64 window
+// sums (sum t, sum t^2) over every chip-sized placement of float32 windows
+// [count][w][w]; returns float64 [count][p][p][2]
+PyObject *stage_window_sums(PyObject *, PyObject *args)
+{
+    PyObject *pyWorker;
+    int stage, wd;
+    Py_buffer win;
+    unsigned long count;
+    if (!PyArg_ParseTuple(args, "O!iy*ik:stage_window_sums", &PyCapsule_Type, &pyWorker, &stage,
+                          &win, &wd, &count))
+        return nullptr;
+    Worker *w = get_worker(pyWorker);
+    if (!w) {
+        PyBuffer_Release(&win);
+        return nullptr;
+    }
+    int P, R, Z;
+    int rc = ampcor_hip_plan_dims(w->plan, &P, &R, &Z);
+    if (rc) {
+        PyBuffer_Release(&win);
+        return raise(rc);
+    }
+    if (wd <= 0 || (size_t)win.len != 4 * count * (size_t)wd * wd || (stage != 0 && stage != 1)) {
+        PyBuffer_Release(&win);
+        PyErr_SetString(PyExc_ValueError, "windows do not hold count x w^2 float32 values, or "
+                                          "stage is not 0 or 1");
+        return nullptr;
+    }
+    const size_t p = (size_t)(stage ? R : P);
+    PyObject *out = bytes_of(16 * count * p * p);
+    if (!out) {
+        PyBuffer_Release(&win);
+        return nullptr;
+    }
+    w->busy = true;
+    Py_BEGIN_ALLOW_THREADS
+    rc = ampcor_hip_stage_window_sums(w->plan, stage, static_cast<const float *>(win.buf), wd, count,
+                                      reinterpret_cast<double *>(PyBytes_AS_STRING(out)));
+    Py_END_ALLOW_THREADS
+    w->busy = false;
+    PyBuffer_Release(&win);
+    if (rc) {
+        Py_DECREF(out);
+        return raise(rc);
+    }
+    return out;
+}
+
+// stage_amplitudes(worker, chip, window) -> (chips, windows): float32 bytes of
+// the amplitudes the last adjust of an FP32 worker detected, [pairs][chip][chip]
+// and [pairs][window][window]
+PyObject *stage_amplitudes(PyObject *, PyObject *args)
+{
+    PyObject *pyWorker;
+    int c, wd;
+    if (!PyArg_ParseTuple(args, "O!ii:stage_amplitudes", &PyCapsule_Type, &pyWorker, &c, &wd))
+        return nullptr;
+    Worker *w = get_worker(pyWorker);
+    if (!w) return nullptr;
+    if (c <= 0 || wd < c) {
+        PyErr_SetString(PyExc_ValueError, "chip and window must be positive, window >= chip");
+        return nullptr;
+    }
+    const size_t pairs = w->pairs, C = (size_t)c, W = (size_t)wd;
+    int rc;
+    PyObject *chips = bytes_of(4 * pairs * C * C);
+    PyObject *wins = chips ? bytes_of(4 * pairs * W * W) : nullptr;
+    if (!chips || !wins) {
+        Py_XDECREF(chips);
+        return nullptr;
+    }
+    Py_BEGIN_ALLOW_THREADS
+    rc = ampcor_hip_stage_amplitudes(w->plan, c, wd,
+                                     reinterpret_cast<float *>(PyBytes_AS_STRING(chips)),
+                                     reinterpret_cast<float *>(PyBytes_AS_STRING(wins)));
+    Py_END_ALLOW_THREADS
+    if (rc) {
+        Py_DECREF(chips);
+        Py_DECREF(wins);
+        return raise(rc);
+    }
+    return Py_BuildValue("(NN)", chips, wins);
+}
+
 // stage_zoom(worker, surfaces): the zoom stage alone on float32 refined
 // surfaces [pairs][R][R]; products() and stage_locations() then read its results
 PyObject *stage_zoom(PyObject *, PyObject *args)
@@ -912,6 +998,8 @@ PyMethodDef methods[] = {
     {"stage_upsample", (PyCFunction)(void (*)(void))stage_upsample, METH_VARARGS | METH_KEYWORDS,
      "run the target upsampling on given tiles"},
     {"stage_contract", stage_contract, METH_VARARGS, "run one contraction on given operands"},
+    {"stage_window_sums", stage_window_sums, METH_VARARGS, "FP64 window sums of given windows"},
+    {"stage_amplitudes", stage_amplitudes, METH_VARARGS, "the last FP32 adjust's amplitudes"},
     {"zoom_stats", zoom_stats, METH_VARARGS, "the zoom screen's exact re-evaluations"},
     {"set_timing", set_timing, METH_VARARGS, "enable per-stage device timing"},
     {"stage_times", stage_times, METH_VARARGS, "per-stage device milliseconds of the last adjust"},

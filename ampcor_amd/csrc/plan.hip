@@ -956,6 +956,55 @@ int ampcor_hip_stage_upsample(ampcor_hip_plan *plan, int dtype, const void *tile
     return rc;
 }
 
+int ampcor_hip_stage_window_sums(ampcor_hip_plan *plan, int stage, const float *windows, int w,
+                                 size_t count, double *sums)
+{
+    if (!plan || !windows || !sums) return fail(AMPCOR_HIP_EINVAL, "null argument");
+    if (stage != 0 && stage != 1) return fail(AMPCOR_HIP_EINVAL, "stage must be 0 or 1");
+    if (count == 0 || count > plan->pairs)
+        return fail(AMPCOR_HIP_EINVAL, "count must be in [1, pairs]");
+    const int C = stage ? plan->Cr : plan->C, W = stage ? plan->Wr : plan->W, P = W - C + 1;
+    if (w != W) return fail(AMPCOR_HIP_EINVAL, "window size is not the stage's");
+    DeviceGuard guard(plan->device);
+    hipStream_t s = plan->stream;
+    float *amp = stage ? plan->amp_tgt_r.ptr : plan->amp_tgt.ptr;
+    double2 *out = stage ? plan->sums_r.ptr : plan->sums.ptr;
+    hipError_t e = hipMemcpyAsync(amp, windows, count * (size_t)W * W * sizeof(float),
+                                  hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = launch_window_sums(s, amp, (int)count, W, C, out);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(sums, out, count * (size_t)P * P * sizeof(double2),
+                           hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        (void)hipStreamSynchronize(s);
+        return hip_fail(e, "running the window sums");
+    }
+    return AMPCOR_HIP_OK;
+}
+
+int ampcor_hip_stage_amplitudes(ampcor_hip_plan *plan, int c, int w, float *chips,
+                                float *windows)
+{
+    if (!plan || !chips || !windows) return fail(AMPCOR_HIP_EINVAL, "null argument");
+    if (c != plan->C || w != plan->W)
+        return fail(AMPCOR_HIP_EINVAL, "chip / window sizes are not the plan's");
+    if (plan->mixed)
+        return fail(AMPCOR_HIP_EINVAL, "the detected amplitudes are kept by FP32 plans only");
+    if (plan->last_n <= 0) return fail(AMPCOR_HIP_ESTATE, "no adjust has run");
+    DeviceGuard guard(plan->device);
+    hipStream_t s = plan->stream;
+    const size_t n = (size_t)plan->last_n;
+    hipError_t e = hipMemcpyAsync(chips, plan->amp_ref.ptr, n * plan->C * plan->C * sizeof(float),
+                                  hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(windows, plan->amp_tgt.ptr, n * plan->W * plan->W * sizeof(float),
+                           hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(e, "copying the amplitudes");
+    return AMPCOR_HIP_OK;
+}
+
 int ampcor_hip_stage_contract(ampcor_hip_plan *plan, int stage, int form, const uint16_t *chip,
                               int c, const uint16_t *window, int w, size_t count,
                               float *numerators)

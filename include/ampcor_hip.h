@@ -167,6 +167,23 @@ int ampcor_hip_stage_upsample(ampcor_hip_plan *plan, int dtype, const void *tile
 int ampcor_hip_stage_contract(ampcor_hip_plan *plan, int stage, int form, const uint16_t *chip,
                               int c, const uint16_t *window, int w, size_t count,
                               float *numerators);
+/* the FP64 window sums (sum t, sum t^2) over every chip-sized placement of
+ * host FP32 windows [count][w][w], as the FP32 form's window-sum kernel
+ * computes them for stage 0 (chip C, window W) or stage 1 (rf C, rf E):
+ * sums [count][p][p][2], p = w - c + 1, count <= pairs; w must be the
+ * stage's window size.  No counterpart in the ABI of the reference, whose
+ * tests reach its tables through SumArea and Sequential::_sat
+ * (tests/libampcor/sumarea-sum.cc:47-148, tests/cuda/tile-sat.cu:216-270).
+ * Overwrites the last adjust's intermediate buffers. */
+int ampcor_hip_stage_window_sums(ampcor_hip_plan *plan, int stage, const float *windows, int w,
+                                 size_t count, double *sums);
+/* the amplitudes the last adjust of an FP32 plan detected (|z| of complex
+ * pixels, |x| of real ones): chips [pairs][c][c], windows [pairs][w][w], c
+ * and w the plan's chip and window sizes; the counterpart of the reference's
+ * detect step (detect.cu:105-112), which its tests check directly
+ * (tests/libampcor_cuda/detect.cc:202-215) */
+int ampcor_hip_stage_amplitudes(ampcor_hip_plan *plan, int c, int w, float *chips,
+                                float *windows);
 /* run the zoom stage alone on host refined surfaces [pairs][R][R] (the
  * reference drives _zoomcor the same way in its tests): the products then hold
  * its results -- refined peak, zoomed location (stage_locations), and offsets

@@ -678,14 +678,16 @@ PyObject *stage_window_sums(PyObject *, PyObject *args)
     return out;
 }
 
-// stage_amplitudes(worker, chip, window) -> (chips, windows): float32 bytes of
-// the amplitudes the last adjust of an FP32 worker detected, [pairs][chip][chip]
-// and [pairs][window][window]
+// stage_amplitudes(worker, stage, chip, window) -> (chips, windows): float32
+// bytes of the amplitudes the last adjust of an FP32 worker worked on, stage 0
+// the detected ones, 1 the refined ones: [pairs][chip][chip] and
+// [pairs][window][window]
 PyObject *stage_amplitudes(PyObject *, PyObject *args)
 {
     PyObject *pyWorker;
-    int c, wd;
-    if (!PyArg_ParseTuple(args, "O!ii:stage_amplitudes", &PyCapsule_Type, &pyWorker, &c, &wd))
+    int stage, c, wd;
+    if (!PyArg_ParseTuple(args, "O!iii:stage_amplitudes", &PyCapsule_Type, &pyWorker, &stage, &c,
+                          &wd))
         return nullptr;
     Worker *w = get_worker(pyWorker);
     if (!w) return nullptr;
@@ -702,7 +704,7 @@ PyObject *stage_amplitudes(PyObject *, PyObject *args)
         return nullptr;
     }
     Py_BEGIN_ALLOW_THREADS
-    rc = ampcor_hip_stage_amplitudes(w->plan, c, wd,
+    rc = ampcor_hip_stage_amplitudes(w->plan, stage, c, wd,
                                      reinterpret_cast<float *>(PyBytes_AS_STRING(chips)),
                                      reinterpret_cast<float *>(PyBytes_AS_STRING(wins)));
     Py_END_ALLOW_THREADS

@@ -983,23 +983,24 @@ int ampcor_hip_stage_window_sums(ampcor_hip_plan *plan, int stage, const float *
     return AMPCOR_HIP_OK;
 }
 
-int ampcor_hip_stage_amplitudes(ampcor_hip_plan *plan, int c, int w, float *chips,
+int ampcor_hip_stage_amplitudes(ampcor_hip_plan *plan, int stage, int c, int w, float *chips,
                                 float *windows)
 {
     if (!plan || !chips || !windows) return fail(AMPCOR_HIP_EINVAL, "null argument");
-    if (c != plan->C || w != plan->W)
-        return fail(AMPCOR_HIP_EINVAL, "chip / window sizes are not the plan's");
+    if (stage != 0 && stage != 1) return fail(AMPCOR_HIP_EINVAL, "stage must be 0 or 1");
+    const int C = stage ? plan->Cr : plan->C, W = stage ? plan->Wr : plan->W;
+    if (c != C || w != W) return fail(AMPCOR_HIP_EINVAL, "chip / window sizes are not the stage's");
     if (plan->mixed)
         return fail(AMPCOR_HIP_EINVAL, "the detected amplitudes are kept by FP32 plans only");
     if (plan->last_n <= 0) return fail(AMPCOR_HIP_ESTATE, "no adjust has run");
     DeviceGuard guard(plan->device);
     hipStream_t s = plan->stream;
     const size_t n = (size_t)plan->last_n;
-    hipError_t e = hipMemcpyAsync(chips, plan->amp_ref.ptr, n * plan->C * plan->C * sizeof(float),
-                                  hipMemcpyDeviceToHost, s);
+    const float *ca = stage ? plan->amp_ref_r.ptr : plan->amp_ref.ptr;
+    const float *wa = stage ? plan->amp_tgt_r.ptr : plan->amp_tgt.ptr;
+    hipError_t e = hipMemcpyAsync(chips, ca, n * C * C * sizeof(float), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(windows, plan->amp_tgt.ptr, n * plan->W * plan->W * sizeof(float),
-                           hipMemcpyDeviceToHost, s);
+        e = hipMemcpyAsync(windows, wa, n * W * W * sizeof(float), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_fail(e, "copying the amplitudes");
     return AMPCOR_HIP_OK;

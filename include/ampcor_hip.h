@@ -177,12 +177,14 @@ int ampcor_hip_stage_contract(ampcor_hip_plan *plan, int stage, int form, const 
  * Overwrites the last adjust's intermediate buffers. */
 int ampcor_hip_stage_window_sums(ampcor_hip_plan *plan, int stage, const float *windows, int w,
                                  size_t count, double *sums);
-/* the amplitudes the last adjust of an FP32 plan detected (|z| of complex
- * pixels, |x| of real ones): chips [pairs][c][c], windows [pairs][w][w], c
- * and w the plan's chip and window sizes; the counterpart of the reference's
- * detect step (detect.cu:105-112), which its tests check directly
- * (tests/libampcor_cuda/detect.cc:202-215) */
-int ampcor_hip_stage_amplitudes(ampcor_hip_plan *plan, int c, int w, float *chips,
+/* the amplitudes the last adjust of an FP32 plan worked on: stage 0 the
+ * detected chips and windows (|z| of complex pixels, |x| of real ones), stage
+ * 1 the refined ones (the migrated tiles, upsampled by rf, then detected);
+ * chips [pairs][c][c], windows [pairs][w][w], c and w the stage's sizes.  The
+ * counterparts of the reference's detect and migrate steps (detect.cu:105-112,
+ * migrate.cu), which its tests check directly
+ * (tests/libampcor_cuda/detect.cc:202-215, migrate.cc:350-426) */
+int ampcor_hip_stage_amplitudes(ampcor_hip_plan *plan, int stage, int c, int w, float *chips,
                                 float *windows);
 /* run the zoom stage alone on host refined surfaces [pairs][R][R] (the
  * reference drives _zoomcor the same way in its tests): the products then hold

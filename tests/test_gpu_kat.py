@@ -11,7 +11,11 @@ kernels (the same cases pin the oracle in tests/test_oracle.py):
   every tiling of the matrix-core contraction;
 - detection (tests/libampcor_cuda/detect.cc:202-215): the amplitudes the FP32
   form detects equal |z| rounded once to float32 (exact on Pythagorean triples),
-  and |x| for real rasters.
+  and |x| for real rasters;
+- migration (tests/libampcor_cuda/migrate.cc:350-426): the refined tiles hold
+  the pixels of the nudged windows -- with refine factor 1 the refined
+  amplitudes are those windows' |z| bitwise, and with refine factor 2 their
+  even samples are, to the 1 ulp of the upsampler's FP32 square root.
 """
 
 import numpy as np
@@ -107,7 +111,7 @@ def test_detected_amplitudes(lib, real):
     ras = lib.raster(raster, lines, samples, dtype="float32" if real else "complex64")
     lib.addPairs(worker, ras, ras, 0, ref_begin.tobytes(), tgt_begin.tobytes())
     lib.adjust(worker, offsets=False)
-    chips, wins = lib.stage_amplitudes(worker, CHIP, WINDOW)
+    chips, wins = lib.stage_amplitudes(worker, 0, CHIP, WINDOW)
     chips = np.frombuffer(chips, np.float32).reshape(n, CHIP, CHIP)
     wins = np.frombuffer(wins, np.float32).reshape(n, WINDOW, WINDOW)
     for k in range(n):
@@ -122,4 +126,41 @@ def test_detected_amplitudes(lib, real):
 def test_amplitudes_are_kept_by_fp32_plans_only(lib):
     worker = lib.sequential(1, (CHIP, CHIP), (WINDOW, WINDOW), RM, RF, ZF, precision="bf16")
     with pytest.raises(ValueError, match="FP32"):
-        lib.stage_amplitudes(worker, CHIP, WINDOW)
+        lib.stage_amplitudes(worker, 0, CHIP, WINDOW)
+
+
+@pytest.mark.parametrize("rf", [1, 2])
+def test_refined_tiles_hold_the_migrated_windows(lib, rf):
+    rng = np.random.default_rng(31 + rf)
+    lines = samples = 2 * WINDOW
+    raster = (rng.standard_normal((lines, samples)) +
+              1j * rng.standard_normal((lines, samples))).astype(np.complex64)
+    ref_begin = np.array([[8, 16], [40, 24], [60, 60]], np.int64)
+    tgt_begin = ref_begin - (WINDOW - CHIP) // 2
+    tgt_begin = np.clip(tgt_begin, 0, lines - WINDOW)
+    n = len(ref_begin)
+    worker = lib.sequential(n, (CHIP, CHIP), (WINDOW, WINDOW), RM, rf, ZF)
+    ras = lib.raster(raster, lines, samples)
+    lib.addPairs(worker, ras, ras, 0, ref_begin.tobytes(), tgt_begin.tobytes())
+    lib.adjust(worker, offsets=False)
+    E = CHIP + 2 * RM
+    chips, wins = lib.stage_amplitudes(worker, 1, rf * CHIP, rf * E)
+    chips = np.frombuffer(chips, np.float32).reshape(n, rf * CHIP, rf * CHIP)
+    wins = np.frombuffer(wins, np.float32).reshape(n, rf * E, rf * E)
+    top = np.frombuffer(lib.stage_locations(worker)["window"], np.int32).reshape(n, 2)
+
+    def amp(z):
+        return np.sqrt(z.real.astype(np.float64) ** 2 + z.imag.astype(np.float64) ** 2).astype(
+            np.float32)
+    for k in range(n):
+        r0, c0 = ref_begin[k]
+        t0, s0 = tgt_begin[k] + top[k]
+        want_c = amp(raster[r0:r0 + CHIP, c0:c0 + CHIP])
+        want_w = amp(raster[t0:t0 + E, s0:s0 + E])
+        print(f"rf {rf} pair {k}: window at {tuple(top[k])}", flush=True)
+        if rf == 1:
+            assert np.array_equal(chips[k], want_c)
+            assert np.array_equal(wins[k], want_w)
+        else:   # the x2 upsampler detects with the FP32 square root (within 1 ulp)
+            np.testing.assert_array_max_ulp(chips[k][::2, ::2], want_c, maxulp=1)
+            np.testing.assert_array_max_ulp(wins[k][::2, ::2], want_w, maxulp=1)

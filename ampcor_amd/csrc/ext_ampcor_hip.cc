@@ -581,7 +581,8 @@ PyObject *stage_upsample(PyObject *, PyObject *args, PyObject *kwds)
 // stage_contract(worker, stage, form, chip, c, window, w, count) -> bytes: one
 // contraction of a plain BF16 / FP16 worker on 16-bit operand patterns
 // chip [count][c][c], window [count][w][w]; returns float32 numerators
-// [count][p][p], p = w - c + 1 (form: 0 current, 1 tiles, 2 streaming, 3 packed)
+// [count][p][p], p = w - c + 1 (form: 0 current, 1 tiles, 2 streaming, 3 packed,
+// 4 / 5 packed in two / four chip-row ranges)
 PyObject *stage_contract(PyObject *, PyObject *args)
 {
     PyObject *pyWorker;

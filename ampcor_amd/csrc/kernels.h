@@ -157,9 +157,14 @@ MixedShape choose_mixed_shape(int C, int P, int nsplit = 0, bool split = false, 
 // the streaming form's shape (plain operands only); nk = 0 where it does not apply
 MixedShape choose_mixed_stream_shape(int C, int P, int max_vt = 0);
 // the packed form's shape (P = 32 NB + 1, chips of whole 16-column blocks;
-// split: hi / lo operands, three passes); nk = 0 where it does not apply
+// split: hi / lo operands, three passes; nsplit: chip-row ranges, a power of
+// two whose ranges hold whole chunks); nk = 0 where it does not apply
 MixedShape choose_mixed_packed_shape(int C, int P, int max_vt = 0, int wpe = 0,
-                                     bool split = false);
+                                     bool split = false, int nsplit = 1);
+// a packed shape launched over `pairs` pairs: its chip-row ranges raised (up to
+// maxsplit, whole chunks each) until the launch has enough waves to fill the
+// chip; other shapes unchanged
+MixedShape packed_fill_ranges(const MixedShape &sh, int C, int pairs, int maxsplit);
 // chip_lo / tgt_lo (optional): the rounding remainders, for the split form
 hipError_t launch_prep_mixed(hipStream_t s, const float *ref_amp, const float *tgt_amp, int pairs,
                              int C, int W, int CS, bool fp16, uint16_t *chip_q, uint16_t *tgt_q,

@@ -253,11 +253,12 @@ MixedShape choose_mixed_stream_shape(int C, int P, int max_vt)
     return s;
 }
 
-MixedShape choose_mixed_packed_shape(int C, int P, int max_vt, int wpe, bool split)
+MixedShape choose_mixed_packed_shape(int C, int P, int max_vt, int wpe, bool split, int nsplit)
 {
     MixedShape s{};
     const int nk = (C + 15 + 15) / 16;
     if (P < 33 || (P - 1) % 32 || C % 16 || !packed_nk_built(nk)) return s;
+    if (nsplit < 1 || (nsplit & (nsplit - 1)) || C % nsplit) return s;
     const int nb = (P - 1) / 32, vtt = 2 * nb + 1;
     // the (v-tiles, chunk rows, waves per SIMD) combinations, built where
     // their registers fit
@@ -269,7 +270,7 @@ MixedShape choose_mixed_packed_shape(int C, int P, int max_vt, int wpe, bool spl
         if ((max_vt > 0 && vt > max_vt) || (wpe && w != wpe)) continue;
         if (!packed_kind_fits(nk, vt, w)) continue;
         const int nvg = (vtt + vt - 1) / vt, waves = nb * nvg;
-        if (waves > std::min(packed_detail::kMaxWaves, 4 * w) || C % ch) continue;
+        if (waves > std::min(packed_detail::kMaxWaves, 4 * w) || (C / nsplit) % ch) continue;
         // one-wave workgroups refill four chip rows at a time (a smaller LDS
         // footprint, more workgroups per CU); larger ones eight
         if (ch != (waves == 1 ? 4 : 8)) continue;
@@ -287,12 +288,27 @@ MixedShape choose_mixed_packed_shape(int C, int P, int max_vt, int wpe, bool spl
     s.wg_vt = vt;
     s.tiles_u = nb;
     s.tiles_v = nvg;
-    s.nsplit = 1;
+    s.nsplit = nsplit;
     s.packed = kinds[best][1];
     s.wpe = kinds[best][2];
     s.lds_bytes = packed_lds_bytes(nk, vt, s.packed, nb, nvg);
     s.stream = false;
     return s;
+}
+
+MixedShape packed_fill_ranges(const MixedShape &sh, int C, int pairs, int maxsplit)
+{
+    // four waves per SIMD of the 256 CUs: below that, a launch of one- and
+    // two-wave workgroups leaves SIMDs idle (measured on the scene's refined
+    // contraction: 512 pairs 0.099 -> 0.050 ms in four ranges, 2048 pairs
+    // 0.154 -> 0.124 ms in two; no change from 3072 pairs up)
+    constexpr long kFillWaves = 4 * 4 * 256;
+    MixedShape r = sh;
+    if (!sh.packed || sh.nsplit < 1) return r;
+    while ((long)pairs * r.wg_ut * r.nsplit < kFillWaves && 2 * r.nsplit <= maxsplit &&
+           C % (2 * r.nsplit) == 0 && (C / (2 * r.nsplit)) % r.packed == 0)
+        r.nsplit *= 2;
+    return r;
 }
 
 hipError_t launch_prep_mixed(hipStream_t s, const float *ref_amp, const float *tgt_amp, int pairs,
@@ -322,8 +338,9 @@ hipError_t launch_correlate_mixed(hipStream_t s, const MixedShape &sh, const uin
         return hipErrorInvalidValue;
     if (sh.packed) {
         const int P = W - C + 1;
-        if (sh.stream || sh.nsplit != 1 || P < 33 || (P - 1) % 32 || C % 16 ||
-            C % sh.packed || sh.tiles_u != (P - 1) / 32 || sh.wg_ut != sh.tiles_u * sh.tiles_v ||
+        if (sh.stream || sh.nsplit < 1 || C % sh.nsplit || P < 33 || (P - 1) % 32 || C % 16 ||
+            (C / sh.nsplit) % sh.packed || sh.tiles_u != (P - 1) / 32 ||
+            sh.wg_ut != sh.tiles_u * sh.tiles_v ||
             (sh.wpe != 2 && sh.wpe != 3) || !packed_nk_built(sh.nk) ||
             !packed_kind_fits(sh.nk, sh.wg_vt, sh.wpe) ||
             sh.wg_ut > std::min(packed_detail::kMaxWaves, 4 * sh.wpe) ||

@@ -23,8 +23,11 @@
 // The last v-tile of a surface (v = P - 1 only) is an ordinary tile of which
 // one column is kept.
 //
-// A workgroup is one pair: NB u-blocks x nvg groups of VT v-tiles, one wave
-// each.  Tiles per pair and chip-row pair: 2 NB nvg VT + nvg against
+// A workgroup is one pair, or one of its nsplit ranges of C / nsplit chip
+// rows (each written to a partial surface of its own, which the normalizer
+// sums in order): NB u-blocks x nvg groups of VT v-tiles, one wave each.
+// Ranges give surfaces of one or two waves enough workgroups to spread evenly
+// over the CUs.  Tiles per pair and chip-row pair: 2 NB nvg VT + nvg against
 // (2 NB + 1)^2 for tiles of 16 consecutive rows (63 instead of 81 MFMAs per
 // K-step column for P = 33 and one group of 3 v-tiles; 105 instead of 125 for
 // P = 65).
@@ -79,7 +82,7 @@ template <int NK, int VT, int CH, int WPE, bool FP16>
 __global__ __attribute__((amdgpu_flat_work_group_size(64, 256 * WPE),
                           amdgpu_waves_per_eu(WPE, WPE))) void k_correlate_packed(
     const uint16_t *__restrict__ chip_q, const uint16_t *__restrict__ tgt_q, int C, int W, int P,
-    int nvg, float *__restrict__ partial, int accumulate)
+    int nvg, int nsplit, float *__restrict__ partial, int accumulate)
 {
     constexpr int NF = VT + NK - 1;          // target fragments per tile
     constexpr int CS = 16 * NK + 16;         // chip row: 16 zeros, C values, zeros
@@ -88,16 +91,21 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 256 * WPE),
     extern __shared__ uint4 lds_u4[];
     uint16_t *lds = reinterpret_cast<uint16_t *>(lds_u4);
 
-    const int pair = xcd_block(blockIdx.x, gridDim.x);
+    // the ranges of a pair in consecutive workgroups (one XCD's L2 holds the window)
+    const int wg = xcd_block(blockIdx.x, gridDim.x);
+    const int pair = wg / nsplit, ks = wg - pair * nsplit;
+    const int CR = C / nsplit, i0 = ks * CR;     // the range's chip rows [i0, i0 + CR)
     const int NB = (P - 1) >> 5;
     const int TS = packed_row_stride(NK, VT, nvg);
     const int TW = TS - 8;                   // staged columns per ring row
     const int CPR = TW / 8;                  // 16-byte chunks per ring row
     uint16_t *ring = lds;                                    // [NB][RR][TS]
     uint16_t *chip = lds + (size_t)NB * RR * TS;             // [2][CH][2][CS]
-    const uint16_t *src = tgt_q + (size_t)pair * W * W;
-    const uint16_t *csrc = chip_q + (size_t)pair * C * CS;
-    const int nchunks = C / CH;
+    // target row u + i of chip row i is row u + (i - i0) from src
+    const uint16_t *src = tgt_q + (size_t)pair * W * W + (size_t)i0 * W;
+    const uint16_t *csrc = chip_q + ((size_t)pair * C + i0) * CS;
+    const int WR = W - i0;                       // window rows from src
+    const int nchunks = CR / CH;
 
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
@@ -121,7 +129,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 256 * WPE),
         const int bb = k / ((CH + 1) * CPR), r = k - bb * (CH + 1) * CPR, o = r / CPR;
         const int q = r - o * CPR, gr = 32 * (bb + 1) + o, gc = 8 * q;
         uint4 v = make_uint4(0u, 0u, 0u, 0u);
-        if (gr < W && gc < W) v = *reinterpret_cast<const uint4 *>(src + (size_t)gr * W + gc);
+        if (gr < WR && gc < W) v = *reinterpret_cast<const uint4 *>(src + (size_t)gr * W + gc);
         *reinterpret_cast<uint4 *>(ring + ((size_t)bb * RR + o) * TS + gc) = v;
     }
     for (int k = threadIdx.x; k < CH * CCH; k += blockDim.x) {
@@ -140,9 +148,9 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 256 * WPE),
     const int rq = RPI == 2 ? (lane & 31) : lane;       // column chunk of this lane
     const int rh = RPI == 2 ? (lane >> 5) : 0;          // row of the pair
     const __amdgpu_buffer_rsrc_t trs =
-        __builtin_amdgcn_make_buffer_rsrc((void *)src, (short)0, W * W * 2, kBufferWord3);
+        __builtin_amdgcn_make_buffer_rsrc((void *)src, (short)0, WR * W * 2, kBufferWord3);
     const __amdgpu_buffer_rsrc_t crs =
-        __builtin_amdgcn_make_buffer_rsrc((void *)csrc, (short)0, C * CS * 2, kBufferWord3);
+        __builtin_amdgcn_make_buffer_rsrc((void *)csrc, (short)0, CR * CS * 2, kBufferWord3);
     // ---- the wave's fragments at chip rows (0, 1), from global memory ---------
     // tile t, fragment f, lane (m, lk): target row 32 b + t + 2 m + d, columns
     // col0 + 16 f + 8 h .. + 7
@@ -291,7 +299,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 256 * WPE),
 
     // ---- outputs: lane (n, lk) holds M-rows 4 lk + e of column n; with
     // accumulate set they are added to what the surface holds ----------------
-    float *dst = partial + (size_t)pair * P * P;
+    float *dst = partial + ((size_t)pair * nsplit + ks) * P * P;
     auto put = [&](size_t i, float x) { dst[i] = accumulate ? dst[i] + x : x; };
 #pragma unroll
     for (int v = 0; v < VT; ++v) {
@@ -367,14 +375,15 @@ hipError_t launch_correlate_packed_nk(hipStream_t s, const MixedShape &sh, const
     const int P = W - C + 1;
     const dim3 block(64 * sh.wg_ut);
     const int passes = sh.split ? 3 : 1;
-    // one workgroup per pair: at most 2^31 - 1 of them per launch
-    const int chunk = std::min(pairs, 0x7fffffff);
+    // one workgroup per pair and chip-row range: at most 2^31 - 1 of them per launch
+    const int ns = sh.nsplit;
+    const int chunk = std::min(pairs, 0x7fffffff / ns);
     for (int pass = 0; pass < passes; ++pass)
     for (int first = 0; first < pairs; first += chunk) {
         const int n = std::min(chunk, pairs - first);
         const uint16_t *cq = (pass == 2 ? chip_lo : chip_q) + (size_t)first * C * sh.cs;
         const uint16_t *tq = (pass == 1 ? tgt_lo : tgt_q) + (size_t)first * W * W;
-        float *pt = partial + (size_t)first * P * P;
+        float *pt = partial + (size_t)first * ns * P * P;
         const int acc = pass > 0;
         bool launched = false;
         auto kind = [&](auto vt, auto ch, auto wpe) {
@@ -384,13 +393,13 @@ hipError_t launch_correlate_packed_nk(hipStream_t s, const MixedShape &sh, const
                 if (launched || sh.wg_vt != VTV || sh.packed != CHV || sh.wpe != WPEV) return;
                 launched = true;
                 if (fp16)
-                    hipLaunchKernelGGL((k_correlate_packed<NK, VTV, CHV, WPEV, true>), dim3(n),
-                                       block, sh.lds_bytes, s, cq, tq, C, W, P, sh.tiles_v, pt,
-                                       acc);
+                    hipLaunchKernelGGL((k_correlate_packed<NK, VTV, CHV, WPEV, true>),
+                                       dim3(n * ns), block, sh.lds_bytes, s, cq, tq, C, W, P,
+                                       sh.tiles_v, ns, pt, acc);
                 else
-                    hipLaunchKernelGGL((k_correlate_packed<NK, VTV, CHV, WPEV, false>), dim3(n),
-                                       block, sh.lds_bytes, s, cq, tq, C, W, P, sh.tiles_v, pt,
-                                       acc);
+                    hipLaunchKernelGGL((k_correlate_packed<NK, VTV, CHV, WPEV, false>),
+                                       dim3(n * ns), block, sh.lds_bytes, s, cq, tq, C, W, P,
+                                       sh.tiles_v, ns, pt, acc);
             }
         };
         using I2 = std::integral_constant<int, 2>;

@@ -317,9 +317,14 @@ int plan_configure(ampcor_hip_plan *p, int device, size_t pairs, size_t ref_line
         // (no padded placement rows; split operands in three passes), else the
         // refined contraction streams its chip rows where it can (plain
         // operands; one partial surface per pair); the autotuner weighs every form
-        const MixedShape pc = choose_mixed_packed_shape(p->C, p->P);
+        // (AMPCOR_HIP_PACKED_SPLIT="coarse,refined" sets its chip-row ranges)
+        int pcs = 1, prs = 1;
+        if (const char *e = std::getenv("AMPCOR_HIP_PACKED_SPLIT")) std::sscanf(e, "%d,%d", &pcs, &prs);
+        MixedShape pc = choose_mixed_packed_shape(p->C, p->P, 0, 0, false, pcs);
+        if (!pc.nk) pc = choose_mixed_packed_shape(p->C, p->P);
         if (pc.nk) p->mixed_coarse = pc;
-        const MixedShape pr = choose_mixed_packed_shape(p->Cr, p->R, 0, 0, p->split);
+        MixedShape pr = choose_mixed_packed_shape(p->Cr, p->R, 0, 0, p->split, prs);
+        if (!pr.nk) pr = choose_mixed_packed_shape(p->Cr, p->R, 0, 0, p->split);
         const MixedShape st = p->split ? MixedShape{} : choose_mixed_stream_shape(p->Cr, p->R);
         if (pr.nk) p->mixed_refined = pr;
         else if (st.nk) p->mixed_refined = st;
@@ -692,6 +697,14 @@ int plan_enqueue(ampcor_hip_plan *plan, int N)
     }
     // the FP32 coarse stage keeps the separate window sums, whose FP64 order the
     // CPU twin reproduces; the other stages fuse them into the normalizer
+    // the packed contractions of small launches in more chip-row ranges (the
+    // partial buffers hold the autotuner's largest splits)
+    const MixedShape mc = p->mixed ? packed_fill_ranges(p->mixed_coarse, p->C, (int)N,
+                                                        kTuneMaxSplitCoarse)
+                                   : MixedShape{};
+    const MixedShape mr = p->mixed ? packed_fill_ranges(p->mixed_refined, p->Cr, (int)N,
+                                                        std::min(kTuneMaxSplitRefined, 4))
+                                   : MixedShape{};
     const bool fuse_c = p->mixed && normalize_fuses(p->W, p->C);
     const bool fuse_r = normalize_fuses(p->Wr, p->Cr);
     if (!fuse_c)
@@ -705,17 +718,17 @@ int plan_enqueue(ampcor_hip_plan *plan, int N)
                                     p->fp8_coarse.nsplit, nullptr, p->mstats.ptr, p->gamma.ptr),
              "normalize");
     } else if (p->mixed) {
-        STEP(launch_correlate_mixed(s, p->mixed_coarse, p->chip_q.ptr, p->tgt_q.ptr, N, p->C, p->W,
-                                    fp16, p->partial.ptr),
+        STEP(launch_correlate_mixed(s, mc, p->chip_q.ptr, p->tgt_q.ptr, N, p->C, p->W, fp16,
+                                    p->partial.ptr),
              "correlate");
         if (fuse_c)
             STEP(launch_normalize_fused(s, fp16 ? kFuseFP16 : kFuseBF16, p->tgt_q.ptr, N, p->W,
-                                        p->C, p->partial.ptr, p->mixed_coarse.nsplit, nullptr,
-                                        p->mstats.ptr, p->gamma.ptr),
+                                        p->C, p->partial.ptr, mc.nsplit, nullptr, p->mstats.ptr,
+                                        p->gamma.ptr),
                  "normalize");
         else
-            STEP(launch_normalize_mixed(s, p->partial.ptr, p->mixed_coarse.nsplit, p->sums.ptr,
-                                        p->mstats.ptr, N, p->C, p->W, p->gamma.ptr),
+            STEP(launch_normalize_mixed(s, p->partial.ptr, mc.nsplit, p->sums.ptr, p->mstats.ptr, N,
+                                        p->C, p->W, p->gamma.ptr),
                  "normalize");
     } else {
         // the exact-order contraction normalizes in its epilogue
@@ -773,19 +786,18 @@ int plan_enqueue(ampcor_hip_plan *plan, int N)
         STEP(launch_window_sums(s, p->amp_tgt_r.ptr, N, p->Wr, p->Cr, p->sums_r.ptr),
              "window_sums_refined");
     if (p->mixed) {
-        STEP(launch_correlate_mixed(s, p->mixed_refined, p->chip_q_r.ptr, p->tgt_q_r.ptr, N, p->Cr,
-                                    p->Wr, fp16, p->partial_r.ptr, p->chip_lo_r.ptr,
-                                    p->tgt_lo_r.ptr),
+        STEP(launch_correlate_mixed(s, mr, p->chip_q_r.ptr, p->tgt_q_r.ptr, N, p->Cr, p->Wr, fp16,
+                                    p->partial_r.ptr, p->chip_lo_r.ptr, p->tgt_lo_r.ptr),
              "correlate_refined");
         if (fuse_r)
             STEP(launch_normalize_fused(s, quant_sums_r ? (fp16 ? kFuseFP16 : kFuseBF16) : kFuseMixed,
                                         quant_sums_r ? (const void *)p->tgt_q_r.ptr
                                                      : (const void *)p->amp_tgt_r.ptr,
-                                        N, p->Wr, p->Cr, p->partial_r.ptr, p->mixed_refined.nsplit,
-                                        nullptr, p->mstats_r.ptr, p->gamma_r.ptr),
+                                        N, p->Wr, p->Cr, p->partial_r.ptr, mr.nsplit, nullptr,
+                                        p->mstats_r.ptr, p->gamma_r.ptr),
                  "normalize_refined");
         else
-            STEP(launch_normalize_mixed(s, p->partial_r.ptr, p->mixed_refined.nsplit, p->sums_r.ptr,
+            STEP(launch_normalize_mixed(s, p->partial_r.ptr, mr.nsplit, p->sums_r.ptr,
                                         p->mstats_r.ptr, N, p->Cr, p->Wr, p->gamma_r.ptr),
                  "normalize_refined");
     } else {
@@ -1025,7 +1037,9 @@ int ampcor_hip_stage_contract(ampcor_hip_plan *plan, int stage, int form, const 
     case 1: sh = choose_mixed_shape(C, P); break;
     case 2: sh = choose_mixed_stream_shape(C, P); break;
     case 3: sh = choose_mixed_packed_shape(C, P); break;
-    default: return fail(AMPCOR_HIP_EINVAL, "form must be 0 .. 3");
+    case 4: sh = choose_mixed_packed_shape(C, P, 0, 0, false, 2); break;
+    case 5: sh = choose_mixed_packed_shape(C, P, 0, 0, false, 4); break;
+    default: return fail(AMPCOR_HIP_EINVAL, "form must be 0 .. 5");
     }
     if (!sh.nk) return fail(AMPCOR_HIP_EINVAL, "the form does not apply to this geometry");
     // the plan's buffers hold the partials of at most its own splits
@@ -1285,10 +1299,11 @@ int ampcor_hip_plan_autotune(ampcor_hip_plan *plan, int reps)
                     continue;
                 consider(sh);
             }
-            // the packed form (P = 32 NB + 1) at both register budgets
-            for (int wpe : {3, 2}) {
-                if (rc) break;
-                MixedShape sh = choose_mixed_packed_shape(st.C, st.P, 0, wpe, st.split);
+            // the packed form (P = 32 NB + 1) at both register budgets, whole
+            // or in chip-row ranges (more workgroups where a pair has few waves)
+            for (int wpe : {3, 2})
+            for (int ns = 1; ns <= std::min(st.maxsplit, 4) && !rc; ns *= 2) {
+                MixedShape sh = choose_mixed_packed_shape(st.C, st.P, 0, wpe, st.split, ns);
                 if (sh.nk) consider(sh);
             }
             *st.shape = pick;

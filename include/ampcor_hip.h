@@ -162,7 +162,8 @@ int ampcor_hip_stage_upsample(ampcor_hip_plan *plan, int dtype, const void *tile
  * sum_ij window[u + i][v + j] chip[i][j] [count][p][p] (p = w - c + 1) as the
  * form's tiling computes it: form 0 the plan's current tiling, 1 the 16 x 16
  * tiles (chip-row ranges summed in order), 2 the streaming form, 3 the packed
- * form (p = 32 n + 1); AMPCOR_HIP_EINVAL where the form does not apply.
+ * form (p = 32 n + 1), 4 and 5 the packed form in two and four chip-row ranges
+ * (summed in order); AMPCOR_HIP_EINVAL where the form does not apply.
  * Overwrites the last adjust's intermediate buffers. */
 int ampcor_hip_stage_contract(ampcor_hip_plan *plan, int stage, int form, const uint16_t *chip,
                               int c, const uint16_t *window, int w, size_t count,

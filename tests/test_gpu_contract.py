@@ -12,7 +12,8 @@ row) or a missing edge column fails bitwise.
 Forms: 1 = 16 x 16 tiles of consecutive placements (chip-row ranges summed in
 order), 2 = the streaming form, 3 = the packed form (surfaces of 32 n + 1
 placements per axis; chips of 16 .. 160 samples and of 256, the refined chip
-of the 128-sample chips).  The geometries cover the scene's coarse (chip 64,
+of the 128-sample chips), 4 / 5 = the packed form in two / four chip-row
+ranges (each range's window rows start at its first chip row).  The geometries cover the scene's coarse (chip 64,
 window 128) and refined (128 / 160) stages, the serving stream's chips 32 and
 128 with margins 16, 32 and 64, and the packed form's multi-wave layouts (two
 and four u-blocks, two v-groups).
@@ -23,7 +24,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-FORMS = {1: "tiles", 2: "streaming", 3: "packed"}
+FORMS = {1: "tiles", 2: "streaming", 3: "packed", 4: "packed, 2 ranges", 5: "packed, 4 ranges"}
 
 
 def bits(x, precision):
@@ -76,8 +77,10 @@ def test_tilings_exact_on_integers(lib, chip, window, rm, precision):
                                          bits(tq, precision).tobytes(), w, n)
             except Exception as e:  # the form does not apply to this geometry
                 assert "does not apply" in str(e) or "exceeds" in str(e), str(e)
-                if form == 3:   # built for chips of 16 .. 160 samples and 256
-                    assert (p - 1) % 32 or c % 16 or (c > 160 and c != 256), (stage, p, c)
+                if form >= 3:   # built for chips of 16 .. 160 samples and 256
+                    ranges = 1 << (form - 3)
+                    assert ((p - 1) % 32 or c % 16 or (c > 160 and c != 256)
+                            or (c // ranges) % 4), (stage, p, c, ranges)
                 continue
             got = np.frombuffer(raw, np.float32).reshape(n, p, p)
             bad = np.argwhere(got.astype(np.int64) != want)

@@ -546,6 +546,7 @@ constexpr int kFuseMaxSplit = 4; // numerator partials loaded a block ahead
 constexpr int kFuseScan = 6;     // LDS entries in flight per step of the row scans
 
 constexpr int kFuseInit = 4;     // rows in flight per thread in the first box sums
+constexpr size_t kLdsPerCu = 160 * 1024;   // LDS bytes of a gfx950 CU
 
 // LDS row stride (doubles) of the normalizer's sums: at least W + 1 and 16 mod
 // 32, so that two consecutive rows sit 32 banks apart and the half-wave
@@ -628,12 +629,16 @@ template <int FORM, int COLS, int JO, bool LAZY>
 __global__ __launch_bounds__(kFuseThreads, COLS == 1 && JO < 9 ? (LAZY ? 3 : 4) : 2) void k_normalize_fused(
     const void *__restrict__ target, int W, int C, const float *__restrict__ partial, int nsplit,
     const double2 *__restrict__ stats, const MixedStats *__restrict__ mstats,
-    float *__restrict__ gamma)
+    float *__restrict__ gamma, int xr)
 {
-    extern __shared__ double fl[];   // [2][kFuseRows][W + 1]
+    extern __shared__ double fl[];   // [2][kFuseRows + xr][ws]
     const int pair = blockIdx.x, t = threadIdx.x;
     const int P = W - C + 1, ws = fuse_row_stride(W);
-    double *p1 = fl, *p2 = fl + kFuseRows * ws;
+    // the lone-row path is built into the lazy-offset kernels only: in the
+    // others its extra scan and outputs cost registers (spills at the
+    // four-workgroup 128-VGPR cap, a workgroup per CU in the wide ones)
+    if constexpr (!LAZY) xr = 0;
+    double *p1 = fl, *p2 = fl + (kFuseRows + xr) * ws;
     const size_t tbase = (size_t)pair * W * W;
     auto tval = [&](int row, int x) { return fuse_load<FORM>(target, tbase + (size_t)row * W + x); };
     const double n = (double)C * C, inv_n = 1.0 / n;
@@ -777,10 +782,15 @@ __global__ __launch_bounds__(kFuseThreads, COLS == 1 && JO < 9 ? (LAZY ? 3 : 4) 
     const int x0 = min(W, sl * cw), x1 = min(W, x0 + cw);
     // (lazy form) chunk of column x = (x * inv_cw) >> 20, exact for x < 512 and cw <= 33
     const int inv_cw = (1 << 20) / cw + 1;
-    __shared__ double off1[LAZY ? kFuseRows : 1][16], off2[LAZY ? kFuseRows : 1][16];
+    __shared__ double off1[LAZY ? kFuseRows + 1 : 1][16], off2[LAZY ? kFuseRows + 1 : 1][16];
 
     for (int u0 = 0; u0 < P; u0 += kFuseRows) {
         const int rows = min(kFuseRows, P - u0);
+        // xr: a lone last placement row joins the block before it as its row
+        // kFuseRows (the LDS holds one row more); the threads of row 0 take
+        // its scan and outputs after their own, which saves a whole block's
+        // passes and barriers (every P = 16 n + 1)
+        const bool lone = LAZY && xr && P - u0 == kFuseRows + 1;
         // the numerators of this thread's outputs (row r, columns sl + 16 j):
         // the partials fetched a block ahead, summed in split order, then the
         // next block's fetched
@@ -824,60 +834,82 @@ __global__ __launch_bounds__(kFuseThreads, COLS == 1 && JO < 9 ? (LAZY ? 3 : 4) 
         }
         // the next block's rows, in flight during this block's scans and outputs
         fetch(u0 + kFuseRows, ra, rb);
-        lds_barrier();
-        // row prefix sums: chunk prefix, then the 16-lane scan of chunk totals
-        // (kFuseScan entries in flight per step)
-        double run1 = 0.0, run2 = 0.0;
-        if (r < rows)
-            for (int xb = x0; xb < x1; xb += kFuseScan) {
-                double e1[kFuseScan], e2[kFuseScan];
+        if (lone) {
+            // the lone row u0 + kFuseRows: its entering and leaving rows are
+            // the first ones just fetched (the slide of the block it would be)
 #pragma unroll
-                for (int i = 0; i < kFuseScan; ++i) {
-                    e1[i] = xb + i < x1 ? p1[r * ws + xb + i] : 0.0;
-                    e2[i] = xb + i < x1 ? p2[r * ws + xb + i] : 0.0;
-                }
-#pragma unroll
-                for (int i = 0; i < kFuseScan; ++i) {
-                    run1 += e1[i];
-                    run2 += e2[i];
-                    if (xb + i < x1) {
-                        p1[r * ws + xb + i] = run1;
-                        p2[r * ws + xb + i] = run2;
-                    }
-                }
-            }
-        double o1 = run1, o2 = run2;
-#pragma unroll
-        for (int d = 1; d < 16; d <<= 1) {
-            const double y1 = __shfl_up(o1, d, 16), y2 = __shfl_up(o2, d, 16);
-            if (sl >= d) {
-                o1 += y1;
-                o2 += y2;
+            for (int q = 0; q < COLS; ++q) {
+                const int x = t + kFuseThreads * q;
+                if (x >= W) continue;
+                const double da = ra[q][0], db = rb[q][0];
+                v1[q] = (v1[q] + da) - db;
+                v2[q] = (v2[q] + da * da) - db * db;
+                p1[kFuseRows * ws + x] = v1[q];
+                p2[kFuseRows * ws + x] = v2[q];
             }
         }
-        o1 -= run1;   // exclusive: the totals of the chunks to the left
-        o2 -= run2;
-        if (LAZY && r < rows) {
-            off1[r][sl] = o1;
-            off2[r][sl] = o2;
-        }
-        if (!LAZY && r < rows && sl > 0)
-            for (int xb = x0; xb < x1; xb += kFuseScan) {
-                double e1[kFuseScan], e2[kFuseScan];
-#pragma unroll
-                for (int i = 0; i < kFuseScan; ++i) {
-                    e1[i] = xb + i < x1 ? p1[r * ws + xb + i] : 0.0;
-                    e2[i] = xb + i < x1 ? p2[r * ws + xb + i] : 0.0;
-                }
-#pragma unroll
-                for (int i = 0; i < kFuseScan; ++i)
-                    if (xb + i < x1) {
-                        p1[r * ws + xb + i] = e1[i] + o1;
-                        p2[r * ws + xb + i] = e2[i] + o2;
-                    }
-            }
         lds_barrier();
-        if (r < rows) {
+        // row prefix sums of row rr: chunk prefix, then the 16-lane scan of
+        // chunk totals (kFuseScan entries in flight per step); every lane of
+        // the 16-lane group takes part in the scan, live or not
+        auto scan_row = [&](int rr, bool live) {
+            double run1 = 0.0, run2 = 0.0;
+            if (live)
+                for (int xb = x0; xb < x1; xb += kFuseScan) {
+                    double e1[kFuseScan], e2[kFuseScan];
+#pragma unroll
+                    for (int i = 0; i < kFuseScan; ++i) {
+                        e1[i] = xb + i < x1 ? p1[rr * ws + xb + i] : 0.0;
+                        e2[i] = xb + i < x1 ? p2[rr * ws + xb + i] : 0.0;
+                    }
+#pragma unroll
+                    for (int i = 0; i < kFuseScan; ++i) {
+                        run1 += e1[i];
+                        run2 += e2[i];
+                        if (xb + i < x1) {
+                            p1[rr * ws + xb + i] = run1;
+                            p2[rr * ws + xb + i] = run2;
+                        }
+                    }
+                }
+            double o1 = run1, o2 = run2;
+#pragma unroll
+            for (int d = 1; d < 16; d <<= 1) {
+                const double y1 = __shfl_up(o1, d, 16), y2 = __shfl_up(o2, d, 16);
+                if (sl >= d) {
+                    o1 += y1;
+                    o2 += y2;
+                }
+            }
+            o1 -= run1;   // exclusive: the totals of the chunks to the left
+            o2 -= run2;
+            if (LAZY && live) {
+                off1[rr][sl] = o1;
+                off2[rr][sl] = o2;
+            }
+            if (!LAZY && live && sl > 0)
+                for (int xb = x0; xb < x1; xb += kFuseScan) {
+                    double e1[kFuseScan], e2[kFuseScan];
+#pragma unroll
+                    for (int i = 0; i < kFuseScan; ++i) {
+                        e1[i] = xb + i < x1 ? p1[rr * ws + xb + i] : 0.0;
+                        e2[i] = xb + i < x1 ? p2[rr * ws + xb + i] : 0.0;
+                    }
+#pragma unroll
+                    for (int i = 0; i < kFuseScan; ++i)
+                        if (xb + i < x1) {
+                            p1[rr * ws + xb + i] = e1[i] + o1;
+                            p2[rr * ws + xb + i] = e2[i] + o2;
+                        }
+                }
+        };
+        scan_row(r, r < rows);
+        if (lone && r == 0) scan_row(kFuseRows, true);
+        lds_barrier();
+        // the outputs of row rr of the block from its numerators nm
+        auto emit_row = [&](int rr, const float(&nm)[JO]) {
+            const int r = rr;   // the LDS row
+            const float *num = nm;
             const int u = u0 + r;
             for (int j = 0; j < JO; ++j) {
                 const int v = sl + 16 * j;
@@ -913,8 +945,34 @@ __global__ __launch_bounds__(kFuseThreads, COLS == 1 && JO < 9 ? (LAZY ? 3 : 4) 
                 if (var > 0.0 && chip2 > 0.0) g = (float)(num_c * rsqrt_fp64(chip2 * var));
                 gamma[(size_t)pair * P * P + (size_t)u * P + v] = g;
             }
+        };
+        if (r < rows) emit_row(r, num);
+        if (lone && r == 0) {
+            // the lone row's numerators, summed in split order: fetched a block
+            // ahead (row 0's next block is the lone row), or read now
+            float nl[JO];
+            if (ahead) {
+#pragma unroll
+                for (int j = 0; j < JO; ++j) {
+                    nl[j] = raw[0][j];
+#pragma unroll
+                    for (int k = 1; k < kFuseMaxSplit; ++k)
+                        if (k < nsplit) nl[j] += raw[k][j];
+                }
+            } else {
+                const float *pp =
+                    partial + (size_t)pair * nsplit * P * P + (size_t)(u0 + kFuseRows) * P + sl;
+#pragma unroll
+                for (int j = 0; j < JO; ++j) nl[j] = 0.0f;
+                for (int k = 0; k < nsplit; ++k)
+#pragma unroll
+                    for (int j = 0; j < JO; ++j)
+                        if (sl + 16 * j < P) nl[j] += pp[(size_t)k * P * P + 16 * j];
+            }
+            emit_row(kFuseRows, nl);
         }
         lds_barrier();
+        if (lone) break;
     }
 }
 
@@ -3521,16 +3579,27 @@ hipError_t launch_normalize_fused(hipStream_t s, int form, const void *target, i
 {
     if (!normalize_fuses(W, C) || (form == kFuseFP32 ? !stats : !mstats))
         return hipErrorInvalidValue;
-    const size_t lds = 2 * (size_t)kFuseRows * fuse_row_stride(W) * sizeof(double);
     const bool one = W <= kFuseThreads;   // one target column per thread
     // outputs per thread and block: 3, 5 or 9 (P <= 48, 80, 144)
     const int P = W - C + 1, jo = P <= 48 ? 3 : P <= 80 ? 5 : 9;
-#define AMPCOR_FUSE_K(F, CO, J, LZ)                                                                 \
-    hipLaunchKernelGGL((k_normalize_fused<F, CO, J, LZ>), dim3(pairs), dim3(kFuseThreads), lds, s,  \
-                       target, W, C, partial, nsplit, stats, mstats, gamma)
     // wide windows with few outputs per thread: the lazy offsets (measured faster
     // for the 160-sample refined window; slower where four workgroups fit a CU)
     const bool lazy = one && jo == 3 && W >= 144;
+    // a lone last placement row (P = 16 n + 1) joins the block before it in
+    // the lazy-offset kernels, where the LDS of one more row leaves as many
+    // workgroups per CU (their register bound: three)
+    auto lds_of = [&](int rows) { return 2 * (size_t)rows * fuse_row_stride(W) * sizeof(double); };
+    auto per_cu = [&](int rows) {
+        const size_t fixed = lazy ? 2 * (size_t)(kFuseRows + 1) * 16 * sizeof(double) : 0;
+        return (int)(kLdsPerCu / (lds_of(rows) + fixed));
+    };
+    const int cap = one && jo < 9 ? (lazy ? 3 : 4) : 2;
+    const int xr = lazy && P > kFuseRows && (P - 1) % kFuseRows == 0 &&
+                   per_cu(kFuseRows + 1) >= std::min(cap, per_cu(kFuseRows));
+    const size_t lds = lds_of(kFuseRows + xr);
+#define AMPCOR_FUSE_K(F, CO, J, LZ)                                                                 \
+    hipLaunchKernelGGL((k_normalize_fused<F, CO, J, LZ>), dim3(pairs), dim3(kFuseThreads), lds, s,  \
+                       target, W, C, partial, nsplit, stats, mstats, gamma, xr)
 #define AMPCOR_FUSE_J(F, CO, J)                                                                     \
     do {                                                                                            \
         if (CO == 1 && J == 3 && lazy) AMPCOR_FUSE_K(F, CO, J, (CO == 1 && J == 3));                \

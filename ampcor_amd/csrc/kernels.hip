@@ -1030,11 +1030,11 @@ __device__ inline void coarse_peak_finish(int pair, int lane, int P, int W, int 
     }
 }
 
-// KEEP > 0: the surface has at most KEEP cells per lane and stays in registers
-// between the argmax and the SNR sums (one read of it); KEEP = 0 reads it twice.
-// Both visit each lane's cells in the same increasing order, so they give the
-// same bits.  near (optional): the number of cells within tau of the peak (the
-// peak included), for the FP8 peak guard.
+// The surface has at most KEEP cells per lane and stays in registers between
+// the argmax and the SNR sums (one read of it); each lane visits its cells in
+// increasing order, so its ties resolve to the first.  near (optional): the
+// number of cells within tau of the peak (the peak included), for the FP8
+// peak guard.
 template <int KEEP>
 __global__ __launch_bounds__(256) void k_coarse_peak(const float *__restrict__ gamma, int pairs,
                                                      int P, int W, int E, int rm, int exclude,
@@ -1049,7 +1049,8 @@ __global__ __launch_bounds__(256) void k_coarse_peak(const float *__restrict__ g
     if (pair >= pairs) return;
     const int cells = P * P;
     const float *g = gamma + (size_t)pair * cells;
-    if constexpr (KEEP > 0) {
+    static_assert(KEEP > 0, "surfaces of more cells take k_coarse_peak_wide");
+    {
         // every load in flight at once; NaN past the last cell never wins and is
         // never summed
         float x[KEEP];
@@ -1094,53 +1095,129 @@ __global__ __launch_bounds__(256) void k_coarse_peak(const float *__restrict__ g
                            sum, count, ties, loc, window, peak, snr, near);
         return;
     }
-    // kPeakBatch loads in flight per lane; each lane visits its cells in
-    // increasing order, so its ties resolve to the first
-    constexpr int kPeakBatch = 8;
+}
+
+// Surfaces above KEEP x 64 cells: one workgroup (four waves) per pair.  Each
+// thread visits cells t, t + 256, ... in increasing order (its ties resolve
+// to the first, and the cell coordinates step without divisions); the
+// waves' maxima and SNR sums meet in LDS.  KEEP > 0: the thread's cells (at
+// most KEEP) stay in registers between the two passes; KEEP = 0 reads them twice.
+template <int KEEP>
+__global__ __launch_bounds__(256) void k_coarse_peak_wide(const float *__restrict__ gamma, int P,
+                                                          int W, int E, int rm, int exclude,
+                                                          int2 *__restrict__ loc,
+                                                          int2 *__restrict__ window,
+                                                          float *__restrict__ peak,
+                                                          float *__restrict__ snr, float tau,
+                                                          int *__restrict__ near)
+{
+    constexpr int kT = 256, kB = 8;   // threads; loads in flight per thread
+    __shared__ float s_best[kT / kWave];
+    __shared__ int s_idx[kT / kWave], s_count[kT / kWave], s_ties[kT / kWave];
+    __shared__ double s_sum[kT / kWave];
+    const int t = threadIdx.x, lane = t & (kWave - 1), wv = t >> 6;
+    const int pair = blockIdx.x;
+    const int cells = P * P;
+    const float *g = gamma + (size_t)pair * cells;
     float best = -INFINITY;
     int bidx = 0x7fffffff;
-    for (int k0 = lane; k0 < cells; k0 += kPeakBatch * kWave) {
-        float x[kPeakBatch];
+    // (KEEP > 0) every load in flight at once; NaN past the last cell never
+    // wins and is never summed
+    float xk[KEEP > 0 ? KEEP : 1];
+    if constexpr (KEEP > 0) {
 #pragma unroll
-        for (int j = 0; j < kPeakBatch; ++j) {
-            const int k = k0 + j * kWave;
-            x[j] = k < cells ? g[k] : -INFINITY;
+        for (int j = 0; j < KEEP; ++j) {
+            const int k = t + j * kT;
+            xk[j] = k < cells ? g[k] : __builtin_nanf("");
         }
 #pragma unroll
-        for (int j = 0; j < kPeakBatch; ++j)
-            if (x[j] > best) {
-                best = x[j];
-                bidx = k0 + j * kWave;
+        for (int j = 0; j < KEEP; ++j)
+            if (xk[j] > best) {
+                best = xk[j];
+                bidx = t + j * kT;
             }
+    } else {
+        for (int k0 = t; k0 < cells; k0 += kB * kT) {
+            float x[kB];
+#pragma unroll
+            for (int j = 0; j < kB; ++j) {
+                const int k = k0 + j * kT;
+                x[j] = k < cells ? g[k] : -INFINITY;
+            }
+#pragma unroll
+            for (int j = 0; j < kB; ++j)
+                if (x[j] > best) {
+                    best = x[j];
+                    bidx = k0 + j * kT;
+                }
+        }
     }
     wave_argmax(best, bidx);
+    if (lane == 0) {
+        s_best[wv] = best;
+        s_idx[wv] = bidx;
+    }
+    __syncthreads();
+    best = s_best[0];
+    bidx = s_idx[0];
+#pragma unroll
+    for (int w = 1; w < kT / kWave; ++w) better(best, bidx, s_best[w], s_idx[w]);
     const bool none = (bidx == 0x7fffffff);
     const int row = none ? 0 : bidx / P;
     const int col = none ? 0 : bidx - row * P;
-    const float pk = none ? __builtin_nanf("") : best;
 
     double sum = 0.0;
     int count = 0, ties = 0;
     const float floor_ = best - tau;
-    for (int k0 = lane; k0 < cells; k0 += kPeakBatch * kWave) {
-        float x[kPeakBatch];
-#pragma unroll
-        for (int j = 0; j < kPeakBatch; ++j) {
-            const int k = k0 + j * kWave;
-            x[j] = k < cells ? g[k] : __builtin_nanf("");
-        }
-#pragma unroll
-        for (int j = 0; j < kPeakBatch; ++j) {
-            const int k = k0 + j * kWave;
-            ties += x[j] >= floor_;
-            const int u = k / P, v = k - u * P;
-            if (x[j] != x[j] || (abs(u - row) <= exclude && abs(v - col) <= exclude)) continue;
-            sum += (double)x[j] * (double)x[j];
+    const int du = kT / P, dv = kT - du * P;
+    int u = t / P, v = t - u * P;
+    // cell x at (u, v); then (u, v) of the thread's next cell
+    auto visit = [&](float x) {
+        ties += x >= floor_;
+        if (x == x && !(abs(u - row) <= exclude && abs(v - col) <= exclude)) {
+            sum += (double)x * (double)x;
             ++count;
         }
+        u += du;
+        v += dv;
+        if (v >= P) {
+            v -= P;
+            ++u;
+        }
+    };
+    if constexpr (KEEP > 0) {
+#pragma unroll
+        for (int j = 0; j < KEEP; ++j) visit(xk[j]);
+    } else {
+        for (int k0 = t; k0 < cells; k0 += kB * kT) {
+            float x[kB];
+#pragma unroll
+            for (int j = 0; j < kB; ++j) {
+                const int k = k0 + j * kT;
+                x[j] = k < cells ? g[k] : __builtin_nanf("");
+            }
+#pragma unroll
+            for (int j = 0; j < kB; ++j) visit(x[j]);
+        }
     }
-    coarse_peak_finish(pair, lane, P, W, E, rm, none, row, col, pk, sum, count, ties, loc, window,
-                       peak, snr, near);
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) {
+        sum += __shfl_xor(sum, d, kWave);
+        count += __shfl_xor(count, d, kWave);
+        ties += __shfl_xor(ties, d, kWave);
+    }
+    if (lane == 0) {
+        s_sum[wv] = sum;
+        s_count[wv] = count;
+        s_ties[wv] = ties;
+    }
+    __syncthreads();
+    if (wv != 0) return;
+    // wave 0 finishes: lanes 0 .. 3 hold the waves' sums, the others nothing
+    const bool mine = lane < kT / kWave;
+    coarse_peak_finish(pair, lane, P, W, E, rm, none, row, col, none ? __builtin_nanf("") : best,
+                       mine ? s_sum[lane] : 0.0, mine ? s_count[lane] : 0,
+                       mine ? s_ties[lane] : 0, loc, window, peak, snr, near);
 }
 
 // ---------------------------------------------------------------------------
@@ -3635,14 +3712,19 @@ hipError_t launch_coarse_peak(hipStream_t s, const float *gamma, int pairs, int 
                               int refine_margin, int exclude, int2 *loc, int2 *window,
                               float *peak, float *snr, float tau, int *near)
 {
-    constexpr int kKeep = 72;   // surfaces up to 67 x 67 stay in registers
+    // surfaces up to 67 x 67 stay in one wave's registers, up to 135 x 135 in
+    // one workgroup's
+    constexpr int kKeep = 72;
     if (P * P <= kKeep * kWave)
         hipLaunchKernelGGL(k_coarse_peak<kKeep>, dim3((pairs + 3) / 4), dim3(256), 0, s, gamma,
                            pairs, P, W, E, refine_margin, exclude, loc, window, peak, snr, tau,
                            near);
+    else if (P * P <= kKeep * 256)   // one workgroup a pair, the surface in registers
+        hipLaunchKernelGGL(k_coarse_peak_wide<kKeep>, dim3(pairs), dim3(256), 0, s, gamma, P, W,
+                           E, refine_margin, exclude, loc, window, peak, snr, tau, near);
     else
-        hipLaunchKernelGGL(k_coarse_peak<0>, dim3((pairs + 3) / 4), dim3(256), 0, s, gamma, pairs,
-                           P, W, E, refine_margin, exclude, loc, window, peak, snr, tau, near);
+        hipLaunchKernelGGL(k_coarse_peak_wide<0>, dim3(pairs), dim3(256), 0, s, gamma, P, W, E,
+                           refine_margin, exclude, loc, window, peak, snr, tau, near);
     return hipGetLastError();
 }
 

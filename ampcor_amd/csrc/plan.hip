@@ -1282,6 +1282,10 @@ int ampcor_hip_plan_autotune(ampcor_hip_plan *plan, int reps)
                     return;
                 if (ms < best) best = ms, pick = sh;
             };
+            // as for the FP32 form: the current tiling first, and a candidate
+            // replaces it only when it is more than 5 % faster
+            consider(pick);
+            best *= 0.95f;
             // v-tiles per workgroup: the widest, and 3 (which divides some tile
             // counts more evenly, e.g. 9 = 3 x 3 rather than 5 + 5)
             for (int vt : {0, 3})
@@ -1313,6 +1317,13 @@ int ampcor_hip_plan_autotune(ampcor_hip_plan *plan, int reps)
             Fp8Shape pick = plan->fp8_coarse;
             const auto *cq8 = reinterpret_cast<const uint8_t *>(plan->chip_q.ptr);
             const auto *tq8 = reinterpret_cast<const uint8_t *>(plan->tgt_q.ptr);
+            // the current tiling first; a candidate must be more than 5 % faster
+            rc = timeit([&] {
+                hipError_t e = launch_correlate_fp8(s, pick, cq8, tq8, N, plan->C, plan->W,
+                                                   plan->partial.ptr);
+                return e != hipSuccess ? e : normalize_coarse(pick.nsplit);
+            }, &best);
+            best *= 0.95f;
             for (int vt : {0, 3})
             for (int ns = 1; ns <= kTuneMaxSplitCoarse && !rc; ns *= 2) {
                 Fp8Shape sh = choose_fp8_shape(plan->C, plan->P, ns, vt);

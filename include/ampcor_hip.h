@@ -257,6 +257,11 @@ int ampcor_hip_adjust_noalloc(int device, void *stream, int dtype, const void *r
  * pairs on slots_per_lane concurrent plans per geometry.  With use_graphs each
  * batch replays a HIP graph captured for its size bucket.  The first batch of
  * a geometry runs the plan autotuner.  Rasters must outlive their jobs.
+ * The budget bounds the server's own device memory (each lane's plans:
+ * slots_per_lane x batch_pairs x the geometry's bytes per pair); rasters are
+ * the caller's, resident before their jobs are submitted, so a caller gives
+ * the server the HBM its rasters leave.  A lane that does not fit evicts idle
+ * lanes (least recently used first) or waits for running ones to finish.
  */
 typedef struct ampcor_hip_server ampcor_hip_server;
 
